@@ -1,0 +1,771 @@
+// gps_api.cpp — host orchestration and the C ABI of include/gps.h.
+//
+// A gps_pass owns every per-frame device buffer; gps_pass_render() only
+// enqueues kernels on the caller's stream (no host synchronization, no
+// allocation unless the supersampling factor changed), replacing
+// GaussianPointSplatting::render (gaussian_point_splatting.cu:599-737) whose
+// workload distributor read three scalars back to the host per pass
+// (workload_distributor.cu:70-107).  Here the point count stays on the device
+// and the splat kernel is persistent over a device-side chunk count.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gps.h"
+#include "gps_fb.h"
+#include "gps_hostmath.h"
+#include "gps_internal.h"
+#include "gps_kernels.h"
+
+using namespace gps;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+gps_status hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? GPS_ERR_OUT_OF_MEMORY : GPS_ERR_HIP;
+}
+
+#define GPS_HIP(call)                                       \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+    } while (0)
+
+template <typename T> gps_status dalloc(T** p, uint64_t count, bool zero) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    GPS_HIP(hipMalloc((void**)p, count * sizeof(T)));
+    if (zero) GPS_HIP(hipMemset(*p, 0, count * sizeof(T)));
+    return GPS_OK;
+}
+template <typename T> void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+uint16_t gps_float_to_half_rne(float f);
+void gps_encode_sh_record(const float* sh, int ncoef, uint8_t out[64]);
+
+namespace {
+
+
+}  // namespace
+
+struct gps_scene {
+    DevScene view{};
+    void* planes = nullptr;  // FP32 planes or HALF_FP8 means
+    void* geo = nullptr;
+    void* shrec = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct gps_pass {
+    gps_scene* scene = nullptr;
+    gps_render_settings settings{};
+    uint32_t N = 0;
+    uint32_t budget = 0;
+    // distributor
+    uint32_t* weights = nullptr;
+    uint32_t* maskA = nullptr;
+    uint32_t* maskB = nullptr;
+    SplatRecord* recs = nullptr;
+    uint32_t ntiles = 0;
+    uint32_t *tile_sum = nullptr, *tile_cnt = nullptr, *tile_off = nullptr, *tile_voff = nullptr;
+    uint32_t *cidx = nullptr, *coff = nullptr, *chunk_start = nullptr;
+    FrameCounters* ctr = nullptr;
+    uint32_t* dbg_indices = nullptr;
+    uint64_t dbg_indices_cap = 0;
+    // framebuffers
+    uint64_t* fbA = nullptr;
+    uint64_t* fbB = nullptr;
+    uint64_t fb_words = 0;
+    int fb_ss = -1;
+    uint64_t* cur_fb = nullptr;
+    // occlusion
+    HiZView hiz{};
+    uint32_t hiz_size = 0;
+    CullNode* nodes = nullptr;
+    uint32_t* cull_mask = nullptr;
+    CullView cull{};
+    bool hier_available = false;
+    // splat grid
+    uint32_t splat_grid = 0;
+    // last frame
+    int last_occl = 0;
+    int last_K = 1;
+    int last_spp = 1;
+};
+
+namespace {
+
+uint64_t mip_total(HiZView& h, int w, int hgt) {
+    h.width = w;
+    h.height = hgt;
+    int maxd = std::max(w, hgt), e = 0;
+    while ((1 << e) < maxd) ++e;
+    h.levels = e + 1;
+    uint64_t total = 0;
+    for (int l = 0; l < h.levels; ++l) {
+        const int fe = h.levels - 1;
+        h.lw[l] = std::max(1, (w + (1 << (fe - l)) - 1) >> (fe - l));
+        h.lh[l] = std::max(1, (hgt + (1 << (fe - l)) - 1) >> (fe - l));
+        h.off[l] = (uint32_t)total;
+        total += (uint64_t)h.lw[l] * h.lh[l];
+    }
+    return total;
+}
+
+gps_status ensure_fb(gps_pass* p, int ss) {
+    const int uw = p->settings.width * ss, uh = p->settings.height * ss;
+    const uint64_t words = fb_words(uw, uh, p->settings.render_pass_count);
+    if (words == p->fb_words && p->fbA) return GPS_OK;
+    if (words >= (1ull << 31)) {
+        g_last_error = "framebuffer exceeds 2^31 words";
+        return GPS_ERR_UNSUPPORTED;
+    }
+    dfree(p->fbA);
+    dfree(p->fbB);
+    gps_status s = dalloc(&p->fbA, words, false);
+    if (s != GPS_OK) return s;
+    s = dalloc(&p->fbB, words, false);
+    if (s != GPS_OK) return s;
+    p->fb_words = words;
+    p->fb_ss = ss;
+    return GPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gps_abi_version(void) { return GPS_ABI_VERSION; }
+
+const char* gps_status_str(gps_status s) {
+    switch (s) {
+        case GPS_OK: return "ok";
+        case GPS_ERR_INVALID_ARG: return "invalid argument";
+        case GPS_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case GPS_ERR_HIP: return "HIP runtime error";
+        case GPS_ERR_UNSUPPORTED: return "unsupported";
+        case GPS_ERR_IO: return "I/O error";
+        case GPS_ERR_NOT_READY: return "not ready";
+    }
+    return "unknown status";
+}
+
+const char* gps_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------------ scenes
+gps_status gps_scene_create(const gps_scene_desc* d, gps_scene** out) {
+    if (!d || !out) return GPS_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (d->sh_degree < 0 || d->sh_degree > 3) return GPS_ERR_INVALID_ARG;
+    if (d->num_gaussians >= (1ull << 32) - GPS_TILE) return GPS_ERR_UNSUPPORTED;
+    if (d->num_gaussians > 0 && (!d->means || !d->rotations || !d->scales || !d->opacities || !d->shs))
+        return GPS_ERR_INVALID_ARG;
+    if (d->format != GPS_FORMAT_FP32 && d->format != GPS_FORMAT_HALF_FP8) return GPS_ERR_INVALID_ARG;
+    const uint64_t N = d->num_gaussians;
+    const int ncoef = (d->sh_degree + 1) * (d->sh_degree + 1);
+    const uint64_t stride = round_up(std::max<uint64_t>(N, 1), 64);
+    gps_scene* s = new gps_scene();
+    s->view.N = (uint32_t)N;
+    s->view.sh_degree = d->sh_degree;
+    s->view.format = d->format;
+    s->view.stride = stride;
+    const uint64_t chunk = 1u << 20;
+    if (d->format == GPS_FORMAT_FP32) {
+        const int nplanes = 11 + 3 * ncoef;
+        float* planes = nullptr;
+        gps_status st = dalloc(&planes, stride * nplanes, false);
+        if (st != GPS_OK) {
+            delete s;
+            return st;
+        }
+        s->planes = planes;
+        s->bytes = stride * nplanes * sizeof(float);
+        std::vector<float> stage(chunk);
+        for (uint64_t b = 0; b < N; b += chunk) {
+            const uint64_t n = std::min(chunk, N - b);
+            for (int p = 0; p < nplanes; ++p) {
+                for (uint64_t i = 0; i < n; ++i) {
+                    const uint64_t g = b + i;
+                    float v;
+                    if (p < 3) v = d->means[3 * g + p];
+                    else if (p < 7) v = d->rotations[4 * g + (p - 3)];
+                    else if (p < 10) v = d->scales[3 * g + (p - 7)];
+                    else if (p == 10) v = d->opacities[g];
+                    else v = d->shs[(uint64_t)3 * ncoef * g + (p - 11)];
+                    stage[i] = v;
+                }
+                hipError_t e = hipMemcpy(planes + (uint64_t)p * stride + b, stage.data(), n * sizeof(float),
+                                         hipMemcpyHostToDevice);
+                if (e != hipSuccess) {
+                    gps_scene_destroy(s);
+                    return hip_fail(e, "scene upload");
+                }
+            }
+        }
+        s->view.planes = planes;
+    } else {
+        // GPS_FORMAT_HALF_FP8: encode on the host (fp16 RNE, fp8 E4M3 RNE with a
+        // per-Gaussian scale = max|rest coefficient| / 448).
+        float* means = nullptr;
+        uint4* geo = nullptr;
+        uint4* shrec = nullptr;
+        gps_status st = dalloc(&means, 3 * stride, false);
+        if (st == GPS_OK) st = dalloc(&geo, stride, false);
+        if (st == GPS_OK) st = dalloc(&shrec, 4 * stride, true);
+        s->planes = means;
+        s->geo = geo;
+        s->shrec = shrec;
+        if (st != GPS_OK) {
+            gps_scene_destroy(s);
+            return st;
+        }
+        s->bytes = stride * (12 + 16 + 64);
+        std::vector<float> mstage(chunk);
+        std::vector<uint32_t> gstage(4 * chunk), sstage(16 * chunk);
+        for (uint64_t b = 0; b < N; b += chunk) {
+            const uint64_t n = std::min(chunk, N - b);
+            for (int c = 0; c < 3; ++c) {
+                for (uint64_t i = 0; i < n; ++i) mstage[i] = d->means[3 * (b + i) + c];
+                hipError_t e = hipMemcpy(means + (uint64_t)c * stride + b, mstage.data(), n * 4, hipMemcpyHostToDevice);
+                if (e != hipSuccess) {
+                    gps_scene_destroy(s);
+                    return hip_fail(e, "scene upload");
+                }
+            }
+            std::vector<uint8_t> enc(64);
+            for (uint64_t i = 0; i < n; ++i) {
+                const uint64_t g = b + i;
+                uint16_t h[8];
+                for (int k = 0; k < 4; ++k) h[k] = gps_float_to_half_rne(d->rotations[4 * g + k]);
+                for (int k = 0; k < 3; ++k) h[4 + k] = gps_float_to_half_rne(d->scales[3 * g + k]);
+                h[7] = gps_float_to_half_rne(d->opacities[g]);
+                std::memcpy(&gstage[4 * i], h, 16);
+                gps_encode_sh_record(d->shs + (uint64_t)3 * ncoef * g, ncoef, enc.data());
+                std::memcpy(&sstage[16 * i], enc.data(), 64);
+            }
+            hipError_t e = hipMemcpy(geo + b, gstage.data(), n * 16, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(shrec + 4 * b, sstage.data(), n * 64, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                gps_scene_destroy(s);
+                return hip_fail(e, "scene upload");
+            }
+        }
+        s->view.means = means;
+        s->view.geo = geo;
+        s->view.shrec = shrec;
+    }
+    *out = s;
+    return GPS_OK;
+}
+
+void gps_scene_destroy(gps_scene* s) {
+    if (!s) return;
+    if (s->planes) (void)hipFree(s->planes);
+    if (s->geo) (void)hipFree(s->geo);
+    if (s->shrec) (void)hipFree(s->shrec);
+    delete s;
+}
+
+uint64_t gps_scene_device_bytes(const gps_scene* s) { return s ? s->bytes : 0; }
+
+// ------------------------------------------------------------------ passes
+gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_pass** out) {
+    if (!st || !scene || !out) return GPS_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (st->width <= 0 || st->height <= 0 || st->render_pass_count < 1 || st->supersampling_factor < 1 ||
+        st->max_number_of_points < 0)
+        return GPS_ERR_INVALID_ARG;
+    if ((int64_t)st->width * st->supersampling_factor >= 65536 || (int64_t)st->height * st->supersampling_factor >= 65536)
+        return GPS_ERR_UNSUPPORTED;
+    gps_pass* p = new gps_pass();
+    p->scene = scene;
+    p->settings = *st;
+    p->N = scene->view.N;
+    p->budget = (uint32_t)std::min<int64_t>(st->max_number_of_points, 0x7FFFFFFFll);
+    const uint64_t N = p->N;
+    p->ntiles = (uint32_t)((N + GPS_TILE - 1) / GPS_TILE);
+    const uint64_t npad = (uint64_t)p->ntiles * GPS_TILE;
+    const uint64_t nwords = (N + 31) / 32 + 2;
+    gps_status s = GPS_OK;
+    auto chk = [&](gps_status r) {
+        if (s == GPS_OK) s = r;
+    };
+    chk(dalloc(&p->weights, npad, true));
+    chk(dalloc(&p->maskA, nwords, true));
+    chk(dalloc(&p->maskB, nwords, true));
+    chk(dalloc(&p->recs, N, false));
+    chk(dalloc(&p->tile_sum, p->ntiles, true));
+    chk(dalloc(&p->tile_cnt, p->ntiles, true));
+    chk(dalloc(&p->tile_off, p->ntiles, true));
+    chk(dalloc(&p->tile_voff, p->ntiles, true));
+    chk(dalloc(&p->cidx, N, false));
+    chk(dalloc(&p->coff, N, false));
+    chk(dalloc(&p->chunk_start, (uint64_t)p->budget / GPS_SPLAT_CHUNK + 2, true));
+    chk(dalloc(&p->ctr, 1, true));
+    // Hi-Z pyramid at output resolution, initialised to +inf (nothing occluded).
+    p->hiz_size = (uint32_t)mip_total(p->hiz, st->width, st->height);
+    chk(dalloc(&p->hiz.mip, p->hiz_size, false));
+    if (s == GPS_OK) launch_fill_f32(p->hiz.mip, p->hiz_size, __builtin_inff(), nullptr);
+    // Culling blocks (gaussian_bvh.cu:171-195).
+    p->hier_available = st->fully_disable_hierarchical_culling == 0;
+    uint32_t gpb = st->gaussians_per_block > 0 ? (uint32_t)st->gaussians_per_block
+                                               : (uint32_t)std::max(4.0f, std::floor((float)N / (float)1e6));
+    const uint32_t nb = (uint32_t)((N + gpb - 1) / gpb);
+    p->cull.gpb = gpb;
+    p->cull.node_count = nb;
+    p->cull.mask_words = (nb + 31) / 32;
+    if (p->hier_available) {
+        chk(dalloc(&p->nodes, nb, false));
+        chk(dalloc(&p->cull_mask, p->cull.mask_words + 2, true));
+        if (s == GPS_OK) launch_build_nodes(scene->view, gpb, nb, p->nodes, nullptr);
+    } else {
+        chk(dalloc(&p->cull_mask, 2, true));
+    }
+    p->cull.nodes = p->nodes;
+    p->cull.mask = p->cull_mask;
+    if (s == GPS_OK) s = ensure_fb(p, st->supersampling_factor);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    p->splat_grid = (uint32_t)cus * 8u;
+    if (s == GPS_OK) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) s = hip_fail(e, "pass setup");
+    }
+    if (s != GPS_OK) {
+        gps_pass_destroy(p);
+        return s;
+    }
+    *out = p;
+    return GPS_OK;
+}
+
+void gps_pass_destroy(gps_pass* p) {
+    if (!p) return;
+    dfree(p->weights);
+    dfree(p->maskA);
+    dfree(p->maskB);
+    dfree(p->recs);
+    dfree(p->tile_sum);
+    dfree(p->tile_cnt);
+    dfree(p->tile_off);
+    dfree(p->tile_voff);
+    dfree(p->cidx);
+    dfree(p->coff);
+    dfree(p->chunk_start);
+    dfree(p->ctr);
+    dfree(p->dbg_indices);
+    dfree(p->fbA);
+    dfree(p->fbB);
+    dfree(p->hiz.mip);
+    dfree(p->nodes);
+    dfree(p->cull_mask);
+    delete p;
+}
+
+gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_color, float* d_depth, void* stream) {
+    if (!p || !fp || !d_color || !d_depth) return GPS_ERR_INVALID_ARG;
+    if (fp->width != p->settings.width || fp->height != p->settings.height) {
+        g_last_error = "runtime resolution change is not supported (main.cu:82-86)";
+        return GPS_ERR_INVALID_ARG;
+    }
+    if (fp->supersampling_factor < 1 || !(fp->far_plane > fp->near_plane)) return GPS_ERR_INVALID_ARG;
+    const int ss = fp->supersampling_factor;
+    if ((int64_t)fp->width * ss >= 65536 || (int64_t)fp->height * ss >= 65536) return GPS_ERR_UNSUPPORTED;
+    hipStream_t st = (hipStream_t)stream;
+    gps_status s = ensure_fb(p, ss);
+    if (s != GPS_OK) return s;
+
+    // Frame constants (gaussian_point_splatting.cu:605-615).
+    hm::M4 V, P;
+    std::memcpy(V.m, fp->view, 64);
+    std::memcpy(P.m, fp->proj, 64);
+    const hm::M4 cam = hm::affine_inverse(V);
+    const hm::M4 VP = hm::mul(P, V);
+    FrameConsts f;
+    std::memcpy(f.V, V.m, 64);
+    std::memcpy(f.VP, VP.m, 64);
+    f.campos[0] = cam.at(3, 0);
+    f.campos[1] = cam.at(3, 1);
+    f.campos[2] = cam.at(3, 2);
+    const float r = float(fp->width) / fp->height;
+    const float t_f_y = std::tan(fp->fovy_degrees * 0.01745329251994329576923690768489f * 0.5f);
+    const float fovx = 2.0f * std::atan(t_f_y * r);
+    const float t_f_x = std::tan(fovx * 0.5f);
+    f.tan_fov[0] = t_f_x;
+    f.tan_fov[1] = t_f_y;
+    f.focal[0] = (float)(fp->width * ss) / (2.0f * t_f_x);
+    f.focal[1] = (float)(fp->height * ss) / (2.0f * t_f_y);
+    f.dilation = 0.3f * (float)ss * (float)ss;
+    f.near_plane = fp->near_plane;
+    f.far_plane = fp->far_plane;
+    f.up_w = fp->width * ss;
+    f.up_h = fp->height * ss;
+    f.ss = ss;
+    f.spp = p->settings.render_pass_count;
+    f.Ki = fp->reduce_point_count ? ss * ss : 1;
+    f.K = fp->reduce_point_count ? (float)(ss * ss) : 1.0f;
+    f.point_cap = det_f2u_rz((float)(f.up_w * f.up_h) / (2.0f * f.K));
+    f.occl = fp->enable_occlusion_culling ? 1 : 0;
+    const bool hier = fp->enable_hierarchical_culling && p->hier_available;
+    f.hier = hier ? 1 : 0;
+    f.cull_small = fp->cull_small_gaussians ? 1 : 0;
+    f.unbiased = fp->use_unbiased_2d_splatting ? 1 : 0;
+    f.reduce = fp->reduce_point_count ? 1 : 0;
+
+    const uint64_t bg = background_word(fp->background[0], fp->background[1], fp->background[2]);
+    p->cur_fb = (fp->frame % 2 == 0) ? p->fbA : p->fbB;
+    launch_clear(p->cur_fb, p->fb_words, bg, st);
+    // reset per-frame counters (pass results, culled counts, atomics), keep
+    // the Hi-Z ticket and the mask iteration
+    launch_zero_u32(reinterpret_cast<uint32_t*>(p->ctr), 10, st);
+    launch_zero_u32(&p->ctr->atomics, 1, st);
+
+    Mat4 vpm;
+    std::memcpy(vpm.m, VP.m, 64);
+    auto render_pass = [&](bool first) {
+        const int slot = first ? 0 : 1;
+        const uint32_t frame_seed = (uint32_t)(4 * fp->frame + (first ? 0 : 2));
+        if (hier)
+            launch_cull(p->cull, fp->frustum_planes, vpm, p->hiz, f.occl, &p->ctr->culled_blocks[slot], st);
+        PreprocessArgs a;
+        a.scene = p->scene->view;
+        a.f = f;
+        a.f.frame_seed = frame_seed;
+        a.f.only_prev = first ? 0 : 1;
+        a.maskA = p->maskA;
+        a.maskB = p->maskB;
+        a.iteration = &p->ctr->iteration;
+        a.weights = p->weights;
+        a.recs = p->recs;
+        a.tile_sum = p->tile_sum;
+        a.tile_cnt = p->tile_cnt;
+        a.cull = p->cull;
+        a.hiz = p->hiz;
+        launch_preprocess(a, st);
+        DistributeArgs da;
+        da.tile_sum = p->tile_sum;
+        da.tile_cnt = p->tile_cnt;
+        da.ntiles = p->ntiles;
+        da.tile_off = p->tile_off;
+        da.tile_voff = p->tile_voff;
+        da.ctr = &p->ctr->pass[slot];
+        da.budget = p->budget;
+        da.weights = p->weights;
+        da.cidx = p->cidx;
+        da.coff = p->coff;
+        da.chunk_start = p->chunk_start;
+        da.iteration = &p->ctr->iteration;
+        launch_distribute(da, st);
+        SplatArgs sa;
+        sa.recs = p->recs;
+        sa.cidx = p->cidx;
+        sa.coff = p->coff;
+        sa.chunk_start = p->chunk_start;
+        sa.ctr = &p->ctr->pass[slot];
+        sa.fb = p->cur_fb;
+        sa.atomics_count = &p->ctr->atomics;
+        sa.spp = f.spp;
+        sa.K = f.Ki;
+        sa.up_w = f.up_w;
+        sa.unbiased = f.unbiased;
+        sa.frame_seed = frame_seed + 1u;
+        sa.grid = p->splat_grid;
+        launch_splat(sa, st);
+    };
+    render_pass(true);
+    if (fp->enable_occlusion_culling) {
+        launch_combine(p->cur_fb, d_color, d_depth, fp->width, fp->height, ss, f.spp, fp->near_plane, fp->far_plane, 1,
+                       st);
+        launch_hiz_build(p->hiz, d_depth, fp->occlusion_culling_min_reduce_count, &p->ctr->hiz_ticket, st);
+        render_pass(false);
+    }
+    launch_combine(p->cur_fb, d_color, d_depth, fp->width, fp->height, ss, f.spp, fp->near_plane, fp->far_plane, 0, st);
+    if (fp->enable_occlusion_culling)
+        launch_hiz_build(p->hiz, d_depth, fp->occlusion_culling_min_reduce_count, &p->ctr->hiz_ticket, st);
+    p->last_occl = fp->enable_occlusion_culling ? 1 : 0;
+    p->last_K = f.Ki;
+    p->last_spp = f.spp;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return GPS_OK;
+}
+
+gps_status gps_pass_stats(gps_pass* p, gps_stats* out, void* stream) {
+    if (!p || !out) return GPS_ERR_INVALID_ARG;
+    FrameCounters c;
+    GPS_HIP(hipStreamSynchronize((hipStream_t)stream));
+    GPS_HIP(hipMemcpy(&c, p->ctr, sizeof(c), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->num_gaussians = p->N;
+    out->passes = p->last_occl ? 2 : 1;
+    out->K = p->last_K;
+    for (int i = 0; i < out->passes; ++i) {
+        out->valid_gaussians[i] = c.pass[i].valid;
+        out->total_weight[i] = c.pass[i].total;
+        out->num_points[i] = c.pass[i].points;
+        out->culled_blocks[i] = c.culled_blocks[i];
+        out->samples += (uint64_t)c.pass[i].points * (uint64_t)p->last_K * (uint64_t)p->last_spp;
+        if (c.pass[i].total > c.pass[i].points) out->budget_truncated = 1;
+    }
+    out->reserved = (int32_t)c.atomics;
+    return GPS_OK;
+}
+
+uint64_t gps_pass_debug_size(gps_pass* p, int32_t which) {
+    if (!p) return 0;
+    FrameCounters c;
+    if (hipDeviceSynchronize() != hipSuccess) return 0;
+    if (hipMemcpy(&c, p->ctr, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    const PassCounters& lp = c.pass[p->last_occl ? 1 : 0];
+    switch (which) {
+        case GPS_DBG_WEIGHTS: return p->N;
+        case GPS_DBG_MASK_CURRENT:
+        case GPS_DBG_MASK_PREVIOUS: return (p->N + 31) / 32;
+        case GPS_DBG_VALID_IDS:
+        case GPS_DBG_VALID_OFFSETS: return lp.valid;
+        case GPS_DBG_INDICES: return lp.points;
+        case GPS_DBG_FRAMEBUFFER: return p->fb_words;
+        case GPS_DBG_RECORDS: return 16ull * p->N;
+        case GPS_DBG_HIZ: return p->hiz_size;
+        case GPS_DBG_CULL_MASK: return p->cull.mask_words;
+    }
+    return 0;
+}
+
+gps_status gps_pass_debug_copy(gps_pass* p, int32_t which, void* dst, uint64_t count, void* stream) {
+    if (!p || (!dst && count)) return GPS_ERR_INVALID_ARG;
+    GPS_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (count > gps_pass_debug_size(p, which)) return GPS_ERR_INVALID_ARG;
+    if (count == 0) return GPS_OK;
+    FrameCounters c;
+    GPS_HIP(hipMemcpy(&c, p->ctr, sizeof(c), hipMemcpyDeviceToHost));
+    const PassCounters& lp = c.pass[p->last_occl ? 1 : 0];
+    // The mask written by the last pass is "current" unless that pass produced
+    // points, in which case the iteration has since flipped.
+    const uint32_t written = lp.points > 0 ? (c.iteration - 1u) : c.iteration;
+    const uint32_t* mask_written = (written & 1u) ? p->maskB : p->maskA;
+    const uint32_t* mask_other = (written & 1u) ? p->maskA : p->maskB;
+    const void* src = nullptr;
+    uint64_t esz = 4;
+    switch (which) {
+        case GPS_DBG_WEIGHTS: src = p->weights; break;
+        case GPS_DBG_MASK_CURRENT: src = mask_written; break;
+        case GPS_DBG_MASK_PREVIOUS: src = mask_other; break;
+        case GPS_DBG_VALID_IDS: src = p->cidx; break;
+        case GPS_DBG_VALID_OFFSETS: src = p->coff; break;
+        case GPS_DBG_FRAMEBUFFER: src = p->cur_fb; esz = 8; break;
+        case GPS_DBG_RECORDS: src = p->recs; break;
+        case GPS_DBG_HIZ: src = p->hiz.mip; break;
+        case GPS_DBG_CULL_MASK: src = p->cull_mask; break;
+        case GPS_DBG_INDICES: {
+            if (p->dbg_indices_cap < count) {
+                dfree(p->dbg_indices);
+                gps_status s = dalloc(&p->dbg_indices, count, false);
+                if (s != GPS_OK) return s;
+                p->dbg_indices_cap = count;
+            }
+            launch_materialize_indices(p->cidx, p->coff, &p->ctr->pass[p->last_occl ? 1 : 0], p->dbg_indices,
+                                       (uint32_t)count, nullptr);
+            GPS_HIP(hipDeviceSynchronize());
+            src = p->dbg_indices;
+            break;
+        }
+        default: return GPS_ERR_INVALID_ARG;
+    }
+    GPS_HIP(hipMemcpy(dst, src, count * esz, hipMemcpyDeviceToHost));
+    return GPS_OK;
+}
+
+// ------------------------------------------------------------------ resolve
+gps_status gps_resolve_taa(const float* cur_color, const float* cur_depth, const float* prev_color,
+                           const float* prev_depth, const float reprojection[16], float blend_factor, int32_t width,
+                           int32_t height, int32_t frame_number, float* out_color, uint8_t* out_rgb8, void* stream) {
+    (void)prev_depth;
+    if (!cur_color || !cur_depth || !prev_color || !reprojection || !out_color || !out_rgb8 || width <= 0 || height <= 0)
+        return GPS_ERR_INVALID_ARG;
+    if (out_color == cur_color) return GPS_ERR_INVALID_ARG;
+    Mat4 M;
+    std::memcpy(M.m, reprojection, 64);
+    launch_taa(cur_color, cur_depth, prev_color, M, blend_factor, width, height, frame_number, out_color, out_rgb8,
+               (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "taa launch");
+    return GPS_OK;
+}
+
+gps_status gps_resolve_progressive(float* color, float* accum, int32_t width, int32_t height, int32_t num_frames,
+                                   uint8_t* out_rgb8, void* stream) {
+    if (!color || !accum || !out_rgb8 || width <= 0 || height <= 0 || num_frames < 1) return GPS_ERR_INVALID_ARG;
+    launch_progressive(color, accum, width, height, num_frames, out_rgb8, (hipStream_t)stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "progressive launch");
+    return GPS_OK;
+}
+
+// ------------------------------------------------------------------ camera math
+void gps_look_at(const float eye[3], const float center[3], const float up[3], float out_view[16]) {
+    const hm::M4 v = hm::look_at({eye[0], eye[1], eye[2]}, {center[0], center[1], center[2]}, {up[0], up[1], up[2]});
+    std::memcpy(out_view, v.m, 64);
+}
+void gps_perspective(float fovy, float aspect, float zn, float zf, float out[16]) {
+    const hm::M4 p = hm::perspective(fovy, aspect, zn, zf);
+    std::memcpy(out, p.m, 64);
+}
+void gps_frustum_planes(float fovy, const float view[16], float aspect, float zn, float zf, float out[24]) {
+    hm::M4 v;
+    std::memcpy(v.m, view, 64);
+    hm::frustum_planes(fovy, v, aspect, zn, zf, out);
+}
+void gps_frame_params_init(gps_frame_params* f, const float eye[3], const float center[3], const float up[3],
+                           float fovy, int32_t width, int32_t height, int32_t ss, float zn, float zf) {
+    std::memset(f, 0, sizeof(*f));
+    gps_look_at(eye, center, up, f->view);
+    const float aspect = float(width) / height;
+    gps_perspective(fovy, aspect, zn, zf, f->proj);
+    gps_frustum_planes(fovy, f->view, aspect, zn, zf, f->frustum_planes);
+    f->fovy_degrees = fovy;
+    f->near_plane = zn;
+    f->far_plane = zf;
+    f->width = width;
+    f->height = height;
+    f->supersampling_factor = ss;
+    f->frame = 1;
+    f->enable_occlusion_culling = 1;
+    f->enable_hierarchical_culling = 1;
+    f->use_unbiased_2d_splatting = 1;
+    f->reduce_point_count = 1;
+}
+void gps_taa_reprojection(const float prev_vp[16], const float view[16], const float proj[16], float out[16]) {
+    hm::M4 a, v, p;
+    std::memcpy(a.m, prev_vp, 64);
+    std::memcpy(v.m, view, 64);
+    std::memcpy(p.m, proj, 64);
+    const hm::M4 r = hm::mul(hm::mul(a, hm::affine_inverse(v)), hm::inverse(p));
+    std::memcpy(out, r.m, 64);
+}
+void gps_mat4_mul(const float a[16], const float b[16], float out[16]) {
+    hm::M4 x, y;
+    std::memcpy(x.m, a, 64);
+    std::memcpy(y.m, b, 64);
+    const hm::M4 r = hm::mul(x, y);
+    std::memcpy(out, r.m, 64);
+}
+
+// ------------------------------------------------------------------ plumbing
+gps_status gps_device_count(int32_t* out) {
+    int n = 0;
+    GPS_HIP(hipGetDeviceCount(&n));
+    *out = n;
+    return GPS_OK;
+}
+gps_status gps_set_device(int32_t d) {
+    GPS_HIP(hipSetDevice(d));
+    return GPS_OK;
+}
+gps_status gps_device_malloc(void** out, uint64_t bytes) {
+    GPS_HIP(hipMalloc(out, bytes ? bytes : 1));
+    return GPS_OK;
+}
+gps_status gps_device_free(void* ptr) {
+    GPS_HIP(hipFree(ptr));
+    return GPS_OK;
+}
+gps_status gps_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream) {
+    GPS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    GPS_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return GPS_OK;
+}
+gps_status gps_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream) {
+    GPS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    GPS_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return GPS_OK;
+}
+gps_status gps_memset_d8(void* dst, int32_t value, uint64_t bytes, void* stream) {
+    GPS_HIP(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+    return GPS_OK;
+}
+gps_status gps_stream_create(void** out) {
+    hipStream_t s;
+    GPS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *out = (void*)s;
+    return GPS_OK;
+}
+gps_status gps_stream_destroy(void* s) {
+    GPS_HIP(hipStreamDestroy((hipStream_t)s));
+    return GPS_OK;
+}
+gps_status gps_stream_synchronize(void* s) {
+    GPS_HIP(hipStreamSynchronize((hipStream_t)s));
+    return GPS_OK;
+}
+gps_status gps_device_free_memory(uint64_t* free_b, uint64_t* total_b) {
+    size_t f = 0, t = 0;
+    GPS_HIP(hipMemGetInfo(&f, &t));
+    *free_b = f;
+    *total_b = t;
+    return GPS_OK;
+}
+gps_status gps_event_create(void** out) {
+    hipEvent_t e;
+    GPS_HIP(hipEventCreate(&e));
+    *out = (void*)e;
+    return GPS_OK;
+}
+gps_status gps_event_destroy(void* e) {
+    GPS_HIP(hipEventDestroy((hipEvent_t)e));
+    return GPS_OK;
+}
+gps_status gps_event_record(void* e, void* s) {
+    GPS_HIP(hipEventRecord((hipEvent_t)e, (hipStream_t)s));
+    return GPS_OK;
+}
+gps_status gps_event_elapsed_ms(void* a, void* b, float* out) {
+    GPS_HIP(hipEventSynchronize((hipEvent_t)b));
+    GPS_HIP(hipEventElapsedTime(out, (hipEvent_t)a, (hipEvent_t)b));
+    return GPS_OK;
+}
+
+}  // extern "C"
+
+// IEEE binary16 round-to-nearest-even from binary32.
+uint16_t gps_float_to_half_rne(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7FFFFFFFu;
+    if (ax >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u));
+    if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);  // rounds to >= 65520 -> inf
+    if (ax < 0x38800000u) {                                   // < 2^-14: subnormal half
+        if (ax < 0x33000000u) return (uint16_t)sign;          // < 2^-25 rounds to 0
+        const uint32_t e = ax >> 23;                          // 102..112
+        const uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+        const uint32_t drop = 126u - e;                       // 14..24
+        uint32_t q = m >> drop;
+        const uint32_t rem = m & ((1u << drop) - 1u);
+        const uint32_t half = 1u << (drop - 1u);
+        if (rem > half || (rem == half && (q & 1u))) ++q;
+        return (uint16_t)(sign | q);
+    }
+    uint32_t h = ((ax >> 13) - (112u << 10));
+    const uint32_t rem = ax & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+    return (uint16_t)(sign | h);
+}

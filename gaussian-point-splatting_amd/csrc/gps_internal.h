@@ -1,0 +1,110 @@
+// gps_internal.h — device-side data layout shared by the kernels and the host
+// orchestration of the MI355X path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gps.h"
+#include "gps_project.h"
+
+namespace gps {
+
+// Resident scene, structure-of-arrays ("planes") so every attribute load of a
+// wave is one contiguous 256-byte segment.
+//
+// GPS_FORMAT_FP32: planes of `stride` floats:
+//   0..2 mean xyz, 3..6 quaternion wxyz, 7..9 scale xyz, 10 opacity,
+//   11 + 3*i + c  SH coefficient i of channel c (i < (deg+1)^2)
+// GPS_FORMAT_HALF_FP8:
+//   means: 3 float planes
+//   geo:   uint4 per Gaussian = 8 halves: qw qx qy qz sx sy sz opacity
+//   shrec: 64 bytes per Gaussian: 3 halves DC (rgb), 1 pad half, float scale,
+//          45 fp8-E4M3 rest coefficients (coefficient-major, rgb interleaved),
+//          3 pad bytes
+struct DevScene {
+    const float* planes;        // FP32 format
+    const float* means;         // HALF_FP8 format
+    const uint4* geo;           // HALF_FP8 format
+    const uint4* shrec;         // HALF_FP8 format (4 x uint4 per Gaussian)
+    uint64_t stride;            // plane stride in elements
+    uint32_t N;
+    int32_t sh_degree;
+    int32_t format;
+};
+
+struct Mat4 {
+    float m[16];
+};
+
+// Cached per-Gaussian splat parameters written by preprocess for every valid
+// Gaussian and read by the splat kernel (replaces the reference's per-point
+// re-projection, gaussian_point_splatting.cu:416-423; the values are the same
+// bits either way).  64 bytes = 4 x dwordx4.
+struct SplatRecord {
+    float pmx, pmy;            // pixel-space mean
+    float c0, c1, c2;          // Cholesky factor of cov2d
+    float conx, cony, conz;    // conic
+    float opacity;
+    float inv_alpha;           // 1 / max(opacity, 1e-37)
+    float dilog_alpha;         // dilog(opacity)
+    uint32_t bmin;             // x | y << 16 (tile-aligned bounds)
+    uint32_t bmax;
+    uint32_t packed_lo;        // depth28 << 36 | colour36
+    uint32_t packed_hi;
+    uint32_t pad;
+};
+static_assert(sizeof(SplatRecord) == 64, "record must stay 64 bytes");
+
+// Hi-Z pyramid (coarsest level 0 first), src/core/rendering/occlusion/depth_mip_chain.cuh:10-71.
+#define GPS_HIZ_MAX_LEVELS 16
+struct HiZView {
+    float* mip;
+    int32_t width, height, levels;
+    int32_t lw[GPS_HIZ_MAX_LEVELS];
+    int32_t lh[GPS_HIZ_MAX_LEVELS];
+    uint32_t off[GPS_HIZ_MAX_LEVELS];
+};
+
+// Flat culling blocks (gaussian_bvh.cuh:28-53).
+struct CullNode {
+    float mn[3];
+    float mx[3];
+    uint32_t begin, count;
+};
+struct CullView {
+    const CullNode* nodes;
+    uint32_t* mask;
+    uint32_t node_count;
+    uint32_t mask_words;
+    uint32_t gpb;   // Gaussians per block
+};
+
+// Device-side counters of one render pass.
+struct PassCounters {
+    uint32_t total;       // sum of masked weights (mod 2^32), = reference last_offset + last_count
+    uint32_t valid;       // number of valid Gaussians
+    uint32_t points;      // min(total, budget)
+    uint32_t chunks;      // ceil(points / CHUNK)
+};
+// Per frame: counters of pass 1 and 2, culled block counts, Hi-Z ticket.
+struct FrameCounters {
+    PassCounters pass[2];
+    uint32_t culled_blocks[2];
+    uint32_t hiz_ticket;
+    uint32_t iteration;   // valid-mask ping-pong index (flips when a pass yields points)
+    uint32_t atomics;     // 64-bit atomicMin issued by the splat kernel
+    uint32_t pad[3];
+};
+
+// Splat work decomposition: one workgroup processes CHUNK consecutive points,
+// PPT consecutive points per thread.
+#define GPS_SPLAT_THREADS 256
+#define GPS_SPLAT_PPT 4
+#define GPS_SPLAT_CHUNK (GPS_SPLAT_THREADS * GPS_SPLAT_PPT)
+// Distributor tile: 256 threads x 16 Gaussians.
+#define GPS_TILE_SHIFT 12
+#define GPS_TILE (1u << GPS_TILE_SHIFT)
+#define GPS_PRE_THREADS 256
+
+}  // namespace gps

@@ -1,0 +1,68 @@
+// gps_kernels.h — launch interface of the gfx950 kernels (gps_kernels.hip).
+#pragma once
+
+#include "gps_internal.h"
+
+namespace gps {
+
+struct PreprocessArgs {
+    DevScene scene;
+    FrameConsts f;
+    uint32_t* maskA;
+    uint32_t* maskB;
+    const uint32_t* iteration;
+    uint32_t* weights;
+    SplatRecord* recs;
+    uint32_t* tile_sum;
+    uint32_t* tile_cnt;
+    CullView cull;
+    HiZView hiz;
+};
+
+struct DistributeArgs {
+    uint32_t* tile_sum;
+    uint32_t* tile_cnt;
+    uint32_t ntiles;
+    uint32_t* tile_off;
+    uint32_t* tile_voff;
+    PassCounters* ctr;
+    uint32_t budget;
+    const uint32_t* weights;
+    uint32_t* cidx;
+    uint32_t* coff;
+    uint32_t* chunk_start;
+    uint32_t* iteration;
+};
+
+struct SplatArgs {
+    const SplatRecord* recs;
+    const uint32_t* cidx;
+    const uint32_t* coff;
+    const uint32_t* chunk_start;
+    const PassCounters* ctr;
+    uint64_t* fb;
+    uint32_t* atomics_count;
+    int32_t spp, K, up_w, unbiased;
+    uint32_t frame_seed;
+    uint32_t grid;
+};
+
+void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st);
+void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
+                 uint32_t* culled_count, hipStream_t st);
+void launch_preprocess(const PreprocessArgs& a, hipStream_t st);
+void launch_distribute(const DistributeArgs& a, hipStream_t st);
+void launch_splat(const SplatArgs& a, hipStream_t st);
+void launch_combine(const uint64_t* fb, float* color, float* depth, int W, int H, int ss, int spp, float near_plane,
+                    float far_plane, int depth_only, hipStream_t st);
+void launch_hiz_build(const HiZView& h, const float* depth, int kmin, uint32_t* ticket, hipStream_t st);
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st);
+void launch_fill_f32(float* p, uint32_t n, float v, hipStream_t st);
+void launch_build_nodes(const DevScene& s, uint32_t gpb, uint32_t nb, CullNode* nodes, hipStream_t st);
+void launch_taa(const float* cur, const float* cur_depth, const float* prev, const Mat4& M, float blend, int W, int H,
+                int frame_number, float* out, uint8_t* rgb8, hipStream_t st);
+void launch_progressive(float* color, float* accum, int W, int H, int n, uint8_t* rgb8, hipStream_t st);
+void launch_materialize_indices(const uint32_t* cidx, const uint32_t* coff, const PassCounters* ctr, uint32_t* out,
+                                uint32_t max_points, hipStream_t st);
+
+}  // namespace gps

@@ -1,0 +1,925 @@
+// gps_kernels.hip — gfx950 kernels of the per-frame hot path.
+//
+//   clear_kernel        FB <- background word          (gaussian_point_splatting.cu:580-593)
+//   cull_kernel         block frustum + Hi-Z test      (occlusion/gaussian_bvh.cu:287-329)
+//   preprocess_kernel   project, count, colour, cache  (gaussian_point_splatting.cu:147-307)
+//   tile_scan_kernel +  workload distribution          (workload/workload_distributor.cu:62-145)
+//   distribute_kernel   -> compacted (id, offset) lists + per-chunk start
+//   splat_kernel        point sampling + 64-bit atomicMin (gaussian_point_splatting.cu:67-141,383-432)
+//   combine_kernel      FB -> colour / NDC depth       (gaussian_point_splatting.cu:485-548)
+//   hiz_build_kernel    single-launch Hi-Z pyramid     (occlusion/depth_mip_chain.cu:107-145)
+//   taa / progressive   resolve                        (passes/post_processing_kernels.cuh:108-285)
+//
+// Wave64: every ballot yields two 32-bit mask words so the reference's
+// 32-flags-per-word layout (config.h:28) is kept.
+#include "gps_fb.h"
+#include "gps_internal.h"
+#include "gps_kernels.h"
+#include "gps_rng.h"
+
+namespace gps {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ uint32_t wave_reduce_add(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+// Inclusive scan across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ bool mask_bit(const uint32_t* m, uint32_t i) { return ((m[i >> 5] >> (i & 31u)) & 1u) != 0u; }
+
+// gaussian_bvh.cuh:47-53
+__device__ __forceinline__ bool block_culled(const CullView& c, uint32_t g) {
+    const uint32_t span = c.gpb * 32u;
+    const uint32_t tree = g / span;
+    const uint32_t blk = (g % span) / c.gpb;
+    return ((c.mask[tree] >> blk) & 1u) != 0u;
+}
+
+// ceil(log2(d / 2)) for integer d >= 1, exactly as the float expression of
+// depth_mip_chain.cuh:186 evaluates it (log2 of a power of two is exact; any
+// other d < 2^24 is far from an integer).
+__device__ __forceinline__ int ceil_log2_half(int d) {
+    if (d <= 1) return -1;
+    const int cl = 32 - __clz((uint32_t)(d - 1));  // ceil(log2(d))
+    return cl - 1;
+}
+
+// depth_mip_chain.cuh:172-215
+__device__ __forceinline__ bool hiz_occluded(const HiZView& h, float depth, int minx, int miny, int maxx, int maxy) {
+    minx = max(0, minx);
+    miny = max(0, miny);
+    maxx = min(h.width - 1, maxx);
+    maxy = min(h.height - 1, maxy);
+    const int w = maxx - minx + 1;
+    const int hh = maxy - miny + 1;
+    const int d = max(w, hh);
+    int lod = ceil_log2_half(d);
+    lod = min(max(lod, 0), h.levels - 1);
+    const int l = (h.levels - 1) - lod;
+    const int wl = h.lw[l], hl = h.lh[l];
+    const int tileW = max(1, min(h.width, 1 << (h.levels - l - 1)));
+    const int tileH = max(1, min(h.height, 1 << (h.levels - l - 1)));
+    const int tx0 = max(0, min(wl - 1, minx / tileW));
+    const int ty0 = max(0, min(hl - 1, miny / tileH)) * wl;
+    const int tx1 = max(0, min(wl - 1, maxx / tileW));
+    const int ty1 = max(0, min(hl - 1, maxy / tileH)) * wl;
+    const float* m = h.mip + h.off[l];
+    const float zmax = det_fmax(m[tx0 + ty0], det_fmax(m[tx1 + ty0], det_fmax(m[tx0 + ty1], m[tx1 + ty1])));
+    return zmax < depth;
+}
+
+// ------------------------------------------------------------------ scene access
+__device__ __forceinline__ float plane(const DevScene& s, int p, uint32_t g) {
+    return s.planes[(uint64_t)p * s.stride + g];
+}
+__device__ __forceinline__ float half_bits_to_float(uint32_t h16) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h16);
+}
+// OCP FP8 E4M3 -> float (exact).
+__device__ __forceinline__ float fp8_to_float(uint32_t q) {
+    const uint32_t s = (q & 0x80u) << 24;
+    const uint32_t em = q & 0x7Fu;
+    if (em == 0x7Fu) return __builtin_bit_cast(float, 0x7FC00000u);
+    if (em >= 8u) return __builtin_bit_cast(float, s | ((em << 20) + (120u << 23)));
+    const float v = (float)em * 0x1.0p-9f;
+    return s ? -v : v;
+}
+
+struct Geo {
+    float mx, my, mz, qw, qx, qy, qz, sx, sy, sz, op;
+};
+template <int FORMAT> __device__ __forceinline__ Geo load_geo(const DevScene& s, uint32_t g);
+template <> __device__ __forceinline__ Geo load_geo<GPS_FORMAT_FP32>(const DevScene& s, uint32_t g) {
+    Geo o;
+    o.mx = plane(s, 0, g);
+    o.my = plane(s, 1, g);
+    o.mz = plane(s, 2, g);
+    o.qw = plane(s, 3, g);
+    o.qx = plane(s, 4, g);
+    o.qy = plane(s, 5, g);
+    o.qz = plane(s, 6, g);
+    o.sx = plane(s, 7, g);
+    o.sy = plane(s, 8, g);
+    o.sz = plane(s, 9, g);
+    o.op = plane(s, 10, g);
+    return o;
+}
+template <> __device__ __forceinline__ Geo load_geo<GPS_FORMAT_HALF_FP8>(const DevScene& s, uint32_t g) {
+    Geo o;
+    o.mx = s.means[g];
+    o.my = s.means[s.stride + g];
+    o.mz = s.means[2 * s.stride + g];
+    const uint4 q = s.geo[g];
+    o.qw = half_bits_to_float(q.x & 0xFFFFu);
+    o.qx = half_bits_to_float(q.x >> 16);
+    o.qy = half_bits_to_float(q.y & 0xFFFFu);
+    o.qz = half_bits_to_float(q.y >> 16);
+    o.sx = half_bits_to_float(q.z & 0xFFFFu);
+    o.sy = half_bits_to_float(q.z >> 16);
+    o.sz = half_bits_to_float(q.w & 0xFFFFu);
+    o.op = half_bits_to_float(q.w >> 16);
+    return o;
+}
+
+// SH colour of Gaussian g.
+template <int FORMAT, int DEG> struct ShEval;
+template <int DEG> struct ShEval<GPS_FORMAT_FP32, DEG> {
+    __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch)
+            rgb[ch] = sh_channel<DEG>(x, y, z, [&](int i) { return plane(s, 11 + 3 * i + ch, g); });
+    }
+};
+template <int DEG> struct ShEval<GPS_FORMAT_HALF_FP8, DEG> {
+    __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
+        const uint4* r = s.shrec + 4 * (uint64_t)g;
+        const uint4 w0 = r[0];
+        uint32_t words[16];
+        words[0] = w0.x; words[1] = w0.y; words[2] = w0.z; words[3] = w0.w;
+        if (DEG > 0) {
+            const uint4 w1 = r[1];
+            words[4] = w1.x; words[5] = w1.y; words[6] = w1.z; words[7] = w1.w;
+        }
+        if (DEG > 1) {
+            const uint4 w2 = r[2];
+            words[8] = w2.x; words[9] = w2.y; words[10] = w2.z; words[11] = w2.w;
+        }
+        if (DEG > 2) {
+            const uint4 w3 = r[3];
+            words[12] = w3.x; words[13] = w3.y; words[14] = w3.z; words[15] = w3.w;
+        }
+        const float scale = __builtin_bit_cast(float, words[2]);
+        const float dc[3] = {half_bits_to_float(words[0] & 0xFFFFu), half_bits_to_float(words[0] >> 16),
+                             half_bits_to_float(words[1] & 0xFFFFu)};
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            rgb[ch] = sh_channel<DEG>(x, y, z, [&](int i) -> float {
+                if (i == 0) return dc[ch];
+                const int b = 3 * (i - 1) + ch;           // byte index into the rest block
+                const uint32_t w = words[3 + (b >> 2)];    // rest bytes start at byte 12
+                return fp8_to_float((w >> (8 * (b & 3))) & 0xFFu) * scale;
+            });
+        }
+    }
+};
+
+// ------------------------------------------------------------------ clear
+__global__ __launch_bounds__(256) void clear_kernel(uint4* __restrict__ fb, uint64_t n2, uint64_t value) {
+    const uint4 v = make_uint4((uint32_t)value, (uint32_t)(value >> 32), (uint32_t)value, (uint32_t)(value >> 32));
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * blockDim.x)
+        fb[i] = v;
+}
+
+// ------------------------------------------------------------------ block cull
+struct Planes {
+    float p[24];
+};
+__global__ __launch_bounds__(256) void cull_kernel(CullView cv, Planes pl, Mat4 vp, HiZView h, int occl,
+                                                   uint32_t* __restrict__ culled_count) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    bool culled = false;
+    if (idx < cv.node_count) {
+        const CullNode n = cv.nodes[idx];
+        bool outside = false;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const float nx = pl.p[4 * i], ny = pl.p[4 * i + 1], nz = pl.p[4 * i + 2], d = pl.p[4 * i + 3];
+            const float vx = nx >= 0.0f ? n.mx[0] : n.mn[0];
+            const float vy = ny >= 0.0f ? n.mx[1] : n.mn[1];
+            const float vz = nz >= 0.0f ? n.mx[2] : n.mn[2];
+            if (((nx * vx + ny * vy) + nz * vz) + d < 0.0f) outside = true;
+        }
+        if (outside) {
+            culled = true;
+        } else if (occl) {
+            // gaussian_bvh.cu:237-285
+            int minx = h.width - 1, miny = h.height - 1, maxx = 0, maxy = 0;
+            float minDepth = 1e30f;
+            bool anyFront = false;
+            const float* M = vp.m;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float cx = (i & 1) ? n.mx[0] : n.mn[0];
+                const float cy = (i & 2) ? n.mx[1] : n.mn[1];
+                const float cz = (i & 4) ? n.mx[2] : n.mn[2];
+                const float hx = (M[0] * cx + M[4] * cy) + (M[8] * cz + M[12] * 1.0f);
+                const float hy = (M[1] * cx + M[5] * cy) + (M[9] * cz + M[13] * 1.0f);
+                const float hz = (M[2] * cx + M[6] * cy) + (M[10] * cz + M[14] * 1.0f);
+                const float hw = (M[3] * cx + M[7] * cy) + (M[11] * cz + M[15] * 1.0f);
+                if (hw <= 0.0f) continue;
+                anyFront = true;
+                const float nxp = hx / hw, nyp = hy / hw, nzp = hz / hw;
+                const int px = det_f2i_rz((nxp * 0.5f + 0.5f) * (float)h.width);
+                const int py = det_f2i_rz((nyp * 0.5f + 0.5f) * (float)h.height);
+                minx = min(minx, px);
+                miny = min(miny, py);
+                maxx = max(maxx, px);
+                maxy = max(maxy, py);
+                minDepth = det_fmin(minDepth, nzp);
+            }
+            if (anyFront) {
+                minx = max(minx, 0);
+                miny = max(miny, 0);
+                maxx = min(maxx, h.width - 1);
+                maxy = min(maxy, h.height - 1);
+                if (!(minx > maxx || miny > maxy)) culled = hiz_occluded(h, minDepth, minx, miny, maxx, maxy);
+            }
+        }
+    }
+    const uint64_t b = __ballot(culled);
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        const uint32_t w0 = (idx - lane) >> 5;
+        if (w0 < cv.mask_words) cv.mask[w0] = (uint32_t)b;
+        if (w0 + 1 < cv.mask_words) cv.mask[w0 + 1] = (uint32_t)(b >> 32);
+        const uint32_t c = (uint32_t)__popcll(b);
+        if (c) atomicAdd(culled_count, c);
+    }
+}
+
+// ------------------------------------------------------------------ preprocess
+template <int FORMAT, int DEG>
+__global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
+    DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
+    const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
+    uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h) {
+    const uint32_t g = blockIdx.x * GPS_PRE_THREADS + threadIdx.x;
+    // workload_distributor.cu:32-40: current = A on even iterations.
+    const uint32_t it = *iteration;
+    uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
+    const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
+    uint32_t n = 0;
+    if (g < s.N) {
+        u32x2 seed = make_seed(g, f.frame_seed);
+        do {
+            if (f.hier && block_culled(cv, g)) break;
+            if (f.only_prev && mask_bit(mask_prev, g)) break;
+            const Geo ge = load_geo<FORMAT>(s, g);
+            const float op = glm_min(ge.op, 1.0f);
+            if (op <= 0.0f) break;
+            Projected pr;
+            if (!project_gaussian(f, ge.mx, ge.my, ge.mz, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx, ge.sy, ge.sz, pr)) break;
+            if (pr.minx >= f.up_w || pr.miny >= f.up_h || pr.maxx < 0 || pr.maxy < 0) break;
+            if (f.cull_small) {
+                const float tr = pr.ca + pr.cc;
+                const float t2 = 0.5f * tr;
+                const float term2 = det_sqrt(det_fmax(0.1f, t2 * t2 - pr.det));
+                const float e1 = t2 + term2, e2 = t2 - term2;
+                if (e1 < 0.02f || e2 < 0.02f) break;
+            }
+            const float s_det = det_sqrt(pr.det);
+            const float dl = dilog(op);
+            const float importance = f.unbiased ? GPS_TWO_PI_F * s_det * dl : op * GPS_TWO_PI_F * s_det;
+            if (f.occl && hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss)) break;
+            uint32_t cnt = f.unbiased ? poisson(seed, importance)
+                                      : (uint32_t)stochastic_round(importance, pcg2d(seed).x);
+            if (f.K > 1.0f) cnt = (uint32_t)stochastic_round((float)cnt / f.K, pcg2d(seed).x);
+            if (cnt == 0u) break;
+            cnt = min(cnt, f.point_cap);
+            float dx, dy, dz;
+            view_dir(f, ge.mx, ge.my, ge.mz, dx, dy, dz);
+            float rgb[3];
+            ShEval<FORMAT, DEG>::eval(s, g, dx, dy, dz, rgb);
+            const float c0 = det_sqrt(pr.ca);
+            const float c1 = pr.cb / c0;
+            const float c2 = det_sqrt(pr.cc - c1 * c1);
+            const float inv_det = 1.0f / pr.det;
+            const uint64_t packed = (depth_quantize(pr.view_z, f.near_plane, f.far_plane) << GPS_DEPTH_SHIFT) |
+                                    pack_color36(rgb[0], rgb[1], rgb[2]);
+            uint4* r = reinterpret_cast<uint4*>(recs + g);
+            r[0] = make_uint4(__float_as_uint(pr.pmx), __float_as_uint(pr.pmy), __float_as_uint(c0), __float_as_uint(c1));
+            r[1] = make_uint4(__float_as_uint(c2), __float_as_uint(pr.cc * inv_det), __float_as_uint(-pr.cb * inv_det),
+                              __float_as_uint(pr.ca * inv_det));
+            r[2] = make_uint4(__float_as_uint(op), __float_as_uint(corrected_inv_alpha(op)), __float_as_uint(dl),
+                              (uint32_t)pr.minx | ((uint32_t)pr.miny << 16));
+            r[3] = make_uint4((uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16), (uint32_t)packed,
+                              (uint32_t)(packed >> 32), 0u);
+            n = cnt;
+        } while (false);
+        weights[g] = n;
+    }
+    // Two 32-bit valid-mask words per wave.
+    const uint64_t b = __ballot(n != 0u);
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave_first = g - (uint32_t)lane;
+    const uint32_t sum = wave_reduce_add(n);
+    __shared__ uint32_t s_sum[GPS_PRE_THREADS / 64], s_cnt[GPS_PRE_THREADS / 64];
+    if (lane == 0) {
+        if (wave_first < s.N) mask_cur[wave_first >> 5] = (uint32_t)b;
+        if (wave_first + 32u < s.N) mask_cur[(wave_first >> 5) + 1] = (uint32_t)(b >> 32);
+        s_sum[threadIdx.x >> 6] = sum;
+        s_cnt[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ts = 0, tc = 0;
+#pragma unroll
+        for (int w = 0; w < GPS_PRE_THREADS / 64; ++w) {
+            ts += s_sum[w];
+            tc += s_cnt[w];
+        }
+        const uint32_t tile = (blockIdx.x * GPS_PRE_THREADS) >> GPS_TILE_SHIFT;
+        if (tc) {
+            atomicAdd(tile_sum + tile, ts);
+            atomicAdd(tile_cnt + tile, tc);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ distribution
+// Exclusive scan of the per-tile (point, valid) sums; one workgroup.  Also
+// clears the tile sums for the next pass and publishes the pass totals.
+__global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt,
+                                                         uint32_t ntiles, uint32_t* __restrict__ tile_off,
+                                                         uint32_t* __restrict__ tile_voff, PassCounters* __restrict__ ctr,
+                                                         uint32_t budget, uint32_t* __restrict__ iteration) {
+    __shared__ uint32_t s_a[16], s_c[16];
+    __shared__ uint32_t s_carry_a, s_carry_c;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        s_carry_a = 0;
+        s_carry_c = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < ntiles; base += 1024) {
+        const uint32_t i = base + tid;
+        const uint32_t a = i < ntiles ? tile_sum[i] : 0u;
+        const uint32_t c = i < ntiles ? tile_cnt[i] : 0u;
+        const uint32_t ia = wave_scan_incl(a);
+        const uint32_t ic = wave_scan_incl(c);
+        if (lane == 63) {
+            s_a[wave] = ia;
+            s_c[wave] = ic;
+        }
+        __syncthreads();
+        uint32_t wa = 0, wc = 0;
+        for (int w = 0; w < wave; ++w) {
+            wa += s_a[w];
+            wc += s_c[w];
+        }
+        const uint32_t ca = s_carry_a, cc = s_carry_c;
+        if (i < ntiles) {
+            tile_off[i] = ca + wa + ia - a;
+            tile_voff[i] = cc + wc + ic - c;
+            tile_sum[i] = 0u;
+            tile_cnt[i] = 0u;
+        }
+        __syncthreads();
+        if (tid == 1023) {
+            s_carry_a = ca + wa + ia;
+            s_carry_c = cc + wc + ic;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const uint32_t total = s_carry_a;
+        const uint32_t P = total < budget ? total : budget;
+        ctr->total = total;
+        ctr->valid = s_carry_c;
+        ctr->points = P;
+        ctr->chunks = (uint32_t)(((uint64_t)P + GPS_SPLAT_CHUNK - 1) / GPS_SPLAT_CHUNK);
+        // The reference flips the mask ping-pong only when the pass produced
+        // points (workload_distributor.cu:111-115,144).
+        if (P > 0u) *iteration = *iteration + 1u;
+    }
+}
+
+// Per tile of 4096 Gaussians: local exclusive scan of the masked weights, then
+// the compacted (Gaussian id, point offset) lists and the first compacted
+// entry of every splat chunk.  Equivalent to the reference's scatter +
+// inclusive max-scan: point p belongs to the last valid Gaussian whose offset
+// is <= p.
+__global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restrict__ weights,
+                                                         const uint32_t* __restrict__ tile_off,
+                                                         const uint32_t* __restrict__ tile_voff,
+                                                         const PassCounters* __restrict__ ctr, uint32_t* __restrict__ cidx,
+                                                         uint32_t* __restrict__ coff, uint32_t* __restrict__ chunk_start) {
+    __shared__ uint32_t s_a[4], s_c[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t g0 = (tile << GPS_TILE_SHIFT) + (uint32_t)tid * 16u;
+    const uint4* wp = reinterpret_cast<const uint4*>(weights + g0);
+    uint32_t w[16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 v = wp[k];
+        w[4 * k] = v.x;
+        w[4 * k + 1] = v.y;
+        w[4 * k + 2] = v.z;
+        w[4 * k + 3] = v.w;
+    }
+    uint32_t sa = 0, sc = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        sa += w[k];
+        sc += w[k] ? 1u : 0u;
+    }
+    const uint32_t ia = wave_scan_incl(sa);
+    const uint32_t ic = wave_scan_incl(sc);
+    if (lane == 63) {
+        s_a[wave] = ia;
+        s_c[wave] = ic;
+    }
+    __syncthreads();
+    uint32_t wa = 0, wc = 0;
+    for (int q = 0; q < wave; ++q) {
+        wa += s_a[q];
+        wc += s_c[q];
+    }
+    uint32_t off = tile_off[tile] + wa + ia - sa;
+    uint32_t r = tile_voff[tile] + wc + ic - sc;
+    const uint64_t P = ctr->points;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (w[k]) {
+            cidx[r] = g0 + (uint32_t)k;
+            coff[r] = off;
+            const uint64_t lo = off;
+            const uint64_t hi = (lo + w[k]) < P ? (lo + w[k]) : P;
+            for (uint64_t c = (lo + GPS_SPLAT_CHUNK - 1) / GPS_SPLAT_CHUNK; c * GPS_SPLAT_CHUNK < hi; ++c)
+                chunk_start[c] = r;
+            ++r;
+            off += w[k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ splat
+struct SplatConsts {
+    int32_t spp, K, up_w, unbiased;
+    uint32_t frame_seed;
+};
+
+__global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRecord* __restrict__ recs,
+                                                                  const uint32_t* __restrict__ cidx,
+                                                                  const uint32_t* __restrict__ coff,
+                                                                  const uint32_t* __restrict__ chunk_start,
+                                                                  const PassCounters* __restrict__ ctr,
+                                                                  unsigned long long* __restrict__ fb, SplatConsts sc,
+                                                                  uint32_t* __restrict__ atomics_count) {
+    __shared__ uint32_t s_off[GPS_SPLAT_CHUNK + 1];
+    const uint32_t P = ctr->points;
+    const uint32_t V = ctr->valid;
+    const uint32_t nchunks = ctr->chunks;
+    const int tid = threadIdx.x;
+    const int samples = sc.spp * sc.K;
+    uint32_t issued = 0;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint32_t r_lo = chunk_start[c];
+        for (int j = tid; j <= GPS_SPLAT_CHUNK; j += GPS_SPLAT_THREADS) {
+            const uint32_t r = r_lo + (uint32_t)j;
+            s_off[j] = r < V ? coff[r] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        const uint32_t p0 = c * GPS_SPLAT_CHUNK + (uint32_t)tid * GPS_SPLAT_PPT;
+        if (p0 < P) {
+            // last j with s_off[j] <= p0 (s_off[0] <= c*CHUNK <= p0)
+            int lo = 0, hi = GPS_SPLAT_CHUNK;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= p0) lo = mid;
+                else hi = mid - 1;
+            }
+            int j = lo;
+            uint32_t cur_g = 0xFFFFFFFFu;
+            SplatRecord rec;
+            for (int k = 0; k < GPS_SPLAT_PPT; ++k) {
+                const uint32_t p = p0 + (uint32_t)k;
+                if (p >= P) break;
+                while (j < GPS_SPLAT_CHUNK - 1 && s_off[j + 1] <= p) ++j;
+                if (r_lo + (uint32_t)j >= V) break;
+                const uint32_t g = cidx[r_lo + (uint32_t)j];
+                if (g != cur_g) {
+                    cur_g = g;
+                    const uint4* rp = reinterpret_cast<const uint4*>(recs + g);
+                    const uint4 a = rp[0], b = rp[1], cc = rp[2], d = rp[3];
+                    rec.pmx = __uint_as_float(a.x);
+                    rec.pmy = __uint_as_float(a.y);
+                    rec.c0 = __uint_as_float(a.z);
+                    rec.c1 = __uint_as_float(a.w);
+                    rec.c2 = __uint_as_float(b.x);
+                    rec.conx = __uint_as_float(b.y);
+                    rec.cony = __uint_as_float(b.z);
+                    rec.conz = __uint_as_float(b.w);
+                    rec.opacity = __uint_as_float(cc.x);
+                    rec.inv_alpha = __uint_as_float(cc.y);
+                    rec.dilog_alpha = __uint_as_float(cc.z);
+                    rec.bmin = cc.w;
+                    rec.bmax = d.x;
+                    rec.packed_lo = d.y;
+                    rec.packed_hi = d.z;
+                }
+                const int bminx = (int)(rec.bmin & 0xFFFFu), bminy = (int)(rec.bmin >> 16);
+                const int bmaxx = (int)(rec.bmax & 0xFFFFu), bmaxy = (int)(rec.bmax >> 16);
+                const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
+                const unsigned long long depth_bits = packed & ~(unsigned long long)GPS_COLOR_MASK;
+                u32x2 seed = make_seed(p, sc.frame_seed);
+                for (int q = 0; q < samples; ++q) {
+                    const f32x2 u = pcg2d(seed);
+                    const f32x2 xy = sc.unbiased ? corrected_box_muller_pre(u.x, u.y, rec.inv_alpha, rec.dilog_alpha)
+                                                 : box_muller(u.x, u.y);
+                    const float fx = det_floor(rec.pmx + rec.c0 * xy.x);
+                    const float fy = det_floor((rec.pmy + rec.c1 * xy.x) + rec.c2 * xy.y);
+                    const int x = det_f2i_rz(fx);
+                    const int y = det_f2i_rz(fy);
+                    if (x < bminx || x >= bmaxx || y < bminy || y >= bmaxy) continue;
+                    const float dx = rec.pmx - fx, dy = rec.pmy - fy;
+                    const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
+                    const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
+                    const float alpha = rec.opacity * gw;
+                    if (alpha < 1.0f / 255.0f) continue;
+                    const int idx = fb_index(x, y, q / sc.K, sc.spp, sc.up_w);
+                    if (fb[idx] <= depth_bits) continue;
+                    atomicMin(fb + idx, packed);
+                    ++issued;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (atomics_count) {
+        const uint32_t tot = wave_reduce_add(issued);
+        if ((tid & 63) == 0 && tot) atomicAdd(atomics_count, tot);
+    }
+}
+
+// ------------------------------------------------------------------ combine
+// One workgroup = 16x16 output pixels; lanes follow Morton order inside the
+// block so a wave's framebuffer words are one contiguous range when ss is a
+// power of two.
+__global__ __launch_bounds__(256) void combine_kernel(const unsigned long long* __restrict__ fb, float* __restrict__ color,
+                                                      float* __restrict__ depth, int W, int H, int ss, int spp,
+                                                      int up_w, float near_plane, float far_plane, int depth_only) {
+    const uint32_t t = threadIdx.x;
+    uint32_t mx = 0, my = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        mx |= ((t >> (2 * b)) & 1u) << b;
+        my |= ((t >> (2 * b + 1)) & 1u) << b;
+    }
+    const int x = blockIdx.x * 16 + (int)mx;
+    const int y = blockIdx.y * 16 + (int)my;
+    if (x >= W || y >= H) return;
+    const int bx = x * ss, by = y * ss;
+    float d = 1e9f;
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    for (int dx = 0; dx < ss; ++dx)
+        for (int dy = 0; dy < ss; ++dy)
+            for (int p = 0; p < spp; ++p) {
+                const unsigned long long v = fb[fb_index(bx + dx, by + dy, p, spp, up_w)];
+                const float vd = depth_dequantize((uint32_t)((v >> 36) & GPS_DEPTH_MAX), near_plane, far_plane);
+                const float z = view_to_ndc_depth(vd, near_plane, far_plane);
+                d = glm_min(d, z);
+                if (depth_only) continue;
+                cr += (float)((uint32_t)(v >> 24) & 0xFFFu) / 255.0f;
+                cg += (float)((uint32_t)(v >> 12) & 0xFFFu) / 255.0f;
+                cb += (float)((uint32_t)v & 0xFFFu) / 255.0f;
+            }
+    const uint32_t idx = (uint32_t)(y * W + x);
+    if (!depth_only) {
+        const float inv = 1.0f / (float)(spp * ss * ss);
+        color[3 * idx] = cr * inv;
+        color[3 * idx + 1] = cg * inv;
+        color[3 * idx + 2] = cb * inv;
+    }
+    depth[idx] = d;
+}
+
+// ------------------------------------------------------------------ Hi-Z
+// Single launch: each workgroup reduces a 32x32 block of the finest level
+// through five levels in LDS; the last workgroup to finish (device-scope
+// ticket) reduces the remaining coarse levels.
+__global__ __launch_bounds__(256) void hiz_build_kernel(HiZView h, const float* __restrict__ depth, int kmin,
+                                                        uint32_t* __restrict__ ticket) {
+    __shared__ float s0[1024], s1[1024];
+    __shared__ int s_last;
+    const int L = h.levels, fl = L - 1;
+    const int tid = threadIdx.x;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    float* cur = s0;
+    float* nxt = s1;
+    for (int i = tid; i < 1024; i += 256) {
+        const int X = bx * 32 + (i & 31), Y = by * 32 + (i >> 5);
+        float v = 0.0f;
+        if (X < h.lw[fl] && Y < h.lh[fl]) {
+            v = depth[Y * h.width + X];
+            h.mip[h.off[fl] + (uint32_t)(Y * h.lw[fl] + X)] = v;
+        }
+        cur[i] = v;
+    }
+    __syncthreads();
+    int size = 32;
+    const int last_local = fl - 5 > 0 ? fl - 5 : 0;
+    for (int l = fl - 1; l >= last_local; --l) {
+        const int cs = size;
+        size >>= 1;
+        const bool use_min = ((L - 2) - l) < kmin;
+        for (int i = tid; i < size * size; i += 256) {
+            const int lx = i % size, ly = i / size;
+            const int X = bx * size + lx, Y = by * size + ly;
+            if (X < h.lw[l] && Y < h.lh[l]) {
+                float v = use_min ? __builtin_bit_cast(float, 0x7F800000u) : __builtin_bit_cast(float, 0xFF800000u);
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int CX = 2 * X + dx, CY = 2 * Y + dy;
+                        if (CX < h.lw[l + 1] && CY < h.lh[l + 1]) {
+                            const float c = cur[(2 * ly + dy) * cs + (2 * lx + dx)];
+                            v = use_min ? det_fmin(v, c) : det_fmax(v, c);
+                        }
+                    }
+                h.mip[h.off[l] + (uint32_t)(Y * h.lw[l] + X)] = v;
+                nxt[ly * size + lx] = v;
+            }
+        }
+        __syncthreads();
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+    if (last_local == 0) return;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t prev = atomicAdd(ticket, 1u);
+        s_last = (prev == gridDim.x * gridDim.y - 1u) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int l = last_local - 1; l >= 0; --l) {
+        const bool use_min = ((L - 2) - l) < kmin;
+        const int n = h.lw[l] * h.lh[l];
+        for (int i = tid; i < n; i += 256) {
+            const int X = i % h.lw[l], Y = i / h.lw[l];
+            float v = use_min ? __builtin_bit_cast(float, 0x7F800000u) : __builtin_bit_cast(float, 0xFF800000u);
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int CX = 2 * X + dx, CY = 2 * Y + dy;
+                    if (CX < h.lw[l + 1] && CY < h.lh[l + 1]) {
+                        const float c = __builtin_nontemporal_load(h.mip + h.off[l + 1] + (uint32_t)(CY * h.lw[l + 1] + CX));
+                        v = use_min ? det_fmin(v, c) : det_fmax(v, c);
+                    }
+                }
+            h.mip[h.off[l] + (uint32_t)i] = v;
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+    if (tid == 0) *ticket = 0u;
+}
+
+__global__ __launch_bounds__(64) void zero_u32_kernel(uint32_t* __restrict__ p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += 64) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void fill_f32_kernel(float* __restrict__ p, uint32_t n, float v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------------------ culling blocks
+// gaussian_bvh.cu:111-144 (one thread per block, Gaussians in index order).
+template <int FORMAT>
+__global__ __launch_bounds__(256) void build_nodes_kernel(DevScene s, uint32_t gpb, uint32_t nb, CullNode* __restrict__ nodes) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t begin = b * gpb;
+    const uint32_t end = (begin + gpb < s.N) ? begin + gpb : s.N;
+    float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+    float mx[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    for (uint32_t i = begin; i < end; ++i) {
+        const Geo ge = load_geo<FORMAT>(s, i);
+        float C[9];
+        scale_rot_to_cov(ge.sx, ge.sy, ge.sz, ge.qw, ge.qx, ge.qy, ge.qz, C);
+        const float ex = 3.0f * det_sqrt(C[0]), ey = 3.0f * det_sqrt(C[4]), ez = 3.0f * det_sqrt(C[8]);
+        const float lo[3] = {ge.mx - ex, ge.my - ey, ge.mz - ez};
+        const float hi[3] = {ge.mx + ex, ge.my + ey, ge.mz + ez};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = glm_min(mn[k], lo[k]);
+            mx[k] = glm_max(mx[k], hi[k]);
+        }
+    }
+    CullNode n;
+    for (int k = 0; k < 3; ++k) {
+        n.mn[k] = mn[k];
+        n.mx[k] = mx[k];
+    }
+    n.begin = begin;
+    n.count = end - begin;
+    nodes[b] = n;
+}
+
+// ------------------------------------------------------------------ resolve
+__global__ __launch_bounds__(256) void taa_kernel(const float* __restrict__ cur, const float* __restrict__ cur_depth,
+                                                  const float* __restrict__ prev, Mat4 M, float blend, int W, int H,
+                                                  int frame_number, float* __restrict__ out, uint8_t* __restrict__ rgb8) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= W * H) return;
+    const int x = idx % W, y = idx / W;
+    float c[3] = {cur[3 * idx], cur[3 * idx + 1], cur[3 * idx + 2]};
+    const float depth = cur_depth[idx];
+    if (frame_number > 0) {
+        const float cx = ((float)x + 0.5f) / (float)W * 2.0f - 1.0f;
+        const float cy = ((float)y + 0.5f) / (float)H * 2.0f - 1.0f;
+        const float* m = M.m;
+        const float px0 = (m[0] * cx + m[4] * cy) + (m[8] * depth + m[12] * 1.0f);
+        const float py0 = (m[1] * cx + m[5] * cy) + (m[9] * depth + m[13] * 1.0f);
+        const float pw0 = (m[3] * cx + m[7] * cy) + (m[11] * depth + m[15] * 1.0f);
+        const float ppx = px0 / pw0, ppy = py0 / pw0;
+        const float px = (ppx * 0.5f + 0.5f) * (float)W - 0.5f;
+        const float py = (ppy * 0.5f + 0.5f) * (float)H - 0.5f;
+        if (px >= 0.0f && px < (float)W && py >= 0.0f && py < (float)H) {
+            const float flx = det_floor(px), fly = det_floor(py);
+            int x0 = det_f2i_rz(flx), y0 = det_f2i_rz(fly);
+            int x1 = x0 + 1, y1 = y0 + 1;
+            x0 = (x0 < 0) ? 0 : ((x0 >= W) ? W - 1 : x0);
+            y0 = (y0 < 0) ? 0 : ((y0 >= H) ? H - 1 : y0);
+            x1 = (x1 < 0) ? 0 : ((x1 >= W) ? W - 1 : x1);
+            y1 = (y1 < 0) ? 0 : ((y1 >= H) ? H - 1 : y1);
+            const float tx = px - flx, ty = py - fly;
+            const int i00 = y0 * W + x0, i10 = y0 * W + x1, i01 = y1 * W + x0, i11 = y1 * W + x1;
+            float nmin[3] = {1e9f, 1e9f, 1e9f}, nmax[3] = {-1e9f, -1e9f, -1e9f};
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int yy = y + dy;
+                if (yy < 0 || yy >= H) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int xx = x + dx;
+                    if (xx < 0 || xx >= W) continue;
+                    const int j = yy * W + xx;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        nmin[k] = glm_min(nmin[k], cur[3 * j + k]);
+                        nmax[k] = glm_max(nmax[k], cur[3 * j + k]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float top = prev[3 * i00 + k] * (1.0f - tx) + prev[3 * i10 + k] * tx;
+                const float bot = prev[3 * i01 + k] * (1.0f - tx) + prev[3 * i11 + k] * tx;
+                const float hist = top * (1.0f - ty) + bot * ty;
+                const float hc = glm_clamp(hist, nmin[k] - 0.05f, nmax[k] + 0.05f);
+                c[k] = c[k] * (1.0f - blend) + hc * blend;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        out[3 * idx + k] = c[k];
+        rgb8[3 * idx + k] = (uint8_t)__builtin_roundf(glm_clamp(c[k], 0.0f, 1.0f) * 255.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void progressive_kernel(float* __restrict__ color, float* __restrict__ accum, int W,
+                                                          int H, int n, uint8_t* __restrict__ rgb8) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= W * H) return;
+    const float sc = 255.0f / (float)n;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float c = color[3 * idx + k];
+        if (n > 1) c = c + accum[3 * idx + k];
+        color[3 * idx + k] = c;
+        accum[3 * idx + k] = 0.0f;
+        rgb8[3 * idx + k] = (uint8_t)__builtin_roundf(glm_clamp(c * sc, 0.0f, 255.0f));
+    }
+}
+
+// ------------------------------------------------------------------ debug
+__global__ __launch_bounds__(256) void materialize_indices_kernel(const uint32_t* __restrict__ cidx,
+                                                                  const uint32_t* __restrict__ coff,
+                                                                  const PassCounters* __restrict__ ctr,
+                                                                  uint32_t* __restrict__ out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ctr->points) return;
+    uint32_t lo = 0, hi = ctr->valid - 1;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (coff[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    out[p] = cidx[lo];
+}
+
+// ------------------------------------------------------------------ launchers
+template <int FORMAT, int DEG>
+static void launch_pre(const PreprocessArgs& a, hipStream_t st) {
+    const uint32_t blocks = (a.scene.N + GPS_PRE_THREADS - 1) / GPS_PRE_THREADS;
+    hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st, a.scene, a.f,
+                       a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt, a.cull, a.hiz);
+}
+
+void launch_preprocess(const PreprocessArgs& a, hipStream_t st) {
+    if (a.scene.N == 0) return;
+    const int d = a.scene.sh_degree;
+    if (a.scene.format == GPS_FORMAT_FP32) {
+        if (d == 0) launch_pre<GPS_FORMAT_FP32, 0>(a, st);
+        else if (d == 1) launch_pre<GPS_FORMAT_FP32, 1>(a, st);
+        else if (d == 2) launch_pre<GPS_FORMAT_FP32, 2>(a, st);
+        else launch_pre<GPS_FORMAT_FP32, 3>(a, st);
+    } else {
+        if (d == 0) launch_pre<GPS_FORMAT_HALF_FP8, 0>(a, st);
+        else if (d == 1) launch_pre<GPS_FORMAT_HALF_FP8, 1>(a, st);
+        else if (d == 2) launch_pre<GPS_FORMAT_HALF_FP8, 2>(a, st);
+        else launch_pre<GPS_FORMAT_HALF_FP8, 3>(a, st);
+    }
+}
+
+void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st) {
+    const uint64_t n2 = words / 2;
+    uint64_t blocks = (n2 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(clear_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, reinterpret_cast<uint4*>(fb), n2, value);
+}
+
+void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
+                 uint32_t* culled_count, hipStream_t st) {
+    if (cv.node_count == 0) return;
+    Planes pl;
+    for (int i = 0; i < 24; ++i) pl.p[i] = planes[i];
+    const uint32_t blocks = (cv.node_count + 255) / 256;
+    hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, st, cv, pl, vp, h, occl, culled_count);
+}
+
+void launch_distribute(const DistributeArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, st, a.tile_sum, a.tile_cnt, a.ntiles, a.tile_off,
+                       a.tile_voff, a.ctr, a.budget, a.iteration);
+    if (a.ntiles == 0) return;
+    hipLaunchKernelGGL(distribute_kernel, dim3(a.ntiles), dim3(256), 0, st, a.weights, a.tile_off, a.tile_voff, a.ctr,
+                       a.cidx, a.coff, a.chunk_start);
+}
+
+void launch_splat(const SplatArgs& a, hipStream_t st) {
+    SplatConsts sc;
+    sc.spp = a.spp;
+    sc.K = a.K;
+    sc.up_w = a.up_w;
+    sc.unbiased = a.unbiased;
+    sc.frame_seed = a.frame_seed;
+    hipLaunchKernelGGL(splat_kernel, dim3(a.grid), dim3(GPS_SPLAT_THREADS), 0, st, a.recs, a.cidx, a.coff,
+                       a.chunk_start, a.ctr, reinterpret_cast<unsigned long long*>(a.fb), sc, a.atomics_count);
+}
+
+void launch_combine(const uint64_t* fb, float* color, float* depth, int W, int H, int ss, int spp, float near_plane,
+                    float far_plane, int depth_only, hipStream_t st) {
+    dim3 grid((W + 15) / 16, (H + 15) / 16);
+    hipLaunchKernelGGL(combine_kernel, grid, dim3(256), 0, st, reinterpret_cast<const unsigned long long*>(fb), color,
+                       depth, W, H, ss, spp, W * ss, near_plane, far_plane, depth_only);
+}
+
+void launch_hiz_build(const HiZView& h, const float* depth, int kmin, uint32_t* ticket, hipStream_t st) {
+    dim3 grid((h.width + 31) / 32, (h.height + 31) / 32);
+    hipLaunchKernelGGL(hiz_build_kernel, grid, dim3(256), 0, st, h, depth, kmin, ticket);
+}
+
+void launch_fill_f32(float* p, uint32_t n, float v, hipStream_t st) {
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks), dim3(256), 0, st, p, n, v);
+}
+
+void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, p, n);
+}
+
+void launch_build_nodes(const DevScene& s, uint32_t gpb, uint32_t nb, CullNode* nodes, hipStream_t st) {
+    if (nb == 0) return;
+    const uint32_t blocks = (nb + 255) / 256;
+    if (s.format == GPS_FORMAT_FP32)
+        hipLaunchKernelGGL(build_nodes_kernel<GPS_FORMAT_FP32>, dim3(blocks), dim3(256), 0, st, s, gpb, nb, nodes);
+    else
+        hipLaunchKernelGGL(build_nodes_kernel<GPS_FORMAT_HALF_FP8>, dim3(blocks), dim3(256), 0, st, s, gpb, nb, nodes);
+}
+
+void launch_taa(const float* cur, const float* cur_depth, const float* prev, const Mat4& M, float blend, int W, int H,
+                int frame_number, float* out, uint8_t* rgb8, hipStream_t st) {
+    const int n = W * H;
+    hipLaunchKernelGGL(taa_kernel, dim3((n + 255) / 256), dim3(256), 0, st, cur, cur_depth, prev, M, blend, W, H,
+                       frame_number, out, rgb8);
+}
+
+void launch_progressive(float* color, float* accum, int W, int H, int n, uint8_t* rgb8, hipStream_t st) {
+    const int cnt = W * H;
+    hipLaunchKernelGGL(progressive_kernel, dim3((cnt + 255) / 256), dim3(256), 0, st, color, accum, W, H, n, rgb8);
+}
+
+void launch_materialize_indices(const uint32_t* cidx, const uint32_t* coff, const PassCounters* ctr, uint32_t* out,
+                                uint32_t max_points, hipStream_t st) {
+    if (max_points == 0) return;
+    hipLaunchKernelGGL(materialize_indices_kernel, dim3((max_points + 255) / 256), dim3(256), 0, st, cidx, coff, ctr, out);
+}
+
+}  // namespace gps

@@ -1,0 +1,336 @@
+// gps_render — headless frame driver over the C ABI.
+//
+// Keeps the reference's command-line surface (src/parse_args.cuh:21-127) and
+// frame loop (src/main.cu:79-252, src/core/rendering/render_server.cu:86-196)
+// without a window: frames are written as PPM (RGB8, bottom row first flipped
+// as save_image_to_disk does) and PFM (linear colour).  Flags of the
+// interactive viewer that have no headless meaning are accepted and ignored.
+// Headless additions: --synthetic, --num-gaussians, --seed, --frames, --out,
+// --format, --orbit, --eye/--center, --taa-current-depth.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gps.h"
+
+namespace {
+
+struct Options {
+    // RenderSettings (src/render_settings.cuh:36-88)
+    int width = 1920, height = 1080;
+    int render_pass_count = 1;
+    int supersampling_factor = 2;
+    long long max_number_of_points = 250000000;
+    bool enable_occlusion_culling = true;
+    bool enable_hierarchical_culling = true;
+    bool cull_small = false;
+    bool reduce_point_count = true;
+    bool taa = false;
+    bool run_tests = false;
+    float near_plane = 0.2f, far_plane = 10000.0f, fovy = 45.0f;
+    float background[3] = {0, 0, 0};
+    float rotation[3] = {0, 0, 0}, translation[3] = {0, 0, 0}, scaling[3] = {1, 1, 1};
+    std::string model_path;
+    // headless
+    std::string synthetic = "c1";
+    long long num_gaussians = -1;
+    unsigned long long seed = 0;
+    int frames = 1;
+    std::string out;
+    int format = GPS_FORMAT_FP32;
+    float orbit_radius = -1.0f, orbit_elevation = 20.0f, orbit_step = 3.0f;
+    float eye[3] = {0, 0, -30}, center[3] = {0, 0, 0};
+    bool taa_current_depth = false;
+    int gaussians_per_block = 0;
+    int accumulation_none = 0;
+};
+
+bool parse_vec(int& i, int argc, char** argv, float* out, int n) {
+    if (i + n >= argc) return false;
+    for (int k = 0; k < n; ++k) out[k] = std::stof(argv[++i]);
+    return true;
+}
+
+Options parse(int argc, char** argv) {
+    Options o;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto next = [&]() -> std::string { return (i + 1 < argc) ? std::string(argv[++i]) : std::string(); };
+        if (a == "--background") parse_vec(i, argc, argv, o.background, 3);
+        else if (a == "--rotation") parse_vec(i, argc, argv, o.rotation, 3);
+        else if (a == "--translation") parse_vec(i, argc, argv, o.translation, 3);
+        else if (a == "--scaling") parse_vec(i, argc, argv, o.scaling, 3);
+        else if (a == "--fovy") o.fovy = std::stof(next());
+        else if (a == "--near-view-plane" || a == "--near_view_plane") o.near_plane = std::stof(next());
+        else if (a == "--far-view-plane" || a == "--far_view_plane") o.far_plane = std::stof(next());
+        else if (a == "--samples-per-pixel" || a == "--samples_per_pixel") o.render_pass_count = std::stoi(next());
+        else if (a == "--max-number-of-points" || a == "--max_number_of_points") o.max_number_of_points = std::stoll(next());
+        else if (a == "--supersampling-factor" || a == "--supersampling_factor") o.supersampling_factor = std::stoi(next());
+        else if (a == "--resolution") {
+            float r[2];
+            parse_vec(i, argc, argv, r, 2);
+            o.width = (int)r[0];
+            o.height = (int)r[1];
+        } else if (a == "--set-target-width" || a == "--set_target_width") o.width = std::stoi(next());
+        else if (a == "--cull-small") o.cull_small = true;
+        else if (a == "--one_point_per_thread" || a == "--one-point-per-thread") o.reduce_point_count = false;
+        else if (a == "--disable_hierarchical_culling" || a == "--disable-hierarchical-culling") o.enable_hierarchical_culling = false;
+        else if (a == "--disable_occlusion_culling" || a == "--disable-occlusion-culling") o.enable_occlusion_culling = false;
+        else if (a == "--taa") o.taa = true;
+        else if (a == "--run_tests" || a == "--run-tests") o.run_tests = true;
+        else if (a == "--model-path") o.model_path = next();
+        else if (a == "--max-eval-count" || a == "--max_eval_count" || a == "--cameras-path" || a == "--camera-path" ||
+                 a == "--sorted-model-output-path" || a == "--sorted_model_output_path") next();
+        else if (a == "--freeze-culling" || a == "--enable_profiling" || a == "--disable_gui" || a == "--disable-gui" ||
+                 a == "--sort_morton_order" || a == "--sort-morton-order" || a == "--play_camera_path" ||
+                 a == "--play-camera-path" || a == "--snap_window_to_corner" || a == "--snap-window-to-corner" ||
+                 a == "--create-evaluation-screenshots" || a == "--create_evaluation_screenshots") {
+        }
+        // headless additions
+        else if (a == "--synthetic") o.synthetic = next();
+        else if (a == "--num-gaussians") o.num_gaussians = std::stoll(next());
+        else if (a == "--seed") o.seed = std::stoull(next());
+        else if (a == "--frames") o.frames = std::stoi(next());
+        else if (a == "--out") o.out = next();
+        else if (a == "--format") o.format = next() == "half_fp8" ? GPS_FORMAT_HALF_FP8 : GPS_FORMAT_FP32;
+        else if (a == "--orbit") {
+            float r[2];
+            parse_vec(i, argc, argv, r, 2);
+            o.orbit_radius = r[0];
+            o.orbit_elevation = r[1];
+        } else if (a == "--eye") parse_vec(i, argc, argv, o.eye, 3);
+        else if (a == "--center") parse_vec(i, argc, argv, o.center, 3);
+        else if (a == "--taa-current-depth") o.taa_current_depth = true;
+        else if (a == "--gaussians-per-block") o.gaussians_per_block = std::stoi(next());
+        else if (a == "--accumulation-none") o.accumulation_none = 1;
+        else std::printf("Unknown argument: %s\n", a.c_str());
+    }
+    return o;
+}
+
+#define CHECK(x)                                                                                   \
+    do {                                                                                           \
+        gps_status s_ = (x);                                                                       \
+        if (s_ != GPS_OK) {                                                                        \
+            std::fprintf(stderr, "%s failed: %s (%s)\n", #x, gps_status_str(s_), gps_last_error()); \
+            std::exit(1);                                                                          \
+        }                                                                                          \
+    } while (0)
+
+// Transform::getModelMatrix (src/world/transform.cpp:35-44) of the scene
+// transform: translate * mat4_cast(quat(radians(euler))) * scale.
+void scene_matrix(const Options& o, float out[16]) {
+    const float rx = o.rotation[0] * 0.01745329251994329576923690768489f;
+    const float ry = o.rotation[1] * 0.01745329251994329576923690768489f;
+    const float rz = o.rotation[2] * 0.01745329251994329576923690768489f;
+    // glm quat from euler angles (pitch, yaw, roll)
+    const float cx = std::cos(rx * 0.5f), sx = std::sin(rx * 0.5f);
+    const float cy = std::cos(ry * 0.5f), sy = std::sin(ry * 0.5f);
+    const float cz = std::cos(rz * 0.5f), sz = std::sin(rz * 0.5f);
+    const float w = cx * cy * cz + sx * sy * sz;
+    const float x = sx * cy * cz - cx * sy * sz;
+    const float y = cx * sy * cz + sx * cy * sz;
+    const float z = cx * cy * sz - sx * sy * cz;
+    const float R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y),
+                        2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x),
+                        2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)};
+    std::memset(out, 0, 64);
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) out[4 * c + r] = R[3 * c + r] * o.scaling[c];
+    out[12] = o.translation[0];
+    out[13] = o.translation[1];
+    out[14] = o.translation[2];
+    out[15] = 1.0f;
+}
+
+void write_ppm(const std::string& path, const std::vector<uint8_t>& rgb, int w, int h) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return;
+    std::fprintf(f, "P6\n%d %d\n255\n", w, h);
+    for (int y = h - 1; y >= 0; --y) std::fwrite(&rgb[(size_t)y * w * 3], 1, (size_t)w * 3, f);
+    std::fclose(f);
+}
+
+void write_pfm(const std::string& path, const std::vector<float>& c, int w, int h, float scale) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return;
+    std::fprintf(f, "PF\n%d %d\n-1.0\n", w, h);
+    std::vector<float> row((size_t)w * 3);
+    for (int y = 0; y < h; ++y) {
+        for (int i = 0; i < w * 3; ++i) row[i] = c[(size_t)y * w * 3 + i] * scale;
+        std::fwrite(row.data(), 4, row.size(), f);
+    }
+    std::fclose(f);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Options o = parse(argc, argv);
+    if (!o.model_path.empty()) {
+        std::fprintf(stderr, "--model-path: PLY scenes are not supported by this driver; use --synthetic\n");
+        return 2;
+    }
+    // ---- scene
+    gps_synth_params sp;
+    std::memset(&sp, 0, sizeof(sp));
+    sp.seed = o.seed ? o.seed : 0x5EED0001ull;
+    if (o.synthetic == "c2" || o.synthetic == "clustered") {
+        sp.kind = 1;
+        sp.num_gaussians = 10000000;
+        sp.sh_degree = 3;
+        sp.clusters = 2000;
+        sp.half_extent = 100.0;
+        sp.cluster_sigma = 2.0;
+        sp.log_scale_mu = -3.0;
+        sp.log_scale_sigma = 0.7;
+        sp.opacity_sigma = 1.5;
+        sp.dc_sigma = 0.3;
+        sp.rest_sigma = 0.05;
+        if (!o.seed) sp.seed = 0x5EED0002ull;
+    } else {
+        sp.kind = 0;
+        sp.num_gaussians = 1000000;
+        sp.sh_degree = 0;
+        sp.half_extent = 10.0;
+        sp.log_scale_mu = -4.0;
+        sp.log_scale_sigma = 0.5;
+        sp.opacity_sigma = 1.0;
+        sp.dc_sigma = 0.3;
+    }
+    if (o.num_gaussians >= 0) sp.num_gaussians = (uint64_t)o.num_gaussians;
+    const uint64_t N = sp.num_gaussians;
+    const int nc = (sp.sh_degree + 1) * (sp.sh_degree + 1);
+    std::vector<float> means(3 * N), rots(4 * N), scales(3 * N), ops(N), shs((size_t)3 * nc * N);
+    CHECK(gps_synth_generate(&sp, 0, N, means.data(), rots.data(), scales.data(), ops.data(), shs.data()));
+    gps_scene_desc d;
+    d.num_gaussians = N;
+    d.sh_degree = sp.sh_degree;
+    d.format = o.format;
+    d.means = means.data();
+    d.rotations = rots.data();
+    d.scales = scales.data();
+    d.opacities = ops.data();
+    d.shs = shs.data();
+    gps_scene* scene = nullptr;
+    CHECK(gps_scene_create(&d, &scene));
+    gps_render_settings rs;
+    std::memset(&rs, 0, sizeof(rs));
+    rs.width = o.width;
+    rs.height = o.height;
+    rs.render_pass_count = o.render_pass_count;
+    rs.supersampling_factor = o.supersampling_factor;
+    rs.max_number_of_points = o.max_number_of_points;
+    rs.fully_disable_hierarchical_culling = 0;
+    rs.gaussians_per_block = o.gaussians_per_block;
+    gps_pass* pass = nullptr;
+    CHECK(gps_pass_create(&rs, scene, &pass));
+
+    const int W = o.width, H = o.height;
+    const size_t npx = (size_t)W * H;
+    void* stream = nullptr;
+    CHECK(gps_stream_create(&stream));
+    void *color[2], *depth[2], *hist[2], *rgb;
+    for (int k = 0; k < 2; ++k) {
+        CHECK(gps_device_malloc(&color[k], npx * 12));
+        CHECK(gps_device_malloc(&depth[k], npx * 4));
+        CHECK(gps_device_malloc(&hist[k], npx * 12));
+        CHECK(gps_memset_d8(color[k], 0, npx * 12, stream));
+        CHECK(gps_memset_d8(depth[k], 0, npx * 4, stream));
+        CHECK(gps_memset_d8(hist[k], 0, npx * 12, stream));
+    }
+    CHECK(gps_device_malloc(&rgb, npx * 3));
+    float scene_m[16];
+    scene_matrix(o, scene_m);
+    float prev_vp[16];
+    std::memset(prev_vp, 0, sizeof(prev_vp));
+    int progressive_frames = 0;
+    std::vector<double> times;
+    std::vector<uint8_t> h_rgb(npx * 3);
+    std::vector<float> h_col(npx * 3);
+    for (int fi = 1; fi <= o.frames; ++fi) {
+        float eye[3] = {o.eye[0], o.eye[1], o.eye[2]};
+        if (o.orbit_radius > 0) {
+            const float az = (float)(fi - 1) * o.orbit_step * 0.01745329251994329576923690768489f;
+            const float el = o.orbit_elevation * 0.01745329251994329576923690768489f;
+            eye[0] = o.orbit_radius * std::cos(el) * std::sin(az);
+            eye[1] = o.orbit_radius * std::sin(el);
+            eye[2] = -o.orbit_radius * std::cos(el) * std::cos(az);
+        }
+        const float up[3] = {0, 1, 0};
+        gps_frame_params f;
+        gps_frame_params_init(&f, eye, o.center, up, o.fovy, W, H, o.supersampling_factor, o.near_plane, o.far_plane);
+        float view_scene[16];
+        gps_mat4_mul(f.view, scene_m, view_scene);
+        std::memcpy(f.view, view_scene, 64);
+        gps_frustum_planes(o.fovy, f.view, float(W) / H, o.near_plane, o.far_plane, f.frustum_planes);
+        f.frame = fi;
+        f.enable_occlusion_culling = o.enable_occlusion_culling;
+        f.enable_hierarchical_culling = o.enable_hierarchical_culling;
+        f.cull_small_gaussians = o.cull_small;
+        f.reduce_point_count = o.reduce_point_count;
+        for (int k = 0; k < 3; ++k) f.background[k] = o.background[k];
+        // render_server.cu:86-160: pass -> (color[a], depth[b]); resolve.
+        const int a = fi % 2, b = 1 - a;
+        const auto t0 = std::chrono::steady_clock::now();
+        CHECK(gps_pass_render(pass, &f, (float*)color[a], (float*)depth[b], stream));
+        if (o.taa) {
+            float M[16];
+            gps_taa_reprojection(prev_vp, f.view, f.proj, M);
+            ++progressive_frames;
+            // The reference hands TAA depth[a] (written the frame before) as the
+            // current depth (render_server.cu:92,131-136); --taa-current-depth
+            // passes this frame's depth instead.
+            const float* cur_depth = (const float*)(o.taa_current_depth ? depth[b] : depth[a]);
+            CHECK(gps_resolve_taa((const float*)color[a], cur_depth, (const float*)hist[b], (const float*)depth[b], M,
+                                  0.9f, W, H, progressive_frames, (float*)hist[a], (uint8_t*)rgb, stream));
+        } else {
+            const bool moved = o.orbit_radius > 0;
+            progressive_frames = (moved || o.accumulation_none) ? 1 : progressive_frames + 1;
+            CHECK(gps_resolve_progressive((float*)color[a], (float*)color[b], W, H, progressive_frames, (uint8_t*)rgb,
+                                          stream));
+        }
+        CHECK(gps_stream_synchronize(stream));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        times.push_back(ms);
+        gps_mat4_mul(f.proj, f.view, prev_vp);
+        if (!o.out.empty()) {
+            CHECK(gps_memcpy_d2h(h_rgb.data(), rgb, npx * 3, stream));
+            char name[64];
+            std::snprintf(name, sizeof(name), "_%04d", fi);
+            write_ppm(o.out + name + ".ppm", h_rgb, W, H);
+            CHECK(gps_memcpy_d2h(h_col.data(), o.taa ? hist[a] : color[a], npx * 12, stream));
+            write_pfm(o.out + name + ".pfm", h_col, W, H, o.taa ? 1.0f : 1.0f / (float)progressive_frames);
+        }
+        gps_stats st;
+        CHECK(gps_pass_stats(pass, &st, stream));
+        if (!o.run_tests)
+            std::printf("frame %d: %.3f ms, points %llu + %llu, samples %llu\n", fi, ms,
+                        (unsigned long long)st.num_points[0], (unsigned long long)st.num_points[1],
+                        (unsigned long long)st.samples);
+    }
+    if (o.run_tests) {
+        // src/main.cu:211-241 envelope
+        std::printf("---BEGIN_TIMINGS_JSON---\n{\n  \"algorithm\": \"GaussianPointSplatting\",\n");
+        std::printf("  \"model_path\": \"synthetic:%s\",\n  \"timings\": [\n    { \"camera\": 0, \"samples\": [",
+                    o.synthetic.c_str());
+        for (size_t i = 0; i < times.size(); ++i) std::printf("%s%.4f", i ? ", " : "", times[i]);
+        std::printf("] }\n  ]\n}\n---END_TIMINGS_JSON---\n");
+    }
+    for (int k = 0; k < 2; ++k) {
+        gps_device_free(color[k]);
+        gps_device_free(depth[k]);
+        gps_device_free(hist[k]);
+    }
+    gps_device_free(rgb);
+    gps_stream_destroy(stream);
+    gps_pass_destroy(pass);
+    gps_scene_destroy(scene);
+    return 0;
+}

@@ -1,0 +1,536 @@
+"""ctypes binding of libgps_amd.so, the MI355X Gaussian point splatting hot path.
+
+The C ABI is declared in include/gps.h.  This module mirrors the reference's
+render-pass surface for Python callers (tests, benchmark, serving loop):
+
+    Scene(...)                 GPUGaussianScene::upload_from_cpu
+                               (src/core/gaussians/gaussian_types.cuh:442-503,568-608)
+    Pass(settings, scene)      GaussianPointSplatting(const RenderSettings&)
+                               (src/core/rendering/passes/gaussian_point_splatting.cu:777-816)
+    Pass.render(frame, ...)    RenderPass::render(d_image, d_depth, settings)
+                               (src/core/rendering/render_pass.cuh:17-24)
+    resolve_taa / resolve_progressive
+                               RenderServer::renderFrame post-processing
+                               (src/core/rendering/render_server.cu:122-157)
+
+Device memory is owned by the library (DeviceArray) so no second HIP runtime
+(e.g. torch's bundled one) is needed in the process.  The shared library must
+have been built (``make -C gaussian-point-splatting_amd``); importing this
+module without it raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgps_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `make -C {os.path.dirname(_HERE)}` "
+        "(the MI355X path has no fallback)")
+
+_lib = C.CDLL(LIB_PATH)
+
+# ----------------------------------------------------------------- constants
+GPS_OK = 0
+FORMAT_FP32 = 0
+FORMAT_HALF_FP8 = 1
+
+DBG_WEIGHTS = 0
+DBG_MASK_CURRENT = 1
+DBG_MASK_PREVIOUS = 2
+DBG_VALID_IDS = 3
+DBG_VALID_OFFSETS = 4
+DBG_INDICES = 5
+DBG_FRAMEBUFFER = 6
+DBG_RECORDS = 7
+DBG_HIZ = 8
+DBG_CULL_MASK = 9
+_DBG_DTYPE = {DBG_FRAMEBUFFER: np.uint64, DBG_HIZ: np.float32}
+
+ACCUM_PROGRESSIVE = 0
+ACCUM_TAA = 1
+ACCUM_NONE = 2
+
+
+# ----------------------------------------------------------------- structs
+class SceneDesc(C.Structure):
+    _fields_ = [("num_gaussians", C.c_uint64), ("sh_degree", C.c_int32), ("format", C.c_int32),
+                ("means", C.c_void_p), ("rotations", C.c_void_p), ("scales", C.c_void_p),
+                ("opacities", C.c_void_p), ("shs", C.c_void_p)]
+
+
+class RenderSettingsC(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("render_pass_count", C.c_int32),
+                ("supersampling_factor", C.c_int32), ("max_number_of_points", C.c_int64),
+                ("fully_disable_hierarchical_culling", C.c_int32), ("gaussians_per_block", C.c_int32)]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("proj", C.c_float * 16), ("frustum_planes", C.c_float * 24),
+                ("fovy_degrees", C.c_float), ("near_plane", C.c_float), ("far_plane", C.c_float),
+                ("background", C.c_float * 3), ("width", C.c_int32), ("height", C.c_int32),
+                ("supersampling_factor", C.c_int32), ("frame", C.c_int32),
+                ("enable_occlusion_culling", C.c_int32), ("enable_hierarchical_culling", C.c_int32),
+                ("cull_small_gaussians", C.c_int32), ("use_unbiased_2d_splatting", C.c_int32),
+                ("reduce_point_count", C.c_int32), ("occlusion_culling_min_reduce_count", C.c_int32)]
+
+    def copy(self) -> "FrameParams":
+        f = FrameParams()
+        C.memmove(C.addressof(f), C.addressof(self), C.sizeof(FrameParams))
+        return f
+
+
+class Stats(C.Structure):
+    _fields_ = [("num_gaussians", C.c_uint64), ("valid_gaussians", C.c_uint64 * 2),
+                ("total_weight", C.c_uint64 * 2), ("num_points", C.c_uint64 * 2), ("samples", C.c_uint64),
+                ("culled_blocks", C.c_uint64 * 2), ("passes", C.c_int32), ("budget_truncated", C.c_int32),
+                ("K", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {"num_gaussians": int(self.num_gaussians), "valid_gaussians": list(self.valid_gaussians),
+                "total_weight": list(self.total_weight), "num_points": list(self.num_points),
+                "samples": int(self.samples), "culled_blocks": list(self.culled_blocks),
+                "passes": int(self.passes), "budget_truncated": int(self.budget_truncated), "K": int(self.K),
+                "atomics": int(self.reserved) & 0xFFFFFFFF}
+
+
+class SynthParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("num_gaussians", C.c_uint64), ("kind", C.c_int32),
+                ("sh_degree", C.c_int32), ("clusters", C.c_int32), ("reserved", C.c_int32),
+                ("half_extent", C.c_double), ("cluster_sigma", C.c_double), ("log_scale_mu", C.c_double),
+                ("log_scale_sigma", C.c_double), ("opacity_sigma", C.c_double), ("dc_sigma", C.c_double),
+                ("rest_sigma", C.c_double), ("offset", C.c_double * 3)]
+
+
+_vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
+
+
+def _sig(name, res, *args):
+    fn = getattr(_lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_sig("gps_abi_version", C.c_int32)
+_sig("gps_status_str", C.c_char_p, C.c_int32)
+_sig("gps_last_error", C.c_char_p)
+_sig("gps_scene_create", C.c_int32, C.POINTER(SceneDesc), C.POINTER(_vp))
+_sig("gps_scene_destroy", None, _vp)
+_sig("gps_scene_device_bytes", C.c_uint64, _vp)
+_sig("gps_pass_create", C.c_int32, C.POINTER(RenderSettingsC), _vp, C.POINTER(_vp))
+_sig("gps_pass_destroy", None, _vp)
+_sig("gps_pass_render", C.c_int32, _vp, C.POINTER(FrameParams), _vp, _vp, _vp)
+_sig("gps_pass_stats", C.c_int32, _vp, C.POINTER(Stats), _vp)
+_sig("gps_pass_debug_size", C.c_uint64, _vp, C.c_int32)
+_sig("gps_pass_debug_copy", C.c_int32, _vp, C.c_int32, _vp, C.c_uint64, _vp)
+_sig("gps_resolve_taa", C.c_int32, _vp, _vp, _vp, _vp, _fp, C.c_float, C.c_int32, C.c_int32, C.c_int32, _vp, _vp,
+     _vp)
+_sig("gps_resolve_progressive", C.c_int32, _vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp, _vp)
+_sig("gps_look_at", None, _fp, _fp, _fp, _fp)
+_sig("gps_perspective", None, C.c_float, C.c_float, C.c_float, C.c_float, _fp)
+_sig("gps_frustum_planes", None, C.c_float, _fp, C.c_float, C.c_float, C.c_float, _fp)
+_sig("gps_frame_params_init", None, C.POINTER(FrameParams), _fp, _fp, _fp, C.c_float, C.c_int32, C.c_int32,
+     C.c_int32, C.c_float, C.c_float)
+_sig("gps_taa_reprojection", None, _fp, _fp, _fp, _fp)
+_sig("gps_mat4_mul", None, _fp, _fp, _fp)
+_sig("gps_synth_generate", C.c_int32, C.POINTER(SynthParams), C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp, _vp)
+_sig("gps_encode_half_fp8", C.c_int32, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("gps_float_to_half", C.c_uint16, C.c_float)
+_sig("gps_float_to_fp8e4m3", C.c_uint8, C.c_float)
+_sig("gps_device_count", C.c_int32, C.POINTER(C.c_int32))
+_sig("gps_set_device", C.c_int32, C.c_int32)
+_sig("gps_device_malloc", C.c_int32, C.POINTER(_vp), C.c_uint64)
+_sig("gps_device_free", C.c_int32, _vp)
+_sig("gps_memcpy_h2d", C.c_int32, _vp, _vp, C.c_uint64, _vp)
+_sig("gps_memcpy_d2h", C.c_int32, _vp, _vp, C.c_uint64, _vp)
+_sig("gps_memset_d8", C.c_int32, _vp, C.c_int32, C.c_uint64, _vp)
+_sig("gps_stream_create", C.c_int32, C.POINTER(_vp))
+_sig("gps_stream_destroy", C.c_int32, _vp)
+_sig("gps_stream_synchronize", C.c_int32, _vp)
+_sig("gps_device_free_memory", C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
+_sig("gps_event_create", C.c_int32, C.POINTER(_vp))
+_sig("gps_event_destroy", C.c_int32, _vp)
+_sig("gps_event_record", C.c_int32, _vp, _vp)
+_sig("gps_event_elapsed_ms", C.c_int32, _vp, _vp, C.POINTER(C.c_float))
+
+_sig("gps_selftest_math", C.c_int32, C.c_int32, _vp, _vp, C.c_uint64, _vp)
+_sig("gps_selftest_rng", C.c_int32, C.c_uint32, C.c_uint32, C.c_int32, _vp, _vp)
+_sig("gps_selftest_poisson", C.c_int32, C.c_uint32, _vp, _vp, _vp)
+
+lib = _lib
+
+
+class GpsError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        msg = _lib.gps_status_str(status).decode()
+        last = _lib.gps_last_error().decode()
+        super().__init__(f"{what}: {msg}" + (f" ({last})" if last else ""))
+        self.status = status
+
+
+def check(status: int, what: str) -> None:
+    if status != GPS_OK:
+        raise GpsError(status, what)
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f(a) -> "C.Array":
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+# ----------------------------------------------------------------- device
+def device_count() -> int:
+    n = C.c_int32(0)
+    st = _lib.gps_device_count(C.byref(n))
+    return int(n.value) if st == GPS_OK else 0
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(_lib.gps_stream_create(C.byref(h)), "stream create")
+        self.handle = h
+
+    def synchronize(self) -> None:
+        check(_lib.gps_stream_synchronize(self.handle), "stream synchronize")
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.gps_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = C.c_void_p()
+        check(_lib.gps_event_create(C.byref(h)), "event create")
+        self.handle = h
+
+    def record(self, stream: Optional[Stream] = None) -> None:
+        check(_lib.gps_event_record(self.handle, stream.handle if stream else None), "event record")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float(0)
+        check(_lib.gps_event_elapsed_ms(self.handle, end.handle, C.byref(ms)), "event elapsed")
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.gps_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation owned by libgps_amd with a numpy shape/dtype."""
+
+    def __init__(self, shape, dtype=np.float32, zero: bool = False):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(_lib.gps_device_malloc(C.byref(p), max(self.nbytes, 1)), "device malloc")
+        self.ptr = p
+        if zero:
+            self.zero()
+
+    @property
+    def addr(self):
+        return self.ptr
+
+    def zero(self, stream: Optional[Stream] = None) -> None:
+        check(_lib.gps_memset_d8(self.ptr, 0, self.nbytes, stream.handle if stream else None), "memset")
+        if stream is None:
+            check(_lib.gps_stream_synchronize(None), "sync")
+
+    def upload(self, a: np.ndarray, stream: Optional[Stream] = None) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.nbytes, self.nbytes)
+        check(_lib.gps_memcpy_h2d(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes,
+                                  stream.handle if stream else None), "h2d")
+
+    def download(self, stream: Optional[Stream] = None) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(_lib.gps_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes,
+                                  stream.handle if stream else None), "d2h")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.gps_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- scenes
+@dataclass
+class SceneArrays:
+    """Host scene in the CPUGaussian layout (gaussian_types.cuh:89-100)."""
+    means: np.ndarray          # (N, 3) float32
+    rotations: np.ndarray      # (N, 4) float32, (w, x, y, z)
+    scales: np.ndarray         # (N, 3) float32, linear
+    opacities: np.ndarray      # (N,)   float32, post-sigmoid
+    shs: np.ndarray            # (N, 3*(deg+1)^2) float32
+    sh_degree: int = 0
+
+    @property
+    def n(self) -> int:
+        return int(self.means.shape[0])
+
+    def desc(self, fmt: int = FORMAT_FP32) -> SceneDesc:
+        for name in ("means", "rotations", "scales", "opacities", "shs"):
+            a = getattr(self, name)
+            assert a.dtype == np.float32 and a.flags.c_contiguous, name
+        d = SceneDesc()
+        d.num_gaussians = self.n
+        d.sh_degree = self.sh_degree
+        d.format = fmt
+        d.means = _ptr(self.means)
+        d.rotations = _ptr(self.rotations)
+        d.scales = _ptr(self.scales)
+        d.opacities = _ptr(self.opacities)
+        d.shs = _ptr(self.shs)
+        return d
+
+    def subset(self, idx) -> "SceneArrays":
+        return SceneArrays(np.ascontiguousarray(self.means[idx]), np.ascontiguousarray(self.rotations[idx]),
+                           np.ascontiguousarray(self.scales[idx]), np.ascontiguousarray(self.opacities[idx]),
+                           np.ascontiguousarray(self.shs[idx]), self.sh_degree)
+
+
+def synth_params(seed: int, n: int, kind: str = "uniform", sh_degree: int = 0, clusters: int = 1,
+                 half_extent: float = 10.0, cluster_sigma: float = 2.0, log_scale_mu: float = -4.0,
+                 log_scale_sigma: float = 0.5, opacity_sigma: float = 1.0, dc_sigma: float = 0.3,
+                 rest_sigma: float = 0.05, offset=(0.0, 0.0, 0.0)) -> SynthParams:
+    p = SynthParams()
+    p.seed = seed
+    p.num_gaussians = n
+    p.kind = 0 if kind == "uniform" else 1
+    p.sh_degree = sh_degree
+    p.clusters = clusters
+    p.half_extent = half_extent
+    p.cluster_sigma = cluster_sigma
+    p.log_scale_mu = log_scale_mu
+    p.log_scale_sigma = log_scale_sigma
+    p.opacity_sigma = opacity_sigma
+    p.dc_sigma = dc_sigma
+    p.rest_sigma = rest_sigma
+    for i in range(3):
+        p.offset[i] = float(offset[i])
+    return p
+
+
+def synth_scene(p: SynthParams, first: int = 0, count: Optional[int] = None) -> SceneArrays:
+    """Generate Gaussians [first, first+count) of a synthetic scene."""
+    n = int(p.num_gaussians - first) if count is None else int(count)
+    nc = (p.sh_degree + 1) ** 2
+    means = np.empty((n, 3), np.float32)
+    rots = np.empty((n, 4), np.float32)
+    scales = np.empty((n, 3), np.float32)
+    ops = np.empty((n,), np.float32)
+    shs = np.empty((n, 3 * nc), np.float32)
+    check(_lib.gps_synth_generate(C.byref(p), first, n, _ptr(means), _ptr(rots), _ptr(scales), _ptr(ops),
+                                  _ptr(shs)), "synth")
+    return SceneArrays(means, rots, scales, ops, shs, int(p.sh_degree))
+
+
+def encode_half_fp8(s: SceneArrays) -> dict:
+    """Host encoding of the compact format (the bytes the GPU scene holds)."""
+    n = s.n
+    out = {"rotations": np.empty((n, 4), np.uint16), "scales": np.empty((n, 3), np.uint16),
+           "opacities": np.empty((n,), np.uint16), "sh_dc": np.empty((n, 3), np.uint16),
+           "sh_rest": np.empty((n, 45), np.uint8), "sh_scale": np.empty((n,), np.float32)}
+    d = s.desc(FORMAT_HALF_FP8)
+    check(_lib.gps_encode_half_fp8(C.byref(d), *[_ptr(out[k]) for k in
+                                                 ("rotations", "scales", "opacities", "sh_dc", "sh_rest",
+                                                  "sh_scale")]), "encode")
+    return out
+
+
+class Scene:
+    """A scene resident in HBM."""
+
+    def __init__(self, arrays: SceneArrays, fmt: int = FORMAT_FP32):
+        h = C.c_void_p()
+        d = arrays.desc(fmt)
+        check(_lib.gps_scene_create(C.byref(d), C.byref(h)), "scene create")
+        self.handle = h
+        self.n = arrays.n
+        self.format = fmt
+        self.sh_degree = arrays.sh_degree
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.gps_scene_device_bytes(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.gps_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class RenderSettings:
+    """The create-time RenderSettings fields (src/render_settings.cuh:36-88)."""
+    width: int = 1920
+    height: int = 1080
+    render_pass_count: int = 1
+    supersampling_factor: int = 2
+    max_number_of_points: int = 250_000_000
+    fully_disable_hierarchical_culling: bool = False
+    gaussians_per_block: int = 0
+
+    def c(self) -> RenderSettingsC:
+        s = RenderSettingsC()
+        s.width, s.height = self.width, self.height
+        s.render_pass_count = self.render_pass_count
+        s.supersampling_factor = self.supersampling_factor
+        s.max_number_of_points = self.max_number_of_points
+        s.fully_disable_hierarchical_culling = int(self.fully_disable_hierarchical_culling)
+        s.gaussians_per_block = self.gaussians_per_block
+        return s
+
+
+class Pass:
+    """GaussianPointSplatting render pass over a resident scene."""
+
+    def __init__(self, settings: RenderSettings, scene: Scene):
+        h = C.c_void_p()
+        self._settings_c = settings.c()
+        check(_lib.gps_pass_create(C.byref(self._settings_c), scene.handle, C.byref(h)), "pass create")
+        self.handle = h
+        self.settings = settings
+        self.scene = scene
+
+    def render(self, frame: FrameParams, color: DeviceArray, depth: DeviceArray,
+               stream: Optional[Stream] = None) -> None:
+        check(_lib.gps_pass_render(self.handle, C.byref(frame), color.ptr, depth.ptr,
+                                   stream.handle if stream else None), "pass render")
+
+    def stats(self, stream: Optional[Stream] = None) -> dict:
+        s = Stats()
+        check(_lib.gps_pass_stats(self.handle, C.byref(s), stream.handle if stream else None), "stats")
+        return s.as_dict()
+
+    def debug(self, which: int, stream: Optional[Stream] = None) -> np.ndarray:
+        n = int(_lib.gps_pass_debug_size(self.handle, which))
+        dt = _DBG_DTYPE.get(which, np.uint32)
+        out = np.empty((n,), dt)
+        check(_lib.gps_pass_debug_copy(self.handle, which, _ptr(out), n, stream.handle if stream else None),
+              f"debug copy {which}")
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.gps_pass_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- camera
+def look_at(eye, center, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    out = np.empty(16, np.float32)
+    e, ep = _f(eye)
+    c, cp = _f(center)
+    u, up_ = _f(up)
+    _lib.gps_look_at(ep, cp, up_, out.ctypes.data_as(_fp))
+    return out
+
+
+def perspective(fovy_deg, aspect, near, far) -> np.ndarray:
+    out = np.empty(16, np.float32)
+    _lib.gps_perspective(fovy_deg, aspect, near, far, out.ctypes.data_as(_fp))
+    return out
+
+
+def mat4_mul(a, b) -> np.ndarray:
+    out = np.empty(16, np.float32)
+    a_, ap = _f(a)
+    b_, bp = _f(b)
+    _lib.gps_mat4_mul(ap, bp, out.ctypes.data_as(_fp))
+    return out
+
+
+def taa_reprojection(prev_vp, view, proj) -> np.ndarray:
+    out = np.empty(16, np.float32)
+    a, ap = _f(prev_vp)
+    v, vp = _f(view)
+    p, pp = _f(proj)
+    _lib.gps_taa_reprojection(ap, vp, pp, out.ctypes.data_as(_fp))
+    return out
+
+
+def frame_params(eye, center, up=(0.0, 1.0, 0.0), fovy: float = 45.0, width: int = 1280, height: int = 720,
+                 ss: int = 1, near: float = 0.2, far: float = 1e4, frame: int = 1, occlusion: bool = False,
+                 hierarchical: bool = False, unbiased: bool = True, reduce_point_count: bool = True,
+                 cull_small: bool = False, background=(0.0, 0.0, 0.0), min_reduce_count: int = 0) -> FrameParams:
+    """Frame inputs for a look-at camera (view = glm::lookAt, proj = glm::perspective)."""
+    f = FrameParams()
+    e, ep = _f(eye)
+    c, cp = _f(center)
+    u, up_ = _f(up)
+    _lib.gps_frame_params_init(C.byref(f), ep, cp, up_, fovy, width, height, ss, near, far)
+    f.frame = frame
+    f.enable_occlusion_culling = int(occlusion)
+    f.enable_hierarchical_culling = int(hierarchical)
+    f.use_unbiased_2d_splatting = int(unbiased)
+    f.reduce_point_count = int(reduce_point_count)
+    f.cull_small_gaussians = int(cull_small)
+    f.occlusion_culling_min_reduce_count = min_reduce_count
+    for i in range(3):
+        f.background[i] = background[i]
+    return f
+
+
+# ----------------------------------------------------------------- resolve
+def resolve_taa(cur_color: DeviceArray, cur_depth: DeviceArray, prev_color: DeviceArray, prev_depth: DeviceArray,
+                reprojection, blend: float, width: int, height: int, frame_number: int, out_color: DeviceArray,
+                out_rgb8: DeviceArray, stream: Optional[Stream] = None) -> None:
+    m, mp = _f(reprojection)
+    check(_lib.gps_resolve_taa(cur_color.ptr, cur_depth.ptr, prev_color.ptr, prev_depth.ptr, mp, blend, width,
+                               height, frame_number, out_color.ptr, out_rgb8.ptr,
+                               stream.handle if stream else None), "taa")
+
+
+def resolve_progressive(color: DeviceArray, accum: DeviceArray, width: int, height: int, num_frames: int,
+                        out_rgb8: DeviceArray, stream: Optional[Stream] = None) -> None:
+    check(_lib.gps_resolve_progressive(color.ptr, accum.ptr, width, height, num_frames, out_rgb8.ptr,
+                                       stream.handle if stream else None), "progressive")

@@ -1,0 +1,292 @@
+/*
+ * gps.h — C ABI of the MI355X-native Gaussian point splatting hot path.
+ *
+ * One frame of the reference renderer is
+ *   clear -> [block cull] -> preprocess -> workload distribution -> splat
+ *   -> [depth combine -> Hi-Z -> second pass] -> combine -> [Hi-Z]
+ *   -> TAA / progressive resolve
+ * (reference: src/core/rendering/passes/gaussian_point_splatting.cu:599-737 and
+ *  src/core/rendering/render_server.cu:86-157).  This header exposes that path
+ * as plain C: POD structs, opaque handles, device pointers as void*, status
+ * codes instead of exceptions.  Streams are hipStream_t passed as void*
+ * (NULL = the null stream).  No call inside gps_pass_render() or gps_resolve()
+ * synchronizes the host, so both can be captured into a hipGraph.
+ *
+ * Every entry point names the reference interface it replaces.
+ */
+#ifndef GPS_H
+#define GPS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPS_ABI_VERSION 1
+
+typedef enum gps_status {
+    GPS_OK = 0,
+    GPS_ERR_INVALID_ARG = 1,
+    GPS_ERR_OUT_OF_MEMORY = 2,
+    GPS_ERR_HIP = 3,
+    GPS_ERR_UNSUPPORTED = 4,
+    GPS_ERR_IO = 5,
+    GPS_ERR_NOT_READY = 6
+} gps_status;
+
+/* Storage format of the resident scene. */
+typedef enum gps_scene_format {
+    /* fp32 everything; the reference's GPUGaussianSceneUncompressed
+       (src/core/gaussians/gaussian_types.cuh:548-646). */
+    GPS_FORMAT_FP32 = 0,
+    /* fp32 positions, fp16 scale / rotation / opacity, fp8-E4M3 SH with one
+       fp32 scale per Gaussian.  Project-defined compact format. */
+    GPS_FORMAT_HALF_FP8 = 1
+} gps_scene_format;
+
+/* Host-side scene description, one record per Gaussian in structure-of-arrays
+ * form.  Mirrors CPUGaussian (src/core/gaussians/gaussian_types.cuh:89-100):
+ *   means      N*3  world position
+ *   rotations  N*4  quaternion stored (w, x, y, z)  (gaussian_loader.cu:214-220)
+ *   scales     N*3  linear (post-exp) scale
+ *   opacities  N    post-sigmoid opacity
+ *   shs        N*3*(sh_degree+1)^2  3DGS SH layout: coefficient-major, RGB
+ *              interleaved (sh[3*i + c]); coefficients above sh_degree are 0.
+ * The arrays are copied; the caller keeps ownership. */
+typedef struct gps_scene_desc {
+    uint64_t num_gaussians;
+    int32_t sh_degree; /* 0..3 */
+    int32_t format;    /* gps_scene_format */
+    const float* means;
+    const float* rotations;
+    const float* scales;
+    const float* opacities;
+    const float* shs;
+} gps_scene_desc;
+
+typedef struct gps_scene gps_scene;
+typedef struct gps_pass gps_pass;
+
+/* Create-time settings of a pass; the fields of RenderSettings the
+ * GaussianPointSplatting constructor reads (gaussian_point_splatting.cu:777-816,
+ * src/render_settings.cuh:36-88). */
+typedef struct gps_render_settings {
+    int32_t width, height;               /* output resolution (Hi-Z size) */
+    int32_t render_pass_count;           /* samples per pixel ("spp"), >= 1 */
+    int32_t supersampling_factor;        /* >= 1 */
+    int64_t max_number_of_points;        /* point budget (default 250e6) */
+    int32_t fully_disable_hierarchical_culling;
+    int32_t gaussians_per_block;         /* culling block size; 0 = reference
+                                            rule max(4, floor(N / 1e6)) */
+} gps_render_settings;
+
+/* Per-frame inputs; the RenderSettings fields read by
+ * GaussianPointSplatting::render (gaussian_point_splatting.cu:599-737).
+ * Matrices are column-major 4x4 (glm layout). */
+typedef struct gps_frame_params {
+    float view[16];            /* cameraViewMatrix */
+    float proj[16];            /* cameraProjectionMatrix */
+    float frustum_planes[24];  /* 6 normalized planes (a,b,c,d): L,R,B,T,N,F */
+    float fovy_degrees;
+    float near_plane, far_plane;
+    float background[3];
+    int32_t width, height;
+    int32_t supersampling_factor;
+    int32_t frame;             /* settings.frame; RNG frames are 4*frame+{0..3} */
+    int32_t enable_occlusion_culling;
+    int32_t enable_hierarchical_culling;
+    int32_t cull_small_gaussians;
+    int32_t use_unbiased_2d_splatting;
+    int32_t reduce_point_count;
+    int32_t occlusion_culling_min_reduce_count;
+} gps_frame_params;
+
+/* Counters of the last rendered frame (Statistics, render_settings.cuh:23-34,
+ * plus the work units the benchmark reports). */
+typedef struct gps_stats {
+    uint64_t num_gaussians;
+    uint64_t valid_gaussians[2];   /* per render pass (pass 2 only with occlusion culling) */
+    uint64_t total_weight[2];      /* uncapped sum of point counts per pass */
+    uint64_t num_points[2];        /* points splatted per pass (after the budget cap) */
+    uint64_t samples;              /* sum over passes of points * K * spp */
+    uint64_t culled_blocks[2];
+    int32_t passes;                /* 1 or 2 */
+    int32_t budget_truncated;      /* 1 if the point budget cut points */
+    int32_t K;                     /* samples per point per layer */
+    int32_t reserved;
+} gps_stats;
+
+/* -------------------------------------------------------------- versioning */
+int32_t gps_abi_version(void);
+const char* gps_status_str(gps_status s);
+/* Last HIP error string recorded by the library in this thread. */
+const char* gps_last_error(void);
+
+/* ------------------------------------------------------------------ scenes */
+/* Uploads a scene into HBM.  Replaces GPUGaussianScene*::upload_from_cpu
+ * (gaussian_types.cuh:442-503, 568-608). */
+gps_status gps_scene_create(const gps_scene_desc* desc, gps_scene** out);
+void gps_scene_destroy(gps_scene* scene);
+/* Device bytes held by the scene. */
+uint64_t gps_scene_device_bytes(const gps_scene* scene);
+
+/* ------------------------------------------------------------------ passes */
+/* Replaces GaussianPointSplatting::GaussianPointSplatting(const RenderSettings&)
+ * (gaussian_point_splatting.cu:777-816) for an already-loaded scene.  The scene
+ * must outlive the pass; several passes may share one scene. */
+gps_status gps_pass_create(const gps_render_settings* settings, gps_scene* scene, gps_pass** out);
+void gps_pass_destroy(gps_pass* pass);
+
+/* Replaces RenderPass::render(glm::vec3* d_image, float* d_depth, const
+ * RenderSettings&) (src/core/rendering/render_pass.cuh:17-24) as implemented
+ * by GaussianPointSplatting::render (gaussian_point_splatting.cu:599-737).
+ * d_color: device float[W*H*3] average linear RGB; d_depth: device float[W*H]
+ * minimum NDC depth.  Enqueued on `stream`; returns without waiting. */
+gps_status gps_pass_render(gps_pass* pass, const gps_frame_params* frame,
+                           float* d_color, float* d_depth, void* stream);
+
+/* Statistics of the last frame; synchronizes `stream`. */
+gps_status gps_pass_stats(gps_pass* pass, gps_stats* out, void* stream);
+
+/* Test/inspection access to the pass's internal device state after a render,
+ * copied to host memory (synchronizes `stream`).  `count` is in elements and
+ * must not exceed gps_pass_debug_size(). */
+typedef enum gps_debug_buffer {
+    GPS_DBG_WEIGHTS = 0,        /* uint32[N]: point count of valid Gaussians, 0 otherwise (last pass) */
+    GPS_DBG_MASK_CURRENT = 1,   /* uint32[ceil(N/32)] valid-mask of the last pass */
+    GPS_DBG_MASK_PREVIOUS = 2,  /* uint32[ceil(N/32)] valid-mask of the pass before */
+    GPS_DBG_VALID_IDS = 3,      /* uint32[V] ids of valid Gaussians, ascending */
+    GPS_DBG_VALID_OFFSETS = 4,  /* uint32[V] exclusive point offsets of those ids */
+    GPS_DBG_INDICES = 5,        /* uint32[P] Gaussian id of every point (materialized on request) */
+    GPS_DBG_FRAMEBUFFER = 6,    /* uint64[F] framebuffer words of the current frame */
+    GPS_DBG_RECORDS = 7,        /* 16 x uint32 per Gaussian: cached splat parameters */
+    GPS_DBG_HIZ = 8,            /* float[mip chain] Hi-Z pyramid, coarsest level first */
+    GPS_DBG_CULL_MASK = 9       /* uint32[ceil(blocks/32)] culled-block mask of the last pass */
+} gps_debug_buffer;
+uint64_t gps_pass_debug_size(gps_pass* pass, int32_t which);
+gps_status gps_pass_debug_copy(gps_pass* pass, int32_t which, void* host_dst, uint64_t count, void* stream);
+
+/* ---------------------------------------------------------------- resolve */
+typedef enum gps_accumulation_mode {
+    GPS_ACCUM_PROGRESSIVE = 0,
+    GPS_ACCUM_TAA = 1,
+    GPS_ACCUM_NONE = 2
+} gps_accumulation_mode;
+
+/* TAA reprojection (post_processing_kernels.cuh:222-285, launched at
+ * render_server.cu:126-142).  Reads cur_color (this frame's combine output)
+ * and its 3x3 neighbourhood, reprojects into the history (prev_color /
+ * prev_depth) with `reprojection` (column-major prevVP * affineInverse(view) *
+ * inverse(proj)), blends 0.9 towards the clamped history, writes the blended
+ * colour to out_color and RGB8 to out_rgb8.  frame_number > 0 enables the
+ * blend.  out_color must not alias cur_color (the reference blends in place;
+ * here the neighbourhood is always the un-blended frame). */
+gps_status gps_resolve_taa(const float* cur_color, const float* cur_depth,
+                           const float* prev_color, const float* prev_depth,
+                           const float reprojection[16], float blend_factor,
+                           int32_t width, int32_t height, int32_t frame_number,
+                           float* out_color, uint8_t* out_rgb8, void* stream);
+
+/* Progressive accumulation (post_processing_kernels.cuh:108-140, launched at
+ * render_server.cu:143-156): color += accum (if num_frames > 1), accum = 0,
+ * rgb8 = round(clamp(color * 255 / num_frames)). Updates `color` in place. */
+gps_status gps_resolve_progressive(float* color, float* accum, int32_t width, int32_t height,
+                                   int32_t num_frames, uint8_t* out_rgb8, void* stream);
+
+/* ----------------------------------------------------------- camera math */
+/* Host helpers reproducing the reference's camera conventions (glm 1.0.2
+ * right-handed, NDC z in [-1, 1]). */
+/* glm::lookAtRH (glm/ext/matrix_transform.inl:153-173). */
+void gps_look_at(const float eye[3], const float center[3], const float up[3], float out_view[16]);
+/* glm::perspective(radians(fovy), aspect, near, far) (src/world/camera.cpp:96-98). */
+void gps_perspective(float fovy_degrees, float aspect, float near_plane, float far_plane, float out_proj[16]);
+/* Camera::getFrustumPlanesFromViewMatrix (src/world/camera.cpp:126-155). */
+void gps_frustum_planes(float fovy_degrees, const float view[16], float aspect, float near_plane,
+                        float far_plane, float out_planes[24]);
+/* Fills the camera-derived fields of `frame` (view, proj, planes, fovy,
+ * near/far, width/height, ss) from a look-at camera, with the reference's
+ * defaults for every flag (unbiased splatting on, point reduction on, both
+ * culls on). */
+void gps_frame_params_init(gps_frame_params* frame, const float eye[3], const float center[3],
+                           const float up[3], float fovy_degrees, int32_t width, int32_t height,
+                           int32_t supersampling_factor, float near_plane, float far_plane);
+/* deltaMatrix of the TAA pass (render_server.cu:128-129). */
+void gps_taa_reprojection(const float prev_view_proj[16], const float view[16], const float proj[16],
+                          float out[16]);
+/* proj * view, the previousViewProjectionMatrix of the next frame (main.cu:176). */
+void gps_mat4_mul(const float a[16], const float b[16], float out[16]);
+
+/* -------------------------------------------------------- synthetic scenes */
+/* Deterministic synthetic scenes (splitmix64 + FP64 Box-Muller):
+ *   kind 0: uniform box   positions U([-half,half]^3), log-scale N(mu, sigma)
+ *   kind 1: clustered     `clusters` centres U([-half,half]^3), members
+ *                          N(centre, cluster_sigma^2)
+ * Quaternions normalize(N(0,1)^4); opacity sigmoid(N(0, opacity_sigma));
+ * SH DC N(0, dc_sigma), higher bands N(0, rest_sigma) up to sh_degree.
+ * Arrays sized as in gps_scene_desc; index order = generation order. */
+typedef struct gps_synth_params {
+    uint64_t seed;
+    uint64_t num_gaussians;
+    int32_t kind;
+    int32_t sh_degree;
+    int32_t clusters;
+    int32_t reserved;
+    double half_extent;
+    double cluster_sigma;
+    double log_scale_mu, log_scale_sigma;
+    double opacity_sigma;
+    double dc_sigma, rest_sigma;
+    double offset[3];   /* translation added to every position */
+} gps_synth_params;
+gps_status gps_synth_generate(const gps_synth_params* p, uint64_t first, uint64_t count, float* means,
+                              float* rotations, float* scales, float* opacities, float* shs);
+
+/* ----------------------------------------------------- compact encoding */
+/* Host encoder of GPS_FORMAT_HALF_FP8 (the same code gps_scene_create uses):
+ * fp16 RNE rotation (N*4), scale (N*3), opacity (N), SH DC (N*3); the
+ * (deg+1)^2-1 higher SH coefficients per channel as FP8 E4M3 RNE codes of
+ * c / sh_scale (N*45, coefficient-major, rgb interleaved, zero-padded) with
+ * sh_scale = max|c| / 448 (1 if all are zero). */
+gps_status gps_encode_half_fp8(const gps_scene_desc* desc, uint16_t* rotations, uint16_t* scales, uint16_t* opacities,
+                               uint16_t* sh_dc, uint8_t* sh_rest, float* sh_scale);
+uint16_t gps_float_to_half(float f);
+uint8_t gps_float_to_fp8e4m3(float f);
+
+/* ---------------------------------------------------------- self-tests */
+/* Device evaluation of the shared deterministic scalar math and RNG, used to
+ * prove host/device bit-exactness.  No reference counterpart (diagnostics).
+ * kind: 0 exp, 1 log, 2 sin, 3 cos, 4 sqrt, 5 reciprocal, 6 dilog,
+ * 7 inv_dilog, 8 float->int round-to-nearest-even (saturating),
+ * 9 Box-Muller x of (in[i], in[i+1]), 10 corrected Box-Muller y (alpha 0.7). */
+gps_status gps_selftest_math(int32_t kind, const float* d_in, float* d_out, uint64_t n, void* stream);
+/* Per item i: make_seed(i, frame) then `draws` pcg2d states (2+2*draws u32). */
+gps_status gps_selftest_rng(uint32_t n, uint32_t frame, int32_t draws, uint32_t* d_out, void* stream);
+/* Per item i: poisson(make_seed(i, 12345), lambda[i]). */
+gps_status gps_selftest_poisson(uint32_t n, const float* d_lambda, uint32_t* d_out, void* stream);
+
+/* ------------------------------------------------------- device plumbing */
+/* Minimal HIP plumbing for hosts without a HIP binding (ctypes, cgo, JNI). */
+gps_status gps_device_count(int32_t* out);
+gps_status gps_set_device(int32_t device);
+gps_status gps_device_malloc(void** out, uint64_t bytes);
+gps_status gps_device_free(void* ptr);
+gps_status gps_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream);
+gps_status gps_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream);
+gps_status gps_memset_d8(void* dst, int32_t value, uint64_t bytes, void* stream);
+gps_status gps_stream_create(void** out);
+gps_status gps_stream_destroy(void* stream);
+gps_status gps_stream_synchronize(void* stream);
+gps_status gps_device_free_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+/* Event-based timing on a stream. */
+gps_status gps_event_create(void** out);
+gps_status gps_event_destroy(void* ev);
+gps_status gps_event_record(void* ev, void* stream);
+gps_status gps_event_elapsed_ms(void* start, void* stop, float* out_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPS_H */

@@ -1,0 +1,84 @@
+/*
+ * gps_oracle.h — C API of the CPU oracle (plain C++ transcription of the
+ * reference's per-frame hot path).
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, bench.py's CPU-baseline leg and the
+ * smoke check load this library, and only as the checker / CPU baseline; the
+ * product never links it.
+ *
+ * The oracle implements the same POD structs as include/gps.h so a test can
+ * drive both with identical inputs.
+ */
+#ifndef GPS_ORACLE_H
+#define GPS_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/gps.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_pass orc_pass;
+
+/* Encoded GPS_FORMAT_HALF_FP8 scene as produced by gps_encode_half_fp8(). */
+typedef struct orc_half_fp8_scene {
+    uint64_t num_gaussians;
+    int32_t sh_degree;
+    int32_t reserved;
+    const float* means;        /* N*3 */
+    const uint16_t* rotations; /* N*4 fp16 (w,x,y,z) */
+    const uint16_t* scales;    /* N*3 fp16 */
+    const uint16_t* opacities; /* N   fp16 */
+    const uint16_t* sh_dc;     /* N*3 fp16 */
+    const uint8_t* sh_rest;    /* N*45 fp8 E4M3 */
+    const float* sh_scale;     /* N */
+} orc_half_fp8_scene;
+
+orc_pass* orc_pass_create(const gps_render_settings* settings, const gps_scene_desc* scene, int32_t nthreads);
+orc_pass* orc_pass_create_half_fp8(const gps_render_settings* settings, const orc_half_fp8_scene* scene,
+                                   int32_t nthreads);
+void orc_pass_destroy(orc_pass* pass);
+void orc_pass_set_threads(orc_pass* pass, int32_t nthreads);
+int32_t orc_pass_render(orc_pass* pass, const gps_frame_params* frame, float* color, float* depth);
+int32_t orc_pass_stats(orc_pass* pass, gps_stats* out);
+uint64_t orc_pass_debug_size(orc_pass* pass, int32_t which);
+int32_t orc_pass_debug_copy(orc_pass* pass, int32_t which, void* dst, uint64_t count);
+/* Per-stage wall times (ms) of the last render: clear, cull, preprocess,
+ * distribute, splat, combine, hiz. */
+void orc_pass_stage_ms(orc_pass* pass, double out[8]);
+
+/* Per-Gaussian projection in FP32 or FP64 for the accuracy checks.  Writes
+ * N x 16 doubles: ok, view_z, pixel_mean.x, pixel_mean.y, cov2d.a, cov2d.b,
+ * cov2d.c, color.r, color.g, color.b, importance, ndc_z, bounds min x/y,
+ * max x/y (FP32 only).  ok = 1 if the
+ * Gaussian passes projection + bounds; colour is evaluated for every such
+ * Gaussian. */
+int32_t orc_project(orc_pass* pass, const gps_frame_params* frame, int32_t use_fp64, double* out);
+
+/* Resolve kernels (post_processing_kernels.cuh). */
+void orc_resolve_taa(const float* cur_color, const float* cur_depth, const float* prev_color,
+                     const float* prev_depth, const float reprojection[16], float blend_factor, int32_t width,
+                     int32_t height, int32_t frame_number, float* out_color, uint8_t* out_rgb8);
+void orc_resolve_progressive(float* color, float* accum, int32_t width, int32_t height, int32_t num_frames,
+                             uint8_t* out_rgb8);
+
+/* Scalar probes for known-answer tests. */
+void orc_rng_probe(uint32_t idx, uint32_t frame, int32_t draws, uint32_t* out_states /* 2 + 2*draws */);
+float orc_dilog(float x);
+float orc_inv_dilog(float x);
+void orc_corrected_box_muller(float u1, float u2, float alpha, float* out2);
+uint32_t orc_poisson(uint32_t s0, uint32_t s1, float lambda);
+int32_t orc_stochastic_round(float x, float u);
+int32_t orc_pos_to_memory(int32_t x, int32_t y, int32_t p, int32_t num_passes, int32_t width, int32_t height);
+uint64_t orc_depth_bits(float view_z, float near_plane, float far_plane);
+/* Shared deterministic math as the oracle evaluates it (kinds as gps_selftest_math). */
+void orc_detmath(int32_t kind, const float* in, float* out, uint64_t n);
+/* computeColorFromSHFloat (full degree 3) at direction dir, FP32 or FP64. */
+void orc_sh_eval(const float dir[3], const float* coeffs48, int32_t fp64, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,191 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: only tests/, bench.py's CPU-baseline leg and the smoke
+check import this module, and only as the checker.  The oracle is a plain C++
+transcription of the reference's per-frame hot path (see gps_oracle.cpp).
+It shares the POD structs of include/gps.h with the product so both can be
+driven with identical inputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+if not os.path.exists(LIB_PATH):
+    build()
+
+_lib = C.CDLL(LIB_PATH)
+
+# The product binding defines the shared structs; import it for the layouts
+# only (no GPU call happens here).
+sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "gaussian-point-splatting_amd"))
+import gps_amd  # noqa: E402
+from gps_amd import FrameParams, RenderSettingsC, SceneDesc, Stats  # noqa: E402,F401
+
+
+class HalfFp8Scene(C.Structure):
+    _fields_ = [("num_gaussians", C.c_uint64), ("sh_degree", C.c_int32), ("reserved", C.c_int32),
+                ("means", C.c_void_p), ("rotations", C.c_void_p), ("scales", C.c_void_p),
+                ("opacities", C.c_void_p), ("sh_dc", C.c_void_p), ("sh_rest", C.c_void_p),
+                ("sh_scale", C.c_void_p)]
+
+
+_vp = C.c_void_p
+
+
+def _sig(name, res, *args):
+    fn = getattr(_lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_sig("orc_pass_create", _vp, C.POINTER(RenderSettingsC), C.POINTER(SceneDesc), C.c_int32)
+_sig("orc_pass_create_half_fp8", _vp, C.POINTER(RenderSettingsC), C.POINTER(HalfFp8Scene), C.c_int32)
+_sig("orc_pass_destroy", None, _vp)
+_sig("orc_pass_set_threads", None, _vp, C.c_int32)
+_sig("orc_pass_render", C.c_int32, _vp, C.POINTER(FrameParams), _vp, _vp)
+_sig("orc_pass_stats", C.c_int32, _vp, C.POINTER(Stats))
+_sig("orc_pass_debug_size", C.c_uint64, _vp, C.c_int32)
+_sig("orc_pass_debug_copy", C.c_int32, _vp, C.c_int32, _vp, C.c_uint64)
+_sig("orc_pass_stage_ms", None, _vp, C.POINTER(C.c_double))
+_sig("orc_project", C.c_int32, _vp, C.POINTER(FrameParams), C.c_int32, _vp)
+_sig("orc_resolve_taa", None, _vp, _vp, _vp, _vp, C.POINTER(C.c_float), C.c_float, C.c_int32, C.c_int32, C.c_int32,
+     _vp, _vp)
+_sig("orc_resolve_progressive", None, _vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp)
+_sig("orc_rng_probe", None, C.c_uint32, C.c_uint32, C.c_int32, _vp)
+_sig("orc_dilog", C.c_float, C.c_float)
+_sig("orc_inv_dilog", C.c_float, C.c_float)
+_sig("orc_corrected_box_muller", None, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float))
+_sig("orc_poisson", C.c_uint32, C.c_uint32, C.c_uint32, C.c_float)
+_sig("orc_stochastic_round", C.c_int32, C.c_float, C.c_float)
+_sig("orc_pos_to_memory", C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32)
+_sig("orc_depth_bits", C.c_uint64, C.c_float, C.c_float, C.c_float)
+
+_sig("orc_detmath", None, C.c_int32, _vp, _vp, C.c_uint64)
+_sig("orc_sh_eval", None, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_double))
+
+lib = _lib
+
+
+def detmath(kind: int, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    _lib.orc_detmath(kind, _ptr(x), _ptr(out), x.size)
+    return out
+
+
+def sh_eval(direction, coeffs48, fp64: bool) -> np.ndarray:
+    d = np.ascontiguousarray(direction, np.float32)
+    c = np.ascontiguousarray(coeffs48, np.float32)
+    out = (C.c_double * 3)()
+    _lib.orc_sh_eval(d.ctypes.data_as(C.POINTER(C.c_float)), c.ctypes.data_as(C.POINTER(C.c_float)), int(fp64), out)
+    return np.array(list(out))
+
+_DTYPE = {gps_amd.DBG_FRAMEBUFFER: np.uint64, gps_amd.DBG_HIZ: np.float32}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OraclePass:
+    """CPU transcription of GaussianPointSplatting over a host scene."""
+
+    def __init__(self, settings: "gps_amd.RenderSettings", scene: "gps_amd.SceneArrays", fmt: int = 0,
+                 threads: int = 1):
+        self._s = settings.c()
+        self._keep = scene
+        if fmt == gps_amd.FORMAT_FP32:
+            d = scene.desc(fmt)
+            self.handle = _lib.orc_pass_create(C.byref(self._s), C.byref(d), threads)
+        else:
+            enc = gps_amd.encode_half_fp8(scene)
+            self._enc = enc
+            h = HalfFp8Scene()
+            h.num_gaussians = scene.n
+            h.sh_degree = scene.sh_degree
+            h.means = _ptr(scene.means)
+            for k in ("rotations", "scales", "opacities", "sh_dc", "sh_rest", "sh_scale"):
+                setattr(h, k, _ptr(enc[k]))
+            self.handle = _lib.orc_pass_create_half_fp8(C.byref(self._s), C.byref(h), threads)
+        if not self.handle:
+            raise RuntimeError("oracle pass create failed")
+        self.width = settings.width
+        self.height = settings.height
+
+    def set_threads(self, n: int) -> None:
+        _lib.orc_pass_set_threads(self.handle, n)
+
+    def render(self, frame: FrameParams):
+        color = np.zeros((self.height, self.width, 3), np.float32)
+        depth = np.zeros((self.height, self.width), np.float32)
+        st = _lib.orc_pass_render(self.handle, C.byref(frame), _ptr(color), _ptr(depth))
+        if st != 0:
+            raise RuntimeError(f"oracle render failed: {st}")
+        return color, depth
+
+    def stats(self) -> dict:
+        s = Stats()
+        _lib.orc_pass_stats(self.handle, C.byref(s))
+        return s.as_dict()
+
+    def stage_ms(self) -> list:
+        out = (C.c_double * 8)()
+        _lib.orc_pass_stage_ms(self.handle, out)
+        return list(out)
+
+    def debug(self, which: int) -> np.ndarray:
+        n = int(_lib.orc_pass_debug_size(self.handle, which))
+        out = np.empty((n,), _DTYPE.get(which, np.uint32))
+        st = _lib.orc_pass_debug_copy(self.handle, which, _ptr(out), n)
+        if st != 0:
+            raise RuntimeError(f"oracle debug copy {which} failed: {st}")
+        return out
+
+    def project(self, frame: FrameParams, fp64: bool) -> np.ndarray:
+        n = self._keep.n
+        out = np.zeros((n, 16), np.float64)
+        _lib.orc_project(self.handle, C.byref(frame), int(fp64), _ptr(out))
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.orc_pass_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def resolve_taa(cur, cur_depth, prev, prev_depth, reprojection, blend, width, height, frame_number):
+    out = np.empty_like(cur)
+    rgb = np.empty(cur.shape, np.uint8)
+    m = np.ascontiguousarray(reprojection, np.float32)
+    _lib.orc_resolve_taa(_ptr(cur), _ptr(cur_depth), _ptr(prev), _ptr(prev_depth),
+                         m.ctypes.data_as(C.POINTER(C.c_float)), blend, width, height, frame_number, _ptr(out),
+                         _ptr(rgb))
+    return out, rgb
+
+
+def resolve_progressive(color, accum, width, height, num_frames):
+    color = np.ascontiguousarray(color, np.float32).copy()
+    accum = np.ascontiguousarray(accum, np.float32).copy()
+    rgb = np.empty(color.shape, np.uint8)
+    _lib.orc_resolve_progressive(_ptr(color), _ptr(accum), width, height, num_frames, _ptr(rgb))
+    return color, accum, rgb

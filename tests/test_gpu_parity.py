@@ -1,0 +1,247 @@
+"""GPU parity: the MI355X kernels against the CPU oracle, through the C ABI.
+
+Integer work (weights, valid masks, compacted offsets, per-point Gaussian
+indices, framebuffer words) must be bit-exact; because the scalar math is the
+shared deterministic header, projection results and final colour/depth images
+are bit-exact too (tolerance 0 ulp, PSNR = inf).  Sizes are chosen so the
+oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+from helpers import assert_same, flush
+
+pytestmark = pytest.mark.gpu
+
+
+# ------------------------------------------------------------------ scalar math
+@pytest.mark.parametrize("kind,lo,hi", [(0, -110.0, 90.0), (1, 1e-38, 1e4), (2, -7.0, 7.0), (3, -7.0, 7.0),
+                                        (4, 0.0, 1e6), (5, -1e3, 1e3), (6, 0.0, 1.0), (7, 0.0, 1.7),
+                                        (8, -1e10, 1e10), (9, 0.0, 1.0), (10, 0.0, 1.0)])
+def test_device_math_bitexact(gpu, kind, lo, hi):
+    import orc
+    flush(f"device math kind={kind}")
+    rng = np.random.default_rng(kind)
+    x = rng.uniform(lo, hi, 200_000).astype(np.float32)
+    if kind == 1:
+        x = np.exp(rng.uniform(np.log(1e-38), np.log(1e30), 200_000)).astype(np.float32)
+    specials = np.array([0.0, -0.0, 1.0, 0.5, 2.0 ** -126, 2.0 ** -149, 1e-37, 0.99999994], np.float32)
+    x[:specials.size] = np.clip(specials, lo, hi)
+    d_in = gpu.DeviceArray(x.shape, np.float32)
+    d_out = gpu.DeviceArray(x.shape, np.float32)
+    d_in.upload(x)
+    gpu.check(gpu.lib.gps_selftest_math(kind, d_in.ptr, d_out.ptr, x.size, None), "selftest")
+    dev = d_out.download()
+    host = orc.detmath(kind, x)
+    assert_same(f"math kind {kind}", dev, host)
+
+
+def test_device_rng_bitexact(gpu):
+    import orc
+    n, draws, frame = 4096, 6, 4 * 7 + 1
+    d = gpu.DeviceArray((n * (2 + 2 * draws),), np.uint32)
+    gpu.check(gpu.lib.gps_selftest_rng(n, frame, draws, d.ptr, None), "rng")
+    dev = d.download().reshape(n, -1)
+    host = np.empty_like(dev)
+    for i in range(n):
+        orc.lib.orc_rng_probe(i, frame, draws, host[i].ctypes.data_as(orc.C.c_void_p))
+    assert_same("rng", dev, host)
+
+
+def test_device_poisson_bitexact(gpu):
+    import orc
+    n = 20000
+    lam = np.exp(np.random.default_rng(3).uniform(np.log(1e-6), np.log(1e6), n)).astype(np.float32)
+    d_l = gpu.DeviceArray((n,), np.float32)
+    d_o = gpu.DeviceArray((n,), np.uint32)
+    d_l.upload(lam)
+    gpu.check(gpu.lib.gps_selftest_poisson(n, d_l.ptr, d_o.ptr, None), "poisson")
+    dev = d_o.download()
+    seeds = np.empty(2, np.uint32)
+    host = np.empty(n, np.uint32)
+    for i in range(n):
+        orc.lib.orc_rng_probe(i, 12345, 0, seeds.ctypes.data_as(orc.C.c_void_p))
+        host[i] = orc.lib.orc_poisson(int(seeds[0]), int(seeds[1]), float(lam[i]))
+    assert_same("poisson", dev, host)
+
+
+# ------------------------------------------------------------------ one frame
+def expected_records(o_proj: np.ndarray, opac: np.ndarray, near: float, far: float) -> np.ndarray:
+    """Splat records rebuilt in float32 numpy from the oracle's FP32
+    projection (pixel mean, dilated cov2d, colour, view z, bounds)."""
+    import orc
+    f32 = np.float32
+    a, b, c = o_proj[:, 4].astype(f32), o_proj[:, 5].astype(f32), o_proj[:, 6].astype(f32)
+    c0 = np.sqrt(a)
+    c1 = b / c0
+    c2 = np.sqrt(c - c1 * c1)
+    det = a * c - b * b
+    inv_det = f32(1.0) / det
+    op = np.minimum(opac.astype(f32), f32(1.0))
+    rec = np.zeros((o_proj.shape[0], 16), np.uint32)
+    cols = [o_proj[:, 2].astype(f32), o_proj[:, 3].astype(f32), c0, c1, c2, c * inv_det, (-b) * inv_det,
+            a * inv_det, op, f32(1.0) / np.maximum(op, f32(1e-37)), orc.detmath(6, op)]
+    for k, v in enumerate(cols):
+        rec[:, k] = np.asarray(v, f32).view(np.uint32)
+    rec[:, 11] = o_proj[:, 12].astype(np.uint32) | (o_proj[:, 13].astype(np.uint32) << 16)
+    rec[:, 12] = o_proj[:, 14].astype(np.uint32) | (o_proj[:, 15].astype(np.uint32) << 16)
+    vz = o_proj[:, 1].astype(f32)
+    depth = np.array([orc.lib.orc_depth_bits(float(v), near, far) for v in vz], np.uint64)
+    col = o_proj[:, 7:10].astype(f32)
+    with np.errstate(invalid="ignore"):
+        q = (col * f32(255.0)).astype(np.uint64) & np.uint64(0xFFF)
+    packed = (depth << np.uint64(36)) | (q[:, 0] << np.uint64(24)) | (q[:, 1] << np.uint64(12)) | q[:, 2]
+    rec[:, 13] = (packed & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    rec[:, 14] = (packed >> np.uint64(32)).astype(np.uint32)
+    return rec
+
+
+def run_pair(arrays, settings, frames, fmt=0, which=H.INTERMEDIATES, check_records=True):
+    g = H.GpuRunner(arrays, settings, fmt)
+    o = H.OracleRunner(arrays, settings, fmt)
+    out = []
+    for fi, f in enumerate(frames):
+        flush(f"frame {fi}: render gpu")
+        rg = g.render(f, which)
+        flush(f"frame {fi}: render oracle")
+        ro = o.render(f, which)
+        for k in which:
+            assert_same(f"frame {fi} debug[{k}]", rg.debug[k], ro.debug[k])
+        assert_same(f"frame {fi} color", rg.color, ro.color)
+        assert_same(f"frame {fi} depth", rg.depth, ro.depth)
+        for key in ("valid_gaussians", "total_weight", "num_points", "samples", "passes", "budget_truncated"):
+            assert rg.stats[key] == ro.stats[key], (key, rg.stats[key], ro.stats[key])
+        if check_records and not f.enable_occlusion_culling and fmt == 0:
+            ids = rg.debug.get(H.G.DBG_VALID_IDS)
+            if ids is not None and ids.size:
+                rec = g.p.debug(H.G.DBG_RECORDS).reshape(-1, 16)[ids]
+                proj = o.p.project(f, fp64=False)[ids]
+                exp = expected_records(proj, arrays.opacities[ids], f.near_plane, f.far_plane)
+                assert_same(f"frame {fi} records", rec[:, :15], exp[:, :15])
+        out.append((rg, ro))
+    return out
+
+
+def c1_frame(w=640, h=360, **kw):
+    return H.G.frame_params((0.0, 0.0, -30.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, w, h, **kw)
+
+
+def test_frame_c1_small(gpu):
+    arrays = H.scene(0x5EED0001, 60_000, H.C1)
+    st = gpu.RenderSettings(640, 360, 1, 1, fully_disable_hierarchical_culling=True)
+    (rg, ro), = run_pair(arrays, st, [c1_frame(ss=1)])
+    assert rg.stats["num_points"][0] > 10_000
+    assert H.psnr(rg.color, ro.color) == float("inf")
+
+
+def test_frame_c2_small_sh3_ss2_spp4(gpu):
+    arrays = H.scene(0x5EED0002, 40_000, H.C2, clusters=200)
+    st = gpu.RenderSettings(480, 270, 4, 2, fully_disable_hierarchical_culling=True)
+    f = H.G.frame_params((0.0, 120.0, -330.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, 480, 270, ss=2)
+    (rg, ro), = run_pair(arrays, st, [f])
+    assert rg.stats["K"] == 4 and rg.stats["num_points"][0] > 0
+
+
+@pytest.mark.parametrize("case", ["empty", "one", "ragged37", "behind", "budget", "biased", "one_point",
+                                  "cull_small", "ss3_spp3", "background", "odd_res"])
+def test_frame_edge_cases(gpu, case):
+    flush(f"edge case {case}")
+    n = {"empty": 0, "one": 1, "ragged37": 37}.get(case, 20_000)
+    arrays = H.scene(0x5EED0100 + len(case), n, H.C1) if n else H.G.SceneArrays(
+        np.zeros((0, 3), np.float32), np.zeros((0, 4), np.float32), np.zeros((0, 3), np.float32),
+        np.zeros((0,), np.float32), np.zeros((0, 3), np.float32), 0)
+    if case == "one":
+        arrays.means[:] = 0.0
+        arrays.scales[:] = 0.5
+        arrays.opacities[:] = 0.9
+    w, h, ss, spp = 320, 200, 1, 1
+    if case == "ss3_spp3":
+        ss, spp = 3, 3
+    if case == "odd_res":
+        w, h, ss = 237, 131, 2
+    st = gpu.RenderSettings(w, h, spp, ss, fully_disable_hierarchical_culling=True,
+                            max_number_of_points=1000 if case == "budget" else 250_000_000)
+    kw = dict(ss=ss)
+    if case == "biased":
+        kw["unbiased"] = False
+    if case == "one_point":
+        kw["reduce_point_count"] = False
+    if case == "cull_small":
+        kw["cull_small"] = True
+    if case == "background":
+        kw["background"] = (0.25, 0.5, 1.0)
+    eye, ctr = ((0.0, 0.0, -30.0), (0.0, 0.0, 0.0)) if case != "behind" else ((0.0, 0.0, -30.0), (0.0, 0.0, -60.0))
+    f = H.G.frame_params(eye, ctr, (0.0, 1.0, 0.0), 45.0, w, h, **kw)
+    (rg, ro), = run_pair(arrays, st, [f])
+    if case == "budget":
+        assert rg.stats["budget_truncated"] == 1 and rg.stats["num_points"][0] == 1000
+    if case in ("empty", "behind"):
+        assert rg.stats["num_points"][0] == 0
+
+
+def moving_frames(n_frames, w, h, ss, radius=40.0, **kw):
+    frames = []
+    for i in range(n_frames):
+        a = 0.15 * i
+        eye = (radius * np.sin(a), 3.0, -radius * np.cos(a))
+        f = H.G.frame_params(eye, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, w, h, ss=ss, frame=i + 1, **kw)
+        frames.append(f)
+    return frames
+
+
+def test_frames_occlusion_and_block_culling(gpu):
+    # Two overlapping slabs so the Hi-Z culls something; Morton-free index order.
+    a = H.scene(0x5EED0201, 30_000, dict(H.C1, half_extent=6.0, log_scale_mu=-2.5))
+    b = H.scene(0x5EED0202, 30_000, dict(H.C1, half_extent=6.0, log_scale_mu=-2.5), offset=(0.0, 0.0, 20.0))
+    arrays = H.G.SceneArrays(np.concatenate([a.means, b.means]), np.concatenate([a.rotations, b.rotations]),
+                             np.concatenate([a.scales, b.scales]), np.concatenate([a.opacities, b.opacities]),
+                             np.concatenate([a.shs, b.shs]), 0)
+    st = gpu.RenderSettings(320, 200, 1, 2, gaussians_per_block=64)
+    frames = moving_frames(4, 320, 200, 2, occlusion=True, hierarchical=True)
+    which = H.INTERMEDIATES + [H.G.DBG_MASK_PREVIOUS, H.G.DBG_HIZ, H.G.DBG_CULL_MASK]
+    res = run_pair(arrays, st, frames, which=which)
+    culled = sum(r[0].stats["culled_blocks"][0] for r in res)
+    second = sum(r[0].stats["num_points"][1] for r in res)
+    flush(f"culled blocks {culled}, second-pass points {second}")
+
+
+def test_frames_half_fp8(gpu):
+    arrays = H.scene(0x5EED0301, 30_000, H.C2, clusters=100)
+    st = gpu.RenderSettings(320, 200, 2, 2, gaussians_per_block=128)
+    frames = moving_frames(3, 320, 200, 2, radius=300.0, occlusion=True, hierarchical=True)
+    run_pair(arrays, st, frames, fmt=H.G.FORMAT_HALF_FP8,
+             which=H.INTERMEDIATES + [H.G.DBG_HIZ, H.G.DBG_CULL_MASK])
+
+
+# ------------------------------------------------------------------ resolve
+def test_resolve_taa_and_progressive(gpu):
+    import orc
+    w, h = 160, 96
+    rng = np.random.default_rng(7)
+    cur = rng.uniform(-0.1, 1.2, (h, w, 3)).astype(np.float32)
+    prev = rng.uniform(0.0, 1.0, (h, w, 3)).astype(np.float32)
+    cdep = rng.uniform(0.9, 1.0, (h, w)).astype(np.float32)
+    pdep = rng.uniform(0.9, 1.0, (h, w)).astype(np.float32)
+    view0 = gpu.look_at((0.0, 0.0, -30.0), (0.0, 0.0, 0.0))
+    view1 = gpu.look_at((0.5, 0.2, -29.0), (0.0, 0.0, 0.0))
+    proj = gpu.perspective(45.0, w / h, 0.2, 1e4)
+    M = gpu.taa_reprojection(gpu.mat4_mul(proj, view0), view1, proj)
+    dc, dd, dp, dpd = (gpu.DeviceArray(a.shape, np.float32) for a in (cur, cdep, prev, pdep))
+    for d_, a_ in ((dc, cur), (dd, cdep), (dp, prev), (dpd, pdep)):
+        d_.upload(a_)
+    out = gpu.DeviceArray(cur.shape, np.float32)
+    rgb = gpu.DeviceArray(cur.shape, np.uint8)
+    for fn in (0, 1, 5):
+        gpu.resolve_taa(dc, dd, dp, dpd, M, 0.9, w, h, fn, out, rgb)
+        oc, orgb = orc.resolve_taa(cur, cdep, prev, pdep, M, 0.9, w, h, fn)
+        assert_same(f"taa color {fn}", out.download(), oc)
+        assert_same(f"taa rgb {fn}", rgb.download(), orgb)
+    for n in (1, 2, 7):
+        dc.upload(cur)
+        dp.upload(prev)
+        gpu.resolve_progressive(dc, dp, w, h, n, rgb)
+        c2, a2, r2 = orc.resolve_progressive(cur, prev, w, h, n)
+        assert_same(f"prog color {n}", dc.download(), c2)
+        assert_same(f"prog accum {n}", dp.download(), a2)
+        assert_same(f"prog rgb {n}", rgb.download(), r2)
