@@ -101,6 +101,12 @@ struct gps_pass {
     bool hier_available = false;
     // splat grid
     uint32_t splat_grid = 0;
+    // stage timing: ring of frames x stages x 2 occurrences x (start, stop)
+    std::vector<hipEvent_t> ev;
+    std::vector<uint8_t> ev_used;
+    int timing_cap = 0;
+    uint64_t timing_frames = 0;
+    int timing_slot = -1;
     // last frame
     int last_occl = 0;
     int last_K = 1;
@@ -143,6 +149,29 @@ gps_status ensure_fb(gps_pass* p, int ss) {
     p->fb_words = words;
     p->fb_ss = ss;
     return GPS_OK;
+}
+
+constexpr int kOcc = 2;
+inline int ev_index(int slot, int stage, int occ, int which) {
+    return ((slot * GPS_NUM_STAGES + stage) * kOcc + occ) * 2 + which;
+}
+
+// Records the start (which = 0) or stop (1) event of `stage`, occurrence `occ`
+// of the current frame's slot.
+void stage_mark(gps_pass* p, int stage, int occ, int which, hipStream_t st) {
+    if (p->timing_slot < 0) return;
+    const int i = ev_index(p->timing_slot, stage, occ, which);
+    (void)hipEventRecord(p->ev[i], st);
+    if (which == 1) p->ev_used[i / 2] = 1;
+}
+
+void destroy_events(gps_pass* p) {
+    for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+    p->ev.clear();
+    p->ev_used.clear();
+    p->timing_cap = 0;
+    p->timing_slot = -1;
+    p->timing_frames = 0;
 }
 
 }  // namespace
@@ -359,6 +388,7 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
 
 void gps_pass_destroy(gps_pass* p) {
     if (!p) return;
+    destroy_events(p);
     dfree(p->weights);
     dfree(p->maskA);
     dfree(p->maskB);
@@ -432,7 +462,16 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
 
     const uint64_t bg = background_word(fp->background[0], fp->background[1], fp->background[2]);
     p->cur_fb = (fp->frame % 2 == 0) ? p->fbA : p->fbB;
+    if (p->timing_cap > 0) {
+        p->timing_slot = (int)(p->timing_frames % (uint64_t)p->timing_cap);
+        for (int k = 0; k < GPS_NUM_STAGES * kOcc; ++k) p->ev_used[p->timing_slot * GPS_NUM_STAGES * kOcc + k] = 0;
+        ++p->timing_frames;
+    } else {
+        p->timing_slot = -1;
+    }
+    stage_mark(p, 0, 0, 0, st);
     launch_clear(p->cur_fb, p->fb_words, bg, st);
+    stage_mark(p, 0, 0, 1, st);
     // reset per-frame counters (pass results, culled counts, atomics), keep
     // the Hi-Z ticket and the mask iteration
     launch_zero_u32(reinterpret_cast<uint32_t*>(p->ctr), 10, st);
@@ -443,8 +482,11 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
     auto render_pass = [&](bool first) {
         const int slot = first ? 0 : 1;
         const uint32_t frame_seed = (uint32_t)(4 * fp->frame + (first ? 0 : 2));
-        if (hier)
+        if (hier) {
+            stage_mark(p, 1, slot, 0, st);
             launch_cull(p->cull, fp->frustum_planes, vpm, p->hiz, f.occl, &p->ctr->culled_blocks[slot], st);
+            stage_mark(p, 1, slot, 1, st);
+        }
         PreprocessArgs a;
         a.scene = p->scene->view;
         a.f = f;
@@ -459,7 +501,9 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         a.tile_cnt = p->tile_cnt;
         a.cull = p->cull;
         a.hiz = p->hiz;
+        stage_mark(p, 2, slot, 0, st);
         launch_preprocess(a, st);
+        stage_mark(p, 2, slot, 1, st);
         DistributeArgs da;
         da.tile_sum = p->tile_sum;
         da.tile_cnt = p->tile_cnt;
@@ -473,7 +517,9 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         da.coff = p->coff;
         da.chunk_start = p->chunk_start;
         da.iteration = &p->ctr->iteration;
+        stage_mark(p, 3, slot, 0, st);
         launch_distribute(da, st);
+        stage_mark(p, 3, slot, 1, st);
         SplatArgs sa;
         sa.recs = p->recs;
         sa.cidx = p->cidx;
@@ -488,23 +534,72 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         sa.unbiased = f.unbiased;
         sa.frame_seed = frame_seed + 1u;
         sa.grid = p->splat_grid;
+        stage_mark(p, 4, slot, 0, st);
         launch_splat(sa, st);
+        stage_mark(p, 4, slot, 1, st);
     };
     render_pass(true);
     if (fp->enable_occlusion_culling) {
+        stage_mark(p, 5, 0, 0, st);
         launch_combine(p->cur_fb, d_color, d_depth, fp->width, fp->height, ss, f.spp, fp->near_plane, fp->far_plane, 1,
                        st);
+        stage_mark(p, 5, 0, 1, st);
+        stage_mark(p, 6, 0, 0, st);
         launch_hiz_build(p->hiz, d_depth, fp->occlusion_culling_min_reduce_count, &p->ctr->hiz_ticket, st);
+        stage_mark(p, 6, 0, 1, st);
         render_pass(false);
     }
+    stage_mark(p, 5, 1, 0, st);
     launch_combine(p->cur_fb, d_color, d_depth, fp->width, fp->height, ss, f.spp, fp->near_plane, fp->far_plane, 0, st);
-    if (fp->enable_occlusion_culling)
+    stage_mark(p, 5, 1, 1, st);
+    if (fp->enable_occlusion_culling) {
+        stage_mark(p, 6, 1, 0, st);
         launch_hiz_build(p->hiz, d_depth, fp->occlusion_culling_min_reduce_count, &p->ctr->hiz_ticket, st);
+        stage_mark(p, 6, 1, 1, st);
+    }
     p->last_occl = fp->enable_occlusion_culling ? 1 : 0;
     p->last_K = f.Ki;
     p->last_spp = f.spp;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return GPS_OK;
+}
+
+gps_status gps_pass_set_timing(gps_pass* p, int32_t capacity) {
+    if (!p || capacity < 0) return GPS_ERR_INVALID_ARG;
+    destroy_events(p);
+    if (capacity == 0) return GPS_OK;
+    const size_t n = (size_t)capacity * GPS_NUM_STAGES * kOcc * 2;
+    p->ev.resize(n);
+    for (size_t i = 0; i < n; ++i) GPS_HIP(hipEventCreate(&p->ev[i]));
+    p->ev_used.assign(n / 2, 0);
+    p->timing_cap = capacity;
+    return GPS_OK;
+}
+
+gps_status gps_pass_timing_summary(gps_pass* p, double out_ms[GPS_NUM_STAGES], uint64_t out_launches[GPS_NUM_STAGES],
+                                   uint64_t* out_frames) {
+    if (!p || !out_ms || !out_launches) return GPS_ERR_INVALID_ARG;
+    for (int k = 0; k < GPS_NUM_STAGES; ++k) {
+        out_ms[k] = 0.0;
+        out_launches[k] = 0;
+    }
+    if (p->timing_cap == 0) return GPS_ERR_NOT_READY;
+    const uint64_t frames = std::min<uint64_t>(p->timing_frames, (uint64_t)p->timing_cap);
+    for (uint64_t slot = 0; slot < frames; ++slot)
+        for (int stg = 0; stg < GPS_NUM_STAGES; ++stg)
+            for (int occ = 0; occ < kOcc; ++occ) {
+                const int i = ev_index((int)slot, stg, occ, 0);
+                if (!p->ev_used[i / 2]) continue;
+                GPS_HIP(hipEventSynchronize(p->ev[i + 1]));
+                float ms = 0.0f;
+                GPS_HIP(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+                out_ms[stg] += ms;
+                out_launches[stg] += 1;
+            }
+    if (out_frames) *out_frames = frames;
+    p->timing_frames = 0;
+    std::fill(p->ev_used.begin(), p->ev_used.end(), 0);
     return GPS_OK;
 }
 

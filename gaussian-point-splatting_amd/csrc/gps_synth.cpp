@@ -6,6 +6,8 @@
 // does not depend on the chunking.
 #include <cmath>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "../../include/gps.h"
 
@@ -38,14 +40,13 @@ uint64_t mix(uint64_t a, uint64_t b) {
 
 }  // namespace
 
-extern "C" gps_status gps_synth_generate(const gps_synth_params* p, uint64_t first, uint64_t count, float* means,
-                                         float* rotations, float* scales, float* opacities, float* shs) {
-    if (!p || !means || !rotations || !scales || !opacities || !shs) return GPS_ERR_INVALID_ARG;
-    if (p->sh_degree < 0 || p->sh_degree > 3 || (p->kind == 1 && p->clusters <= 0)) return GPS_ERR_INVALID_ARG;
-    if (first + count > p->num_gaussians) return GPS_ERR_INVALID_ARG;
+namespace {
+
+void generate_range(const gps_synth_params* p, uint64_t first, uint64_t k0, uint64_t k1, float* means, float* rotations,
+                    float* scales, float* opacities, float* shs) {
     const int ncoef = (p->sh_degree + 1) * (p->sh_degree + 1);
     const double h = p->half_extent;
-    for (uint64_t k = 0; k < count; ++k) {
+    for (uint64_t k = k0; k < k1; ++k) {
         const uint64_t g = first + k;
         SplitMix64 r{mix(p->seed, g)};
         double pos[3];
@@ -70,5 +71,26 @@ extern "C" gps_status gps_synth_generate(const gps_synth_params* p, uint64_t fir
         for (int c = 0; c < 3; ++c) sh[c] = (float)(p->dc_sigma * r.normal());
         for (int i = 3; i < 3 * ncoef; ++i) sh[i] = (float)(p->rest_sigma * r.normal());
     }
+}
+
+}  // namespace
+
+extern "C" gps_status gps_synth_generate(const gps_synth_params* p, uint64_t first, uint64_t count, float* means,
+                                         float* rotations, float* scales, float* opacities, float* shs) {
+    if (!p || !means || !rotations || !scales || !opacities || !shs) return GPS_ERR_INVALID_ARG;
+    if (p->sh_degree < 0 || p->sh_degree > 3 || (p->kind == 1 && p->clusters <= 0)) return GPS_ERR_INVALID_ARG;
+    if (first + count > p->num_gaussians) return GPS_ERR_INVALID_ARG;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt == 0 ? 1 : (nt > 32 ? 32 : nt);
+    if (count < 65536) nt = 1;
+    std::vector<std::thread> ts;
+    const uint64_t chunk = (count + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t k0 = (uint64_t)t * chunk;
+        if (k0 >= count) break;
+        const uint64_t k1 = k0 + chunk < count ? k0 + chunk : count;
+        ts.emplace_back(generate_range, p, first, k0, k1, means, rotations, scales, opacities, shs);
+    }
+    for (auto& t : ts) t.join();
     return GPS_OK;
 }

@@ -162,6 +162,9 @@ _sig("gps_event_destroy", C.c_int32, _vp)
 _sig("gps_event_record", C.c_int32, _vp, _vp)
 _sig("gps_event_elapsed_ms", C.c_int32, _vp, _vp, C.POINTER(C.c_float))
 
+_sig("gps_pass_set_timing", C.c_int32, _vp, C.c_int32)
+_sig("gps_pass_timing_summary", C.c_int32, _vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+     C.POINTER(C.c_uint64))
 _sig("gps_selftest_math", C.c_int32, C.c_int32, _vp, _vp, C.c_uint64, _vp)
 _sig("gps_selftest_rng", C.c_int32, C.c_uint32, C.c_uint32, C.c_int32, _vp, _vp)
 _sig("gps_selftest_poisson", C.c_int32, C.c_uint32, _vp, _vp, _vp)
@@ -452,6 +455,19 @@ class Pass:
         check(_lib.gps_pass_debug_copy(self.handle, which, _ptr(out), n, stream.handle if stream else None),
               f"debug copy {which}")
         return out
+
+    STAGES = ("clear", "cull", "preprocess", "distribute", "splat", "combine", "hiz")
+
+    def set_timing(self, capacity: int) -> None:
+        check(_lib.gps_pass_set_timing(self.handle, capacity), "set timing")
+
+    def timing_summary(self) -> dict:
+        ms = (C.c_double * 7)()
+        n = (C.c_uint64 * 7)()
+        frames = C.c_uint64(0)
+        check(_lib.gps_pass_timing_summary(self.handle, ms, n, C.byref(frames)), "timing summary")
+        return {"frames": int(frames.value),
+                "stages": {k: {"ms": float(ms[i]), "launches": int(n[i])} for i, k in enumerate(self.STAGES)}}
 
     def close(self) -> None:
         if self.handle:

@@ -168,6 +168,17 @@ typedef enum gps_debug_buffer {
 uint64_t gps_pass_debug_size(gps_pass* pass, int32_t which);
 gps_status gps_pass_debug_copy(gps_pass* pass, int32_t which, void* host_dst, uint64_t count, void* stream);
 
+/* Per-stage timing with HIP events recorded on the render stream (no host
+ * synchronization while rendering).  capacity = number of frames whose events
+ * are kept (ring); 0 disables.  The summary synchronizes, returns the summed
+ * milliseconds and launch counts per stage over the frames recorded since the
+ * last summary, and resets.  Stages: 0 clear, 1 block cull, 2 preprocess,
+ * 3 distribute (tile scan + compaction), 4 splat, 5 combine, 6 Hi-Z. */
+#define GPS_NUM_STAGES 7
+gps_status gps_pass_set_timing(gps_pass* pass, int32_t capacity);
+gps_status gps_pass_timing_summary(gps_pass* pass, double out_ms[GPS_NUM_STAGES],
+                                   uint64_t out_launches[GPS_NUM_STAGES], uint64_t* out_frames);
+
 /* ---------------------------------------------------------------- resolve */
 typedef enum gps_accumulation_mode {
     GPS_ACCUM_PROGRESSIVE = 0,
