@@ -807,20 +807,22 @@ int32_t orc_project(orc_pass* p, const gps_frame_params* fp, int32_t use_fp64, d
                 const double det = cov.x * cov.z - cov.y * cov.y;
                 if (det <= 0) continue;
                 const vec2<double> pm = {(pr.x * 0.5 + 0.5) * f.up_w, (pr.y * 0.5 + 0.5) * f.up_h};
+                ivec2_ bmin, bmax;
+                if (!compute2DGaussianBounds3DGS(bmin, bmax, f.up_w, f.up_h, cov, pm)) continue;
                 std::vector<double> sh(48);
                 for (int k = 0; k < 48; ++k) sh[k] = p->sh48[48 * g + k];
                 const vec3<double> c = computeColorFromSHFloat(m, campos, sh.data());
                 const double op = std::min((double)p->opac[g], 1.0);
-                // Li2(op) by its series sum_k op^k / k^2
-                double li2 = 0, t = op;
-                for (int k = 1; k < 2000; ++k) {
-                    li2 += t / ((double)k * k);
-                    t *= op;
-                    if (t < 1e-18) break;
-                }
-                const double imp = 2.0 * M_PI * std::sqrt(det) * li2;
-                const double vals[12] = {1, vm.z, pm.x, pm.y, cov.x, cov.y, cov.z, c.x, c.y, c.z, imp, pr.z};
-                for (int k = 0; k < 12; ++k) o[k] = vals[k];
+                // the reference's dilog fit (random.cuh:88-104) evaluated in FP64
+                const double cf[8] = {-6.09201442e-01, 1.79126616e+00, -2.14953223e+00, 1.26304372e+00,
+                                      -4.59069895e-01, -1.87417414e-01, 1.99603130e+00, 5.99669467e-05};
+                double dl = cf[0];
+                for (int k = 1; k < 8; ++k) dl = dl * op + cf[k];
+                if (1.0 - op > 0.0) dl += (1.0 - op) * std::log(1.0 - op);
+                const double imp = 2.0 * M_PI * std::sqrt(det) * dl;
+                const double vals[16] = {1, vm.z, pm.x, pm.y, cov.x, cov.y, cov.z, c.x, c.y, c.z, imp, pr.z,
+                                         (double)bmin.x, (double)bmin.y, (double)bmax.x, (double)bmax.y};
+                for (int k = 0; k < 16; ++k) o[k] = vals[k];
             }
         }
     });
