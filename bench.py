@@ -100,12 +100,15 @@ def stage_bytes(cfg, st, N):
     }
 
 
-def cpu_baseline(G, cfg, sample_n, threads):
-    """Oracle on a bounded sample: the first `sample_n` Gaussians of the scene,
-    frame 1 of the workload, same resolution / ss / spp."""
+def cpu_baseline(G, cfg, sample_n, threads, sort):
+    """Oracle on a bounded sample: the first `sample_n` generated Gaussians of
+    the scene (Morton-ordered like the GPU scene), frame 1 of the workload,
+    same resolution / ss / spp."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc  # test infrastructure: the CPU baseline leg only
     arrays = make_scene(G, cfg, 0, sample_n)
+    if sort:
+        arrays = arrays.morton_sorted()
     st = G.RenderSettings(cfg["width"], cfg["height"], cfg["spp"], cfg["ss"],
                           fully_disable_hierarchical_culling=not cfg["hierarchical"])
     o = orc.OraclePass(st, arrays, cfg["fmt"], threads)
@@ -129,6 +132,8 @@ def main() -> int:
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0)
+    ap.add_argument("--no-sort", action="store_true",
+                    help="keep generation order instead of the load-time Morton order")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
@@ -142,6 +147,10 @@ def main() -> int:
     t0 = time.time()
     arrays = make_scene(G, cfg)
     log(f"[bench] generated {cfg['n']} Gaussians in {time.time() - t0:.1f}s")
+    if not args.no_sort:
+        t0 = time.time()
+        arrays = arrays.morton_sorted()
+        log(f"[bench] Morton-ordered (reference --sort-morton-order) in {time.time() - t0:.1f}s")
     scene = G.Scene(arrays, cfg["fmt"])
     if args.no_cpu_baseline:
         del arrays
@@ -181,11 +190,13 @@ def main() -> int:
     log(f"[bench] timed {K} frames: {gpu_ms:.2f} ms GPU, {wall * 1e3:.2f} ms wall")
     # Work counters: re-render the same (deterministic) frames untimed.
     samples = 0
+    atomics = 0
     per_frame = []
     for i in range(Wm, Wm + K):
         step(i, frames[i])
         st = ps.stats(stream)
         samples += st["samples"]
+        atomics += st["atomics"]
         per_frame.append(st)
     elapsed_s = max(gpu_ms, wall * 1e3) / 1e3
     ms_per_step = elapsed_s * 1e3 / K
@@ -214,16 +225,19 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         sample_n = args.cpu_sample or min(cfg["n"], 2_000_000)
         log(f"[bench] cpu baseline on {sample_n} Gaussians, {threads} threads")
-        cpu = cpu_baseline(G, cfg, sample_n, threads)
+        cpu = cpu_baseline(G, cfg, sample_n, threads, not args.no_sort)
     st0 = per_frame[0]
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Gpoints/s", "n_gpus": 1, "steps": K, "warmup": Wm,
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.config}: {cfg['desc']}", "gaussians": cfg["n"], "width": W, "height": H,
-                   "ss": cfg["ss"], "spp": cfg["spp"], "format": "fp32" if cfg["fmt"] == 0 else "half_fp8"},
+                   "ss": cfg["ss"], "spp": cfg["spp"], "format": "fp32" if cfg["fmt"] == 0 else "half_fp8",
+                   "order": "generation" if args.no_sort else "morton (load-time, reference --sort-morton-order)"},
         "fps": round(1000.0 / ms_per_step, 2),
         "points_per_frame": round(samples / K),
+        "atomics_per_frame": round(atomics / K),
+        "scene_order": "generation" if args.no_sort else "morton",
         "gaussians_valid_first_frame": st0["valid_gaussians"][0],
         "points_first_frame": st0["num_points"][0],
         "roofline": roofline,

@@ -101,6 +101,7 @@ struct gps_pass {
     bool hier_available = false;
     // splat grid
     uint32_t splat_grid = 0;
+    int32_t xcds = 1;
     // stage timing: ring of frames x stages x 2 occurrences x (start, stop)
     std::vector<hipEvent_t> ev;
     std::vector<uint8_t> ev_used;
@@ -373,6 +374,9 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
     }
+    // MI300-class parts (gfx942 / gfx950) have 8 XCDs of 32 or 38 CUs; the
+    // chunk schedule only needs the count to be a divisor of the grid.
+    p->xcds = (cus % 8 == 0) ? 8 : 1;
     p->splat_grid = (uint32_t)cus * 8u;
     if (s == GPS_OK) {
         hipError_t e = hipDeviceSynchronize();
@@ -534,6 +538,7 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         sa.unbiased = f.unbiased;
         sa.frame_seed = frame_seed + 1u;
         sa.grid = p->splat_grid;
+        sa.xcds = p->xcds;
         stage_mark(p, 4, slot, 0, st);
         launch_splat(sa, st);
         stage_mark(p, 4, slot, 1, st);

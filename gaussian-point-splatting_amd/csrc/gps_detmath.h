@@ -118,16 +118,9 @@ GPS_HD float det_expf(float x) {
 }
 
 // ---------------------------------------------------------------- log
-GPS_HD float det_logf(float x) {
-    if (x != x) return x;
-    if (x < 0.0f) return u2f_bits(0x7FC00000u);
-    if (x == 0.0f) return u2f_bits(0xFF800000u);
-    if (x == u2f_bits(0x7F800000u)) return x;
-    int32_t e = 0;
-    if (x < 0x1.0p-126f) {  // subnormal: scale into the normal range (exact)
-        x = x * 0x1.0p+25f;
-        e = -25;
-    }
+// det_logf for a positive normal finite x (the special cases of det_logf
+// cannot occur); same arithmetic, so the same bits for such x.
+GPS_HD float det_logf_normal(float x, int32_t e = 0) {
     const uint32_t ix = f2u_bits(x);
     e += (int32_t)(ix >> 23) - 127;
     float m = u2f_bits((ix & 0x007FFFFFu) | 0x3F800000u);    // [1, 2)
@@ -150,14 +143,18 @@ GPS_HD float det_logf(float x) {
     return det_fma(fe, 0x1.62e400p-1f, det_fma(fe, 0x1.7f7d1cp-20f, logm));
 }
 
+GPS_HD float det_logf(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return u2f_bits(0x7FC00000u);
+    if (x == 0.0f) return u2f_bits(0xFF800000u);
+    if (x == u2f_bits(0x7F800000u)) return x;
+    if (x < 0x1.0p-126f) return det_logf_normal(x * 0x1.0p+25f, -25);  // subnormal: exact rescale
+    return det_logf_normal(x);
+}
+
 // ---------------------------------------------------------------- sin/cos
-GPS_HD void det_sincosf(float x, float* sp, float* cp) {
-    if (!(det_abs(x) < 0x1.0p+20f)) {  // NaN/inf/huge: outside the renderer's use
-        const float n = u2f_bits(0x7FC00000u);
-        *sp = n;
-        *cp = n;
-        return;
-    }
+// For finite |x| < 2^20 (det_sincosf's valid range); same bits as det_sincosf.
+GPS_HD void det_sincosf_inrange(float x, float* sp, float* cp) {
     const float k = det_rint(x * 0x1.45f306p-1f);            // x * 2/pi
     float r = det_fma(-k, 0x1.921fb6p+0f, x);
     r = det_fma(-k, -0x1.777a5cp-25f, r);
@@ -184,6 +181,16 @@ GPS_HD void det_sincosf(float x, float* sp, float* cp) {
     }
     *sp = s;
     *cp = c;
+}
+
+GPS_HD void det_sincosf(float x, float* sp, float* cp) {
+    if (!(det_abs(x) < 0x1.0p+20f)) {  // NaN/inf/huge: outside the renderer's use
+        const float n = u2f_bits(0x7FC00000u);
+        *sp = n;
+        *cp = n;
+        return;
+    }
+    det_sincosf_inrange(x, sp, cp);
 }
 
 GPS_HD float det_sinf(float x) { float s, c; det_sincosf(x, &s, &c); return s; }

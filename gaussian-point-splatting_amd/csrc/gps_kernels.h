@@ -44,7 +44,8 @@ struct SplatArgs {
     uint32_t* atomics_count;
     int32_t spp, K, up_w, unbiased;
     uint32_t frame_seed;
-    uint32_t grid;
+    uint32_t grid;   // multiple of xcds
+    int32_t xcds;    // accelerator complex dies (workgroups round-robin over them)
 };
 
 void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st);

@@ -459,6 +459,7 @@ __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restr
 struct SplatConsts {
     int32_t spp, K, up_w, unbiased;
     uint32_t frame_seed;
+    int32_t xcds;
 };
 
 __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRecord* __restrict__ recs,
@@ -474,8 +475,18 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
     const uint32_t nchunks = ctr->chunks;
     const int tid = threadIdx.x;
     const int samples = sc.spp * sc.K;
+    const int tiles_per_row = fb_tiles_per_row(sc.up_w);
+    (void)tiles_per_row;
     uint32_t issued = 0;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    // Workgroups are dispatched round-robin over the XCDs (blockIdx % xcds);
+    // give every XCD one contiguous range of chunks so the framebuffer lines
+    // of a compact scene region stay in that XCD's L2.
+    const uint32_t xcds = (uint32_t)sc.xcds;
+    const uint32_t xcd = blockIdx.x % xcds;
+    const uint32_t per_xcd = gridDim.x / xcds;
+    const uint32_t seg_lo = (uint32_t)(((uint64_t)nchunks * xcd) / xcds);
+    const uint32_t seg_hi = (uint32_t)(((uint64_t)nchunks * (xcd + 1)) / xcds);
+    for (uint32_t c = seg_lo + blockIdx.x / xcds; c < seg_hi; c += per_xcd) {
         const uint32_t r_lo = chunk_start[c];
         for (int j = tid; j <= GPS_SPLAT_CHUNK; j += GPS_SPLAT_THREADS) {
             const uint32_t r = r_lo + (uint32_t)j;
@@ -525,7 +536,13 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
                 const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
                 const unsigned long long depth_bits = packed & ~(unsigned long long)GPS_COLOR_MASK;
                 u32x2 seed = make_seed(p, sc.frame_seed);
-                for (int q = 0; q < samples; ++q) {
+                // sample q = layer * K + k lands in layer q / K
+                for (int layer = 0, k = 0, q = 0; q < samples; ++q) {
+                    if (k == sc.K) {
+                        k = 0;
+                        ++layer;
+                    }
+                    ++k;
                     const f32x2 u = pcg2d(seed);
                     const f32x2 xy = sc.unbiased ? corrected_box_muller_pre(u.x, u.y, rec.inv_alpha, rec.dilog_alpha)
                                                  : box_muller(u.x, u.y);
@@ -539,7 +556,7 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
                     const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
                     const float alpha = rec.opacity * gw;
                     if (alpha < 1.0f / 255.0f) continue;
-                    const int idx = fb_index(x, y, q / sc.K, sc.spp, sc.up_w);
+                    const int idx = fb_index(x, y, layer, sc.spp, sc.up_w);
                     if (fb[idx] <= depth_bits) continue;
                     atomicMin(fb + idx, packed);
                     ++issued;
@@ -589,6 +606,77 @@ __global__ __launch_bounds__(256) void combine_kernel(const unsigned long long* 
     const uint32_t idx = (uint32_t)(y * W + x);
     if (!depth_only) {
         const float inv = 1.0f / (float)(spp * ss * ss);
+        color[3 * idx] = cr * inv;
+        color[3 * idx + 1] = cg * inv;
+        color[3 * idx + 2] = cb * inv;
+    }
+    depth[idx] = d;
+}
+
+// Staged combine for power-of-two ss with ss^2 * spp <= 16: the framebuffer
+// words of a 16x16 output block are one contiguous range (Morton order
+// nests: morton(ss*x + dx, ss*y + dy) = ss^2 * morton(x, y) + morton(dx, dy)),
+// so the workgroup streams it into LDS with 16-byte coalesced loads and
+// every thread then reduces its own words in the reference's order
+// (dx outer, dy, layer inner).  Pixel t's words sit at a stride of
+// 2*wpp+1 dwords in LDS, which keeps the per-thread reads conflict-free.
+template <int SS, int SPP>
+__global__ __launch_bounds__(256) void combine_staged_kernel(const uint4* __restrict__ fb, float* __restrict__ color,
+                                                             float* __restrict__ depth, int W, int H, int up_w,
+                                                             float near_plane, float far_plane, int depth_only) {
+    constexpr int WPP = SS * SS * SPP;          // words per output pixel
+    constexpr int STRIDE = 2 * WPP + 1;         // dwords per pixel in LDS
+    __shared__ uint32_t s[256 * STRIDE];
+    const int t = threadIdx.x;
+    // first FB pixel of this block and its word index
+    const int fx0 = blockIdx.x * 16 * SS, fy0 = blockIdx.y * 16 * SS;
+    const int64_t base_word = (int64_t)fb_index(fx0, fy0, 0, SPP, up_w);  // multiple of 256*WPP
+    const uint4* src = fb + base_word / 2;
+    constexpr int CHUNKS = 256 * WPP / 2;      // 16-byte chunks of the block
+#pragma unroll
+    for (int c = t; c < CHUNKS; c += 256) {
+        const uint4 v = src[c];
+        const int w0 = 2 * c;                  // first of the two words
+        const int p0 = w0 / WPP, k0 = w0 % WPP;
+        const int w1 = w0 + 1;
+        const int p1 = w1 / WPP, k1 = w1 % WPP;
+        s[p0 * STRIDE + 2 * k0] = v.x;
+        s[p0 * STRIDE + 2 * k0 + 1] = v.y;
+        s[p1 * STRIDE + 2 * k1] = v.z;
+        s[p1 * STRIDE + 2 * k1 + 1] = v.w;
+    }
+    __syncthreads();
+    uint32_t mx = 0, my = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        mx |= ((uint32_t)(t >> (2 * b)) & 1u) << b;
+        my |= ((uint32_t)(t >> (2 * b + 1)) & 1u) << b;
+    }
+    const int x = blockIdx.x * 16 + (int)mx;
+    const int y = blockIdx.y * 16 + (int)my;
+    if (x >= W || y >= H) return;
+    const uint32_t* mine = s + t * STRIDE;
+    float d = 1e9f;
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+#pragma unroll
+    for (int dx = 0; dx < SS; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < SS; ++dy)
+#pragma unroll
+            for (int p = 0; p < SPP; ++p) {
+                const int k = (int)morton_encode((uint32_t)dx, (uint32_t)dy) * SPP + p;
+                const uint32_t lo = mine[2 * k], hi = mine[2 * k + 1];
+                const float vd = depth_dequantize((hi >> 4) & (uint32_t)GPS_DEPTH_MAX, near_plane, far_plane);
+                const float z = view_to_ndc_depth(vd, near_plane, far_plane);
+                d = glm_min(d, z);
+                if (depth_only) continue;
+                cr += (float)(((lo >> 24) | (hi << 8)) & 0xFFFu) / 255.0f;
+                cg += (float)((lo >> 12) & 0xFFFu) / 255.0f;
+                cb += (float)(lo & 0xFFFu) / 255.0f;
+            }
+    const uint32_t idx = (uint32_t)(y * W + x);
+    if (!depth_only) {
+        const float inv = 1.0f / (float)(SPP * SS * SS);
         color[3 * idx] = cr * inv;
         color[3 * idx + 1] = cg * inv;
         color[3 * idx + 2] = cb * inv;
@@ -868,12 +956,31 @@ void launch_splat(const SplatArgs& a, hipStream_t st) {
     sc.up_w = a.up_w;
     sc.unbiased = a.unbiased;
     sc.frame_seed = a.frame_seed;
+    sc.xcds = a.xcds;
     hipLaunchKernelGGL(splat_kernel, dim3(a.grid), dim3(GPS_SPLAT_THREADS), 0, st, a.recs, a.cidx, a.coff,
                        a.chunk_start, a.ctr, reinterpret_cast<unsigned long long*>(a.fb), sc, a.atomics_count);
 }
 
+template <int SS, int SPP>
+static bool try_staged(const uint64_t* fb, float* color, float* depth, int W, int H, int ss, int spp, float near_plane,
+                       float far_plane, int depth_only, hipStream_t st) {
+    if (ss != SS || spp != SPP) return false;
+    dim3 grid((W + 15) / 16, (H + 15) / 16);
+    hipLaunchKernelGGL((combine_staged_kernel<SS, SPP>), grid, dim3(256), 0, st, reinterpret_cast<const uint4*>(fb),
+                       color, depth, W, H, W * ss, near_plane, far_plane, depth_only);
+    return true;
+}
+
 void launch_combine(const uint64_t* fb, float* color, float* depth, int W, int H, int ss, int spp, float near_plane,
                     float far_plane, int depth_only, hipStream_t st) {
+    if (try_staged<1, 1>(fb, color, depth, W, H, ss, spp, near_plane, far_plane, depth_only, st) ||
+        try_staged<1, 2>(fb, color, depth, W, H, ss, spp, near_plane, far_plane, depth_only, st) ||
+        try_staged<1, 4>(fb, color, depth, W, H, ss, spp, near_plane, far_plane, depth_only, st) ||
+        try_staged<2, 1>(fb, color, depth, W, H, ss, spp, near_plane, far_plane, depth_only, st) ||
+        try_staged<2, 2>(fb, color, depth, W, H, ss, spp, near_plane, far_plane, depth_only, st) ||
+        try_staged<2, 4>(fb, color, depth, W, H, ss, spp, near_plane, far_plane, depth_only, st) ||
+        try_staged<4, 1>(fb, color, depth, W, H, ss, spp, near_plane, far_plane, depth_only, st))
+        return;
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     hipLaunchKernelGGL(combine_kernel, grid, dim3(256), 0, st, reinterpret_cast<const unsigned long long*>(fb), color,
                        depth, W, H, ss, spp, W * ss, near_plane, far_plane, depth_only);

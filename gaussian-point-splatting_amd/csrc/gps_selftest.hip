@@ -49,7 +49,25 @@ __global__ void selftest_poisson_kernel(uint32_t n, const float* __restrict__ la
     out[i] = poisson(s, lambda[i]);
 }
 
+// Plain streaming copy: the reference point for HBM byte counters.
+__global__ __launch_bounds__(256) void selftest_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                            uint64_t n16) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace gps
+
+extern "C" gps_status gps_selftest_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {
+    if (!d_dst || !d_src || bytes % 16) return GPS_ERR_INVALID_ARG;
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return GPS_OK;
+    uint64_t blocks = (n16 + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(gps::selftest_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (const uint4*)d_src, (uint4*)d_dst, n16);
+    return hipGetLastError() == hipSuccess ? GPS_OK : GPS_ERR_HIP;
+}
 
 extern "C" gps_status gps_selftest_math(int32_t kind, const float* d_in, float* d_out, uint64_t n, void* stream) {
     if (!d_in || !d_out || kind < 0 || kind > 10) return GPS_ERR_INVALID_ARG;

@@ -143,6 +143,7 @@ _sig("gps_frame_params_init", None, C.POINTER(FrameParams), _fp, _fp, _fp, C.c_f
 _sig("gps_taa_reprojection", None, _fp, _fp, _fp, _fp)
 _sig("gps_mat4_mul", None, _fp, _fp, _fp)
 _sig("gps_synth_generate", C.c_int32, C.POINTER(SynthParams), C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp, _vp)
+_sig("gps_morton_order", C.c_int32, _vp, C.c_uint64, _vp)
 _sig("gps_encode_half_fp8", C.c_int32, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("gps_float_to_half", C.c_uint16, C.c_float)
 _sig("gps_float_to_fp8e4m3", C.c_uint8, C.c_float)
@@ -168,6 +169,7 @@ _sig("gps_pass_timing_summary", C.c_int32, _vp, C.POINTER(C.c_double), C.POINTER
 _sig("gps_selftest_math", C.c_int32, C.c_int32, _vp, _vp, C.c_uint64, _vp)
 _sig("gps_selftest_rng", C.c_int32, C.c_uint32, C.c_uint32, C.c_int32, _vp, _vp)
 _sig("gps_selftest_poisson", C.c_int32, C.c_uint32, _vp, _vp, _vp)
+_sig("gps_selftest_copy", C.c_int32, _vp, _vp, C.c_uint64, _vp)
 
 lib = _lib
 
@@ -321,6 +323,15 @@ class SceneArrays:
         d.opacities = _ptr(self.opacities)
         d.shs = _ptr(self.shs)
         return d
+
+    def morton_order(self) -> np.ndarray:
+        """perm[new] = old of the reference's --sort-morton-order."""
+        perm = np.empty(self.n, np.uint32)
+        check(_lib.gps_morton_order(_ptr(self.means), self.n, _ptr(perm)), "morton order")
+        return perm
+
+    def morton_sorted(self) -> "SceneArrays":
+        return self.subset(self.morton_order())
 
     def subset(self, idx) -> "SceneArrays":
         return SceneArrays(np.ascontiguousarray(self.means[idx]), np.ascontiguousarray(self.rotations[idx]),

@@ -254,6 +254,11 @@ typedef struct gps_synth_params {
 gps_status gps_synth_generate(const gps_synth_params* p, uint64_t first, uint64_t count, float* means,
                               float* rotations, float* scales, float* opacities, float* shs);
 
+/* Load-time Morton ordering, the reference's --sort-morton-order
+ * (src/core/gaussians/gaussian_loader.cu:363-426): writes perm[new] = old for
+ * a stable sort of the 63-bit Morton codes of the bbox-normalized positions. */
+gps_status gps_morton_order(const float* means, uint64_t n, uint32_t* perm);
+
 /* ----------------------------------------------------- compact encoding */
 /* Host encoder of GPS_FORMAT_HALF_FP8 (the same code gps_scene_create uses):
  * fp16 RNE rotation (N*4), scale (N*3), opacity (N), SH DC (N*3); the
@@ -276,6 +281,8 @@ gps_status gps_selftest_math(int32_t kind, const float* d_in, float* d_out, uint
 gps_status gps_selftest_rng(uint32_t n, uint32_t frame, int32_t draws, uint32_t* d_out, void* stream);
 /* Per item i: poisson(make_seed(i, 12345), lambda[i]). */
 gps_status gps_selftest_poisson(uint32_t n, const float* d_lambda, uint32_t* d_out, void* stream);
+/* Streaming device copy (bytes % 16 == 0): reference kernel for HBM byte counters. */
+gps_status gps_selftest_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
 
 /* ------------------------------------------------------- device plumbing */
 /* Minimal HIP plumbing for hosts without a HIP binding (ctypes, cgo, JNI). */
