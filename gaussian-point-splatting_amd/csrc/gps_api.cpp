@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gps.h"
@@ -110,6 +111,7 @@ struct gps_pass {
     int timing_slot = -1;
     // last frame
     int last_occl = 0;
+    int last_hier = 0;
     int last_K = 1;
     int last_spp = 1;
 };
@@ -197,106 +199,152 @@ const char* gps_status_str(gps_status s) {
 const char* gps_last_error(void) { return g_last_error.c_str(); }
 
 // ------------------------------------------------------------------ scenes
-gps_status gps_scene_create(const gps_scene_desc* d, gps_scene** out) {
-    if (!d || !out) return GPS_ERR_INVALID_ARG;
+extern "C++" {
+namespace {
+
+// Runs fn(lo, hi) over [0, n) on up to 32 host threads.
+template <typename F> void parallel_range(uint64_t n, F fn) {
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt == 0 ? 1 : std::min(nt, 32u);
+    if (n < 16384) nt = 1;
+    std::vector<std::thread> ts;
+    const uint64_t per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t lo = (uint64_t)t * per;
+        if (lo >= n) break;
+        ts.emplace_back(fn, lo, std::min(n, lo + per));
+    }
+    for (auto& t : ts) t.join();
+}
+
+}  // namespace
+}
+
+gps_status gps_scene_create_empty(uint64_t N, int32_t sh_degree, int32_t format, gps_scene** out) {
+    if (!out) return GPS_ERR_INVALID_ARG;
     *out = nullptr;
-    if (d->sh_degree < 0 || d->sh_degree > 3) return GPS_ERR_INVALID_ARG;
-    if (d->num_gaussians >= (1ull << 32) - GPS_TILE) return GPS_ERR_UNSUPPORTED;
-    if (d->num_gaussians > 0 && (!d->means || !d->rotations || !d->scales || !d->opacities || !d->shs))
-        return GPS_ERR_INVALID_ARG;
-    if (d->format != GPS_FORMAT_FP32 && d->format != GPS_FORMAT_HALF_FP8) return GPS_ERR_INVALID_ARG;
-    const uint64_t N = d->num_gaussians;
-    const int ncoef = (d->sh_degree + 1) * (d->sh_degree + 1);
+    if (sh_degree < 0 || sh_degree > 3) return GPS_ERR_INVALID_ARG;
+    if (format != GPS_FORMAT_FP32 && format != GPS_FORMAT_HALF_FP8) return GPS_ERR_INVALID_ARG;
+    if (N >= (1ull << 32) - GPS_TILE) return GPS_ERR_UNSUPPORTED;
+    const int ncoef = (sh_degree + 1) * (sh_degree + 1);
     const uint64_t stride = round_up(std::max<uint64_t>(N, 1), 64);
     gps_scene* s = new gps_scene();
     s->view.N = (uint32_t)N;
-    s->view.sh_degree = d->sh_degree;
-    s->view.format = d->format;
+    s->view.sh_degree = sh_degree;
+    s->view.format = format;
     s->view.stride = stride;
-    const uint64_t chunk = 1u << 20;
-    if (d->format == GPS_FORMAT_FP32) {
+    gps_status st;
+    if (format == GPS_FORMAT_FP32) {
         const int nplanes = 11 + 3 * ncoef;
         float* planes = nullptr;
-        gps_status st = dalloc(&planes, stride * nplanes, false);
-        if (st != GPS_OK) {
-            delete s;
-            return st;
-        }
+        st = dalloc(&planes, stride * nplanes, true);
         s->planes = planes;
-        s->bytes = stride * nplanes * sizeof(float);
-        std::vector<float> stage(chunk);
-        for (uint64_t b = 0; b < N; b += chunk) {
-            const uint64_t n = std::min(chunk, N - b);
-            for (int p = 0; p < nplanes; ++p) {
-                for (uint64_t i = 0; i < n; ++i) {
-                    const uint64_t g = b + i;
-                    float v;
-                    if (p < 3) v = d->means[3 * g + p];
-                    else if (p < 7) v = d->rotations[4 * g + (p - 3)];
-                    else if (p < 10) v = d->scales[3 * g + (p - 7)];
-                    else if (p == 10) v = d->opacities[g];
-                    else v = d->shs[(uint64_t)3 * ncoef * g + (p - 11)];
-                    stage[i] = v;
-                }
-                hipError_t e = hipMemcpy(planes + (uint64_t)p * stride + b, stage.data(), n * sizeof(float),
-                                         hipMemcpyHostToDevice);
-                if (e != hipSuccess) {
-                    gps_scene_destroy(s);
-                    return hip_fail(e, "scene upload");
-                }
-            }
-        }
         s->view.planes = planes;
+        s->bytes = stride * nplanes * sizeof(float);
     } else {
-        // GPS_FORMAT_HALF_FP8: encode on the host (fp16 RNE, fp8 E4M3 RNE with a
-        // per-Gaussian scale = max|rest coefficient| / 448).
         float* means = nullptr;
         uint4* geo = nullptr;
         uint4* shrec = nullptr;
-        gps_status st = dalloc(&means, 3 * stride, false);
-        if (st == GPS_OK) st = dalloc(&geo, stride, false);
+        st = dalloc(&means, 3 * stride, true);
+        if (st == GPS_OK) st = dalloc(&geo, stride, true);
         if (st == GPS_OK) st = dalloc(&shrec, 4 * stride, true);
         s->planes = means;
         s->geo = geo;
         s->shrec = shrec;
-        if (st != GPS_OK) {
-            gps_scene_destroy(s);
-            return st;
-        }
-        s->bytes = stride * (12 + 16 + 64);
-        std::vector<float> mstage(chunk);
-        std::vector<uint32_t> gstage(4 * chunk), sstage(16 * chunk);
-        for (uint64_t b = 0; b < N; b += chunk) {
-            const uint64_t n = std::min(chunk, N - b);
-            for (int c = 0; c < 3; ++c) {
-                for (uint64_t i = 0; i < n; ++i) mstage[i] = d->means[3 * (b + i) + c];
-                hipError_t e = hipMemcpy(means + (uint64_t)c * stride + b, mstage.data(), n * 4, hipMemcpyHostToDevice);
-                if (e != hipSuccess) {
-                    gps_scene_destroy(s);
-                    return hip_fail(e, "scene upload");
-                }
-            }
-            std::vector<uint8_t> enc(64);
-            for (uint64_t i = 0; i < n; ++i) {
-                const uint64_t g = b + i;
-                uint16_t h[8];
-                for (int k = 0; k < 4; ++k) h[k] = gps_float_to_half_rne(d->rotations[4 * g + k]);
-                for (int k = 0; k < 3; ++k) h[4 + k] = gps_float_to_half_rne(d->scales[3 * g + k]);
-                h[7] = gps_float_to_half_rne(d->opacities[g]);
-                std::memcpy(&gstage[4 * i], h, 16);
-                gps_encode_sh_record(d->shs + (uint64_t)3 * ncoef * g, ncoef, enc.data());
-                std::memcpy(&sstage[16 * i], enc.data(), 64);
-            }
-            hipError_t e = hipMemcpy(geo + b, gstage.data(), n * 16, hipMemcpyHostToDevice);
-            if (e == hipSuccess) e = hipMemcpy(shrec + 4 * b, sstage.data(), n * 64, hipMemcpyHostToDevice);
-            if (e != hipSuccess) {
-                gps_scene_destroy(s);
-                return hip_fail(e, "scene upload");
-            }
-        }
         s->view.means = means;
         s->view.geo = geo;
         s->view.shrec = shrec;
+        s->bytes = stride * (12 + 16 + 64);
+    }
+    if (st != GPS_OK) {
+        gps_scene_destroy(s);
+        return st;
+    }
+    *out = s;
+    return GPS_OK;
+}
+
+gps_status gps_scene_upload_range(gps_scene* s, uint64_t first, const gps_scene_desc* d) {
+    if (!s || !d) return GPS_ERR_INVALID_ARG;
+    const uint64_t n_all = d->num_gaussians;
+    if (d->sh_degree != s->view.sh_degree || d->format != s->view.format) return GPS_ERR_INVALID_ARG;
+    if (first > s->view.N || n_all > s->view.N - first) return GPS_ERR_INVALID_ARG;
+    if (n_all > 0 && (!d->means || !d->rotations || !d->scales || !d->opacities || !d->shs))
+        return GPS_ERR_INVALID_ARG;
+    const int ncoef = (d->sh_degree + 1) * (d->sh_degree + 1);
+    const uint64_t stride = s->view.stride;
+    const uint64_t chunk = 1u << 20;
+    if (d->format == GPS_FORMAT_FP32) {
+        const int nplanes = 11 + 3 * ncoef;
+        float* planes = (float*)s->planes;
+        std::vector<float> stage(chunk * (uint64_t)nplanes);
+        for (uint64_t b = 0; b < n_all; b += chunk) {
+            const uint64_t n = std::min(chunk, n_all - b);
+            parallel_range(n, [&](uint64_t lo, uint64_t hi) {
+                for (uint64_t i = lo; i < hi; ++i) {
+                    const uint64_t g = b + i;
+                    for (int c = 0; c < 3; ++c) stage[c * chunk + i] = d->means[3 * g + c];
+                    for (int c = 0; c < 4; ++c) stage[(3 + c) * chunk + i] = d->rotations[4 * g + c];
+                    for (int c = 0; c < 3; ++c) stage[(7 + c) * chunk + i] = d->scales[3 * g + c];
+                    stage[10 * chunk + i] = d->opacities[g];
+                    for (int c = 0; c < 3 * ncoef; ++c) stage[(11 + c) * chunk + i] = d->shs[(uint64_t)3 * ncoef * g + c];
+                }
+            });
+            for (int p = 0; p < nplanes; ++p) {
+                hipError_t e = hipMemcpy(planes + (uint64_t)p * stride + first + b, stage.data() + p * chunk,
+                                         n * sizeof(float), hipMemcpyHostToDevice);
+                if (e != hipSuccess) return hip_fail(e, "scene upload");
+            }
+        }
+    } else {
+        // GPS_FORMAT_HALF_FP8: encoded on the host (fp16 RNE, fp8 E4M3 RNE with a
+        // per-Gaussian scale = max|rest coefficient| / 448).
+        float* means = (float*)s->planes;
+        uint4* geo = (uint4*)s->geo;
+        uint4* shrec = (uint4*)s->shrec;
+        std::vector<float> mstage(3 * chunk);
+        std::vector<uint32_t> gstage(4 * chunk), sstage(16 * chunk);
+        for (uint64_t b = 0; b < n_all; b += chunk) {
+            const uint64_t n = std::min(chunk, n_all - b);
+            parallel_range(n, [&](uint64_t lo, uint64_t hi) {
+                uint8_t enc[64];
+                for (uint64_t i = lo; i < hi; ++i) {
+                    const uint64_t g = b + i;
+                    for (int c = 0; c < 3; ++c) mstage[c * chunk + i] = d->means[3 * g + c];
+                    uint16_t h[8];
+                    for (int k = 0; k < 4; ++k) h[k] = gps_float_to_half_rne(d->rotations[4 * g + k]);
+                    for (int k = 0; k < 3; ++k) h[4 + k] = gps_float_to_half_rne(d->scales[3 * g + k]);
+                    h[7] = gps_float_to_half_rne(d->opacities[g]);
+                    std::memcpy(&gstage[4 * i], h, 16);
+                    gps_encode_sh_record(d->shs + (uint64_t)3 * ncoef * g, ncoef, enc);
+                    std::memcpy(&sstage[16 * i], enc, 64);
+                }
+            });
+            hipError_t e = hipSuccess;
+            for (int c = 0; c < 3 && e == hipSuccess; ++c)
+                e = hipMemcpy(means + (uint64_t)c * stride + first + b, mstage.data() + c * chunk, n * 4,
+                              hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(geo + first + b, gstage.data(), n * 16, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(shrec + 4 * (first + b), sstage.data(), n * 64, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "scene upload");
+        }
+    }
+    return GPS_OK;
+}
+
+gps_status gps_scene_create(const gps_scene_desc* d, gps_scene** out) {
+    if (!d || !out) return GPS_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (d->sh_degree < 0 || d->sh_degree > 3) return GPS_ERR_INVALID_ARG;
+    if (d->num_gaussians > 0 && (!d->means || !d->rotations || !d->scales || !d->opacities || !d->shs))
+        return GPS_ERR_INVALID_ARG;
+    gps_scene* s = nullptr;
+    gps_status st = gps_scene_create_empty(d->num_gaussians, d->sh_degree, d->format, &s);
+    if (st != GPS_OK) return st;
+    st = gps_scene_upload_range(s, 0, d);
+    if (st != GPS_OK) {
+        gps_scene_destroy(s);
+        return st;
     }
     *out = s;
     return GPS_OK;
@@ -521,6 +569,9 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         da.coff = p->coff;
         da.chunk_start = p->chunk_start;
         da.iteration = &p->ctr->iteration;
+        da.cull = p->cull;
+        da.hier = f.hier;
+        da.N = p->N;
         stage_mark(p, 3, slot, 0, st);
         launch_distribute(da, st);
         stage_mark(p, 3, slot, 1, st);
@@ -563,6 +614,7 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         stage_mark(p, 6, 1, 1, st);
     }
     p->last_occl = fp->enable_occlusion_culling ? 1 : 0;
+    p->last_hier = f.hier;
     p->last_K = f.Ki;
     p->last_spp = f.spp;
     hipError_t e = hipGetLastError();
@@ -625,7 +677,11 @@ gps_status gps_pass_stats(gps_pass* p, gps_stats* out, void* stream) {
         out->samples += (uint64_t)c.pass[i].points * (uint64_t)p->last_K * (uint64_t)p->last_spp;
         if (c.pass[i].total > c.pass[i].points) out->budget_truncated = 1;
     }
-    out->reserved = (int32_t)c.atomics;
+    out->atomics = c.atomics;
+    if (p->hier_available) {
+        out->gaussians_per_block = (int32_t)p->cull.gpb;
+        out->num_blocks = p->cull.node_count;
+    }
     return GPS_OK;
 }
 
@@ -691,6 +747,16 @@ gps_status gps_pass_debug_copy(gps_pass* p, int32_t which, void* dst, uint64_t c
         default: return GPS_ERR_INVALID_ARG;
     }
     GPS_HIP(hipMemcpy(dst, src, count * esz, hipMemcpyDeviceToHost));
+    if (which == GPS_DBG_WEIGHTS && p->last_hier) {
+        // Slots of culled blocks were not written this pass: their weight is 0.
+        std::vector<uint32_t> m(p->cull.mask_words);
+        GPS_HIP(hipMemcpy(m.data(), p->cull_mask, m.size() * 4, hipMemcpyDeviceToHost));
+        uint32_t* w = (uint32_t*)dst;
+        for (uint64_t g = 0; g < count; ++g) {
+            const uint64_t b = g / p->cull.gpb;
+            if ((m[b >> 5] >> (b & 31)) & 1u) w[g] = 0;
+        }
+    }
     return GPS_OK;
 }
 

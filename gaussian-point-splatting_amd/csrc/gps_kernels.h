@@ -32,6 +32,9 @@ struct DistributeArgs {
     uint32_t* coff;
     uint32_t* chunk_start;
     uint32_t* iteration;
+    CullView cull;
+    int32_t hier;
+    uint32_t N;
 };
 
 struct SplatArgs {

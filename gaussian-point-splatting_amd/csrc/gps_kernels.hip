@@ -260,10 +260,11 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
     const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
     uint32_t n = 0;
-    if (g < s.N) {
+    // Gaussians of culled blocks leave their weight slot untouched: the
+    // distributor reads the same cull mask and counts them as zero.
+    if (g < s.N && !(f.hier && block_culled(cv, g))) {
         u32x2 seed = make_seed(g, f.frame_seed);
         do {
-            if (f.hier && block_culled(cv, g)) break;
             if (f.only_prev && mask_bit(mask_prev, g)) break;
             const Geo ge = load_geo<FORMAT>(s, g);
             const float op = glm_min(ge.op, 1.0f);
@@ -400,24 +401,46 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
 // entry of every splat chunk.  Equivalent to the reference's scatter +
 // inclusive max-scan: point p belongs to the last valid Gaussian whose offset
 // is <= p.
+// Tiles without valid Gaussians are skipped whole; with block culling on,
+// weights of culled blocks are taken as zero without being read (preprocess
+// does not write them).
 __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restrict__ weights,
                                                          const uint32_t* __restrict__ tile_off,
-                                                         const uint32_t* __restrict__ tile_voff,
+                                                         const uint32_t* __restrict__ tile_voff, uint32_t ntiles,
                                                          const PassCounters* __restrict__ ctr, uint32_t* __restrict__ cidx,
-                                                         uint32_t* __restrict__ coff, uint32_t* __restrict__ chunk_start) {
+                                                         uint32_t* __restrict__ coff, uint32_t* __restrict__ chunk_start,
+                                                         CullView cv, int hier, uint32_t N) {
     __shared__ uint32_t s_a[4], s_c[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t tile = blockIdx.x;
+    const uint32_t vend = tile + 1 < ntiles ? tile_voff[tile + 1] : ctr->valid;
+    if (vend == tile_voff[tile]) return;  // uniform over the workgroup
     const uint32_t g0 = (tile << GPS_TILE_SHIFT) + (uint32_t)tid * 16u;
+    // bit k set: element g0 + k counts (inside the scene and not culled)
+    uint32_t live = g0 >= N ? 0u : (N - g0 >= 16u ? 0xFFFFu : ((1u << (N - g0)) - 1u));
+    if (hier && live) {
+        uint32_t b = g0 / cv.gpb;
+        uint32_t next = (b + 1u) * cv.gpb;
+        bool c = mask_bit(cv.mask, b);
+        const int kmax = N - g0 >= 16u ? 16 : (int)(N - g0);
+        for (int k = 0; k < kmax; ++k) {
+            if (g0 + (uint32_t)k == next) {
+                ++b;
+                next += cv.gpb;
+                c = mask_bit(cv.mask, b);
+            }
+            if (c) live &= ~(1u << k);
+        }
+    }
     const uint4* wp = reinterpret_cast<const uint4*>(weights + g0);
     uint32_t w[16];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const uint4 v = wp[k];
-        w[4 * k] = v.x;
-        w[4 * k + 1] = v.y;
-        w[4 * k + 2] = v.z;
-        w[4 * k + 3] = v.w;
+        const uint4 v = ((live >> (4 * k)) & 15u) ? wp[k] : make_uint4(0u, 0u, 0u, 0u);
+        w[4 * k] = (live >> (4 * k)) & 1u ? v.x : 0u;
+        w[4 * k + 1] = (live >> (4 * k + 1)) & 1u ? v.y : 0u;
+        w[4 * k + 2] = (live >> (4 * k + 2)) & 1u ? v.z : 0u;
+        w[4 * k + 3] = (live >> (4 * k + 3)) & 1u ? v.w : 0u;
     }
     uint32_t sa = 0, sc = 0;
 #pragma unroll
@@ -945,8 +968,8 @@ void launch_distribute(const DistributeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, st, a.tile_sum, a.tile_cnt, a.ntiles, a.tile_off,
                        a.tile_voff, a.ctr, a.budget, a.iteration);
     if (a.ntiles == 0) return;
-    hipLaunchKernelGGL(distribute_kernel, dim3(a.ntiles), dim3(256), 0, st, a.weights, a.tile_off, a.tile_voff, a.ctr,
-                       a.cidx, a.coff, a.chunk_start);
+    hipLaunchKernelGGL(distribute_kernel, dim3(a.ntiles), dim3(256), 0, st, a.weights, a.tile_off, a.tile_voff,
+                       a.ntiles, a.ctr, a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.N);
 }
 
 void launch_splat(const SplatArgs& a, hipStream_t st) {

@@ -42,12 +42,12 @@ uint64_t mix(uint64_t a, uint64_t b) {
 
 namespace {
 
-void generate_range(const gps_synth_params* p, uint64_t first, uint64_t k0, uint64_t k1, float* means, float* rotations,
-                    float* scales, float* opacities, float* shs) {
+void generate_range(const gps_synth_params* p, uint64_t first, const uint64_t* indices, uint64_t k0, uint64_t k1,
+                    float* means, float* rotations, float* scales, float* opacities, float* shs) {
     const int ncoef = (p->sh_degree + 1) * (p->sh_degree + 1);
     const double h = p->half_extent;
     for (uint64_t k = k0; k < k1; ++k) {
-        const uint64_t g = first + k;
+        const uint64_t g = indices ? indices[k] : first + k;
         SplitMix64 r{mix(p->seed, g)};
         double pos[3];
         if (p->kind == 0) {
@@ -58,6 +58,7 @@ void generate_range(const gps_synth_params* p, uint64_t first, uint64_t k0, uint
             for (int c = 0; c < 3; ++c) pos[c] = -h + 2.0 * h * cr.uniform() + p->cluster_sigma * r.normal();
         }
         for (int c = 0; c < 3; ++c) means[3 * k + c] = (float)(pos[c] + p->offset[c]);
+        if (!rotations) continue;  // positions only
         double q[4], n2 = 0.0;
         for (int c = 0; c < 4; ++c) {
             q[c] = r.normal();
@@ -75,11 +76,16 @@ void generate_range(const gps_synth_params* p, uint64_t first, uint64_t k0, uint
 
 }  // namespace
 
-extern "C" gps_status gps_synth_generate(const gps_synth_params* p, uint64_t first, uint64_t count, float* means,
-                                         float* rotations, float* scales, float* opacities, float* shs) {
-    if (!p || !means || !rotations || !scales || !opacities || !shs) return GPS_ERR_INVALID_ARG;
+static gps_status generate(const gps_synth_params* p, uint64_t first, const uint64_t* indices, uint64_t count,
+                           float* means, float* rotations, float* scales, float* opacities, float* shs) {
+    if (!p || !means) return GPS_ERR_INVALID_ARG;
+    const bool means_only = !rotations && !scales && !opacities && !shs;
+    if (!means_only && (!rotations || !scales || !opacities || !shs)) return GPS_ERR_INVALID_ARG;
     if (p->sh_degree < 0 || p->sh_degree > 3 || (p->kind == 1 && p->clusters <= 0)) return GPS_ERR_INVALID_ARG;
-    if (first + count > p->num_gaussians) return GPS_ERR_INVALID_ARG;
+    if (!indices && first + count > p->num_gaussians) return GPS_ERR_INVALID_ARG;
+    if (indices)
+        for (uint64_t k = 0; k < count; ++k)
+            if (indices[k] >= p->num_gaussians) return GPS_ERR_INVALID_ARG;
     unsigned nt = std::thread::hardware_concurrency();
     nt = nt == 0 ? 1 : (nt > 32 ? 32 : nt);
     if (count < 65536) nt = 1;
@@ -89,8 +95,20 @@ extern "C" gps_status gps_synth_generate(const gps_synth_params* p, uint64_t fir
         const uint64_t k0 = (uint64_t)t * chunk;
         if (k0 >= count) break;
         const uint64_t k1 = k0 + chunk < count ? k0 + chunk : count;
-        ts.emplace_back(generate_range, p, first, k0, k1, means, rotations, scales, opacities, shs);
+        ts.emplace_back(generate_range, p, first, indices, k0, k1, means, rotations, scales, opacities, shs);
     }
     for (auto& t : ts) t.join();
     return GPS_OK;
+}
+
+extern "C" gps_status gps_synth_generate(const gps_synth_params* p, uint64_t first, uint64_t count, float* means,
+                                         float* rotations, float* scales, float* opacities, float* shs) {
+    return generate(p, first, nullptr, count, means, rotations, scales, opacities, shs);
+}
+
+extern "C" gps_status gps_synth_generate_indices(const gps_synth_params* p, const uint64_t* indices, uint64_t count,
+                                                 float* means, float* rotations, float* scales, float* opacities,
+                                                 float* shs) {
+    if (!indices && count) return GPS_ERR_INVALID_ARG;
+    return generate(p, 0, indices, count, means, rotations, scales, opacities, shs);
 }

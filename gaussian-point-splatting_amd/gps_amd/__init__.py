@@ -91,14 +91,16 @@ class Stats(C.Structure):
     _fields_ = [("num_gaussians", C.c_uint64), ("valid_gaussians", C.c_uint64 * 2),
                 ("total_weight", C.c_uint64 * 2), ("num_points", C.c_uint64 * 2), ("samples", C.c_uint64),
                 ("culled_blocks", C.c_uint64 * 2), ("passes", C.c_int32), ("budget_truncated", C.c_int32),
-                ("K", C.c_int32), ("reserved", C.c_int32)]
+                ("K", C.c_int32), ("gaussians_per_block", C.c_int32), ("atomics", C.c_uint64),
+                ("num_blocks", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {"num_gaussians": int(self.num_gaussians), "valid_gaussians": list(self.valid_gaussians),
                 "total_weight": list(self.total_weight), "num_points": list(self.num_points),
                 "samples": int(self.samples), "culled_blocks": list(self.culled_blocks),
                 "passes": int(self.passes), "budget_truncated": int(self.budget_truncated), "K": int(self.K),
-                "atomics": int(self.reserved) & 0xFFFFFFFF}
+                "gpb": int(self.gaussians_per_block), "num_blocks": int(self.num_blocks),
+                "atomics": int(self.atomics)}
 
 
 class SynthParams(C.Structure):
@@ -124,6 +126,8 @@ _sig("gps_abi_version", C.c_int32)
 _sig("gps_status_str", C.c_char_p, C.c_int32)
 _sig("gps_last_error", C.c_char_p)
 _sig("gps_scene_create", C.c_int32, C.POINTER(SceneDesc), C.POINTER(_vp))
+_sig("gps_scene_create_empty", C.c_int32, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(_vp))
+_sig("gps_scene_upload_range", C.c_int32, _vp, C.c_uint64, C.POINTER(SceneDesc))
 _sig("gps_scene_destroy", None, _vp)
 _sig("gps_scene_device_bytes", C.c_uint64, _vp)
 _sig("gps_pass_create", C.c_int32, C.POINTER(RenderSettingsC), _vp, C.POINTER(_vp))
@@ -143,6 +147,7 @@ _sig("gps_frame_params_init", None, C.POINTER(FrameParams), _fp, _fp, _fp, C.c_f
 _sig("gps_taa_reprojection", None, _fp, _fp, _fp, _fp)
 _sig("gps_mat4_mul", None, _fp, _fp, _fp)
 _sig("gps_synth_generate", C.c_int32, C.POINTER(SynthParams), C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp, _vp)
+_sig("gps_synth_generate_indices", C.c_int32, C.POINTER(SynthParams), _vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp)
 _sig("gps_morton_order", C.c_int32, _vp, C.c_uint64, _vp)
 _sig("gps_encode_half_fp8", C.c_int32, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("gps_float_to_half", C.c_uint16, C.c_float)
@@ -391,7 +396,10 @@ def encode_half_fp8(s: SceneArrays) -> dict:
 class Scene:
     """A scene resident in HBM."""
 
-    def __init__(self, arrays: SceneArrays, fmt: int = FORMAT_FP32):
+    def __init__(self, arrays: Optional[SceneArrays], fmt: int = FORMAT_FP32):
+        self.handle = None
+        if arrays is None:
+            return
         h = C.c_void_p()
         d = arrays.desc(fmt)
         check(_lib.gps_scene_create(C.byref(d), C.byref(h)), "scene create")
@@ -399,6 +407,22 @@ class Scene:
         self.n = arrays.n
         self.format = fmt
         self.sh_degree = arrays.sh_degree
+
+    @classmethod
+    def empty(cls, n: int, sh_degree: int, fmt: int = FORMAT_FP32) -> "Scene":
+        """An allocated scene of n Gaussians, filled by upload_range."""
+        s = cls(None, fmt)
+        h = C.c_void_p()
+        check(_lib.gps_scene_create_empty(n, sh_degree, fmt, C.byref(h)), "scene create")
+        s.handle = h
+        s.n = n
+        s.format = fmt
+        s.sh_degree = sh_degree
+        return s
+
+    def upload_range(self, first: int, arrays: SceneArrays) -> None:
+        d = arrays.desc(self.format)
+        check(_lib.gps_scene_upload_range(self.handle, first, C.byref(d)), "scene upload")
 
     @property
     def device_bytes(self) -> int:

@@ -115,7 +115,9 @@ typedef struct gps_stats {
     int32_t passes;                /* 1 or 2 */
     int32_t budget_truncated;      /* 1 if the point budget cut points */
     int32_t K;                     /* samples per point per layer */
-    int32_t reserved;
+    int32_t gaussians_per_block;   /* culling block size in use (0 when block culling is off) */
+    uint64_t atomics;              /* framebuffer atomicMin operations issued (diagnostic) */
+    uint64_t num_blocks;           /* culling blocks (0 when block culling is off) */
 } gps_stats;
 
 /* -------------------------------------------------------------- versioning */
@@ -128,6 +130,12 @@ const char* gps_last_error(void);
 /* Uploads a scene into HBM.  Replaces GPUGaussianScene*::upload_from_cpu
  * (gaussian_types.cuh:442-503, 568-608). */
 gps_status gps_scene_create(const gps_scene_desc* desc, gps_scene** out);
+/* Chunked upload of large scenes (the reference uploads in 1M chunks,
+ * gaussian_types.cuh:454-501): create an empty scene of `num_gaussians`
+ * (desc arrays unused), then fill index ranges with gps_scene_upload_range,
+ * whose desc holds `count` Gaussians for [first, first + count). */
+gps_status gps_scene_create_empty(uint64_t num_gaussians, int32_t sh_degree, int32_t format, gps_scene** out);
+gps_status gps_scene_upload_range(gps_scene* scene, uint64_t first, const gps_scene_desc* desc);
 void gps_scene_destroy(gps_scene* scene);
 /* Device bytes held by the scene. */
 uint64_t gps_scene_device_bytes(const gps_scene* scene);
@@ -253,6 +261,12 @@ typedef struct gps_synth_params {
 } gps_synth_params;
 gps_status gps_synth_generate(const gps_synth_params* p, uint64_t first, uint64_t count, float* means,
                               float* rotations, float* scales, float* opacities, float* shs);
+/* With rotations, scales, opacities and shs all NULL only the positions are
+ * written (cheap pass for computing a load-time order of a large scene). */
+/* Same Gaussians for an arbitrary index list: Gaussian `indices[k]` of the
+ * scene into slot k (each Gaussian has its own stream, so order is free). */
+gps_status gps_synth_generate_indices(const gps_synth_params* p, const uint64_t* indices, uint64_t count,
+                                      float* means, float* rotations, float* scales, float* opacities, float* shs);
 
 /* Load-time Morton ordering, the reference's --sort-morton-order
  * (src/core/gaussians/gaussian_loader.cu:363-426): writes perm[new] = old for

@@ -761,6 +761,14 @@ int32_t orc_pass_debug_copy(orc_pass* p, int32_t which, void* dst, uint64_t coun
     return GPS_ERR_INVALID_ARG;
 }
 
+int32_t orc_pass_cull(orc_pass* p, const gps_frame_params* fp) {
+    if (!p || !fp || p->nodes.empty()) return GPS_ERR_INVALID_ARG;
+    Frame f;
+    p->make_frame(fp, f);
+    p->compute_occlusion_mask(f, fp->enable_occlusion_culling != 0);
+    return GPS_OK;
+}
+
 int32_t orc_project(orc_pass* p, const gps_frame_params* fp, int32_t use_fp64, double* out) {
     Frame f;
     p->make_frame(fp, f);

@@ -55,6 +55,9 @@ void orc_pass_stage_ms(orc_pass* pass, double out[8]);
  * max x/y (FP32 only).  ok = 1 if the
  * Gaussian passes projection + bounds; colour is evaluated for every such
  * Gaussian. */
+/* Block culling alone (compute_occlusion_mask) against the Hi-Z chain the
+ * pass currently holds; the mask is then read with GPS_DBG_CULL_MASK. */
+int32_t orc_pass_cull(orc_pass* pass, const gps_frame_params* frame);
 int32_t orc_project(orc_pass* pass, const gps_frame_params* frame, int32_t use_fp64, double* out);
 
 /* Resolve kernels (post_processing_kernels.cuh). */

@@ -61,6 +61,7 @@ _sig("orc_pass_stats", C.c_int32, _vp, C.POINTER(Stats))
 _sig("orc_pass_debug_size", C.c_uint64, _vp, C.c_int32)
 _sig("orc_pass_debug_copy", C.c_int32, _vp, C.c_int32, _vp, C.c_uint64)
 _sig("orc_pass_stage_ms", None, _vp, C.POINTER(C.c_double))
+_sig("orc_pass_cull", C.c_int32, _vp, C.POINTER(FrameParams))
 _sig("orc_project", C.c_int32, _vp, C.POINTER(FrameParams), C.c_int32, _vp)
 _sig("orc_resolve_taa", None, _vp, _vp, _vp, _vp, C.POINTER(C.c_float), C.c_float, C.c_int32, C.c_int32, C.c_int32,
      _vp, _vp)
@@ -154,6 +155,12 @@ class OraclePass:
         if st != 0:
             raise RuntimeError(f"oracle debug copy {which} failed: {st}")
         return out
+
+    def cull(self, frame: FrameParams) -> np.ndarray:
+        """Block-cull mask for `frame` against the Hi-Z chain held now."""
+        if _lib.orc_pass_cull(self.handle, C.byref(frame)) != 0:
+            raise RuntimeError("oracle cull failed (block culling disabled?)")
+        return self.debug(gps_amd.DBG_CULL_MASK)
 
     def project(self, frame: FrameParams, fp64: bool) -> np.ndarray:
         n = self._keep.n

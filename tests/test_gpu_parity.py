@@ -214,6 +214,52 @@ def test_frames_half_fp8(gpu):
              which=H.INTERMEDIATES + [H.G.DBG_HIZ, H.G.DBG_CULL_MASK])
 
 
+def test_c3_construction_streamed_culling_taa(gpu):
+    """The 100M-Gaussian workload's construction at test size: 4 translated
+    copies of a clustered SH3 scene, Morton load order over the whole scene,
+    compact fp16/fp8 storage uploaded in several ranges, blocks + Hi-Z
+    culling and TAA along the fly-through (in order, then two camera cuts).
+    Every frame's framebuffer, colour, depth and TAA output bit-exact."""
+    import orc
+    from gps_amd import workloads as W
+    w = W.scaled(W.WORKLOADS["c3"], 30_000, width=320, height=180, gaussians_per_block=64)
+    perm = W.order(w)
+    arrays = W.host_scene(w, perm)
+    scene = W.device_scene(w, perm, chunk=25_000)  # 5 upload ranges, last one ragged
+    loop = W.FrameLoop(w, scene)
+    o = orc.OraclePass(w.settings(), arrays, w.fmt, 8)
+    h_color = [np.zeros((w.height, w.width, 3), np.float32) for _ in range(2)]
+    h_depth = [np.zeros((w.height, w.width), np.float32) for _ in range(2)]
+    h_hist = [np.zeros((w.height, w.width, 3), np.float32) for _ in range(2)]
+    prev_vp = np.zeros(16, np.float32)
+    culled = second = 0
+    for k, pos in enumerate([0, 1, 2, 3, 150, 151, 299]):
+        f = W.frame(w, pos, k + 1)
+        flush(f"c3 tiny frame {k} (camera {pos})")
+        loop.step(f)
+        fg = loop.pass_.debug(gpu.DBG_FRAMEBUFFER, loop.stream)
+        sg = loop.pass_.stats(loop.stream)
+        c, d = o.render(f)
+        a, b = (k + 1) % 2, k % 2
+        h_color[a], h_depth[b] = c, d
+        M = gpu.taa_reprojection(prev_vp, np.array(f.view, np.float32), np.array(f.proj, np.float32))
+        h_hist[a], rgb = orc.resolve_taa(h_color[a], h_depth[a], h_hist[b], h_depth[b], M, 0.9, w.width, w.height,
+                                         k + 1)
+        prev_vp = gpu.mat4_mul(np.array(f.proj, np.float32), np.array(f.view, np.float32))
+        so = o.stats()
+        for key in ("valid_gaussians", "num_points", "culled_blocks", "passes"):
+            assert sg[key] == so[key], (k, key, sg[key], so[key])
+        assert_same(f"c3 fb {k}", fg, o.debug(gpu.DBG_FRAMEBUFFER))
+        assert_same(f"c3 color {k}", loop.color[a].download(), c)
+        assert_same(f"c3 depth {k}", loop.depth[b].download(), d)
+        assert_same(f"c3 taa {k}", loop.hist[a].download(), h_hist[a])
+        assert_same(f"c3 rgb {k}", loop.rgb.download(), rgb)
+        culled += sum(sg["culled_blocks"][:sg["passes"]])
+        second += sg["num_points"][1] if sg["passes"] == 2 else 0
+    flush(f"c3 tiny: culled blocks {culled}, second-pass points {second}")
+    assert culled > 0
+
+
 # ------------------------------------------------------------------ resolve
 def test_resolve_taa_and_progressive(gpu):
     import orc

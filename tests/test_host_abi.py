@@ -153,3 +153,38 @@ def test_synthetic_generator_deterministic_and_chunk_independent():
     u = G.synth_scene(G.synth_params(8, 20000, kind="uniform", half_extent=10.0))
     assert np.abs(u.means).max() <= 10.0
     assert abs(np.log(u.scales).mean() + 4.0) < 0.02 and abs(np.log(u.scales).std() - 0.5) < 0.02
+
+
+def test_synthetic_gather_and_positions_only():
+    from gps_amd import workloads as W
+    w = W.scaled(W.WORKLOADS["c3"], 5000)
+    p = W.params(w)
+    base = G.synth_scene(p)
+    idx = np.array([4999, 0, 17, 17, 5000 + 3, 3 * 5000 + 42], np.uint64)
+    got = W._generate(w, idx)
+    g = (idx % 5000).astype(np.int64)
+    shift = (idx // 5000).astype(np.float32) * np.float32(250.0)
+    assert np.array_equal(got.means[:, 0], base.means[g, 0] + shift)
+    assert np.array_equal(got.means[:, 1:], base.means[g, 1:])
+    for name in ("rotations", "scales", "opacities", "shs"):
+        assert np.array_equal(getattr(got, name), getattr(base, name)[g]), name
+    assert np.array_equal(W._generate(w, idx, means_only=True), got.means)
+    # whole-scene Morton order is a permutation and is spatially coherent
+    perm = W.order(w)
+    assert np.array_equal(np.sort(perm), np.arange(w.n, dtype=np.uint64))
+    m = W._generate(w, perm, means_only=True)
+    step = np.linalg.norm(np.diff(m, axis=0), axis=1)
+    rnd = np.linalg.norm(np.diff(m[np.random.default_rng(0).permutation(w.n)], axis=0), axis=1)
+    assert np.median(step) < 0.1 * np.median(rnd)
+    bad = np.array([w.n], np.uint64)
+    with pytest.raises(G.GpsError):
+        G.check(G.lib.gps_synth_generate_indices(C.byref(p), G._ptr(bad % np.uint64(10**9)), 1,
+                                                 G._ptr(np.empty((1, 3), np.float32)), None, None, None, None),
+                "synth")
+
+
+def test_streamed_scene_upload_validation():
+    h = C.c_void_p()
+    assert G.lib.gps_scene_create_empty(10, 4, 0, C.byref(h)) == 1
+    assert G.lib.gps_scene_create_empty(10, 0, 7, C.byref(h)) == 1
+    assert G.lib.gps_scene_upload_range(None, 0, None) == 1
