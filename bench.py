@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gaussian-point-splatting_amd"))
 
@@ -96,6 +98,123 @@ def cpu_baseline(G, W, w, sample_n, threads, perm):
             "frame_s": dt}
 
 
+def serving(G, W, args, resident):
+    """Serving workload: resident 1M / 10M / 100M scenes, Poisson requests at
+    200/s of mixed resolution and spp for `--serve-seconds`, admitted by the
+    native scheduler into `--serve-slots` HIP-stream slots.  Also the same
+    request mix closed-loop on 1 slot (one request at a time, the plain form)
+    and on all slots, for the throughput ratio."""
+    sw = W.replace(W.SERVE, seconds=float(args.serve_seconds))
+    scenes = []
+    for w in sw.scenes:
+        if w.name in resident:
+            scenes.append(resident[w.name])
+        else:
+            log(f"[serve] building {w.name} ({w.n} Gaussians)")
+            scenes.append(W.device_scene(w, log=lambda m: log("[serve] " + m)))
+    mw = max(r[0] for r in sw.resolutions)
+    mh = max(r[1] for r in sw.resolutions)
+
+    def server(slots):
+        srv = G.Server(slots=slots, max_width=mw, max_height=mh, max_spp=max(sw.spps), max_supersampling=sw.ss)
+        ids = [srv.add_scene(sc) for sc in scenes]
+        srv.prepare()
+        srv.start()
+        return srv, ids
+
+    def run(slots, reqs, lead):
+        srv, ids = server(slots)
+        base = srv.clock() + lead
+        for k, f, t in reqs:
+            srv.submit(ids[k], f, base + t)
+        srv.drain(900_000)
+        res = srv.poll(len(reqs) + 8)
+        st = srv.stats()
+        srv.close()
+        bad = [r for r in res if r["status"] != 0]
+        if bad or len(res) != len(reqs):
+            raise RuntimeError(f"serving: {len(bad)} failed, {len(res)} of {len(reqs)} done")
+        return W.serve_stats(res, base), st
+
+    closed = W.serve_requests(sw, count=args.serve_closed, rate=0.0)
+    log(f"[serve] closed loop, {len(closed)} requests, 1 slot (plain form)")
+    plain, _ = run(1, closed, 0.2)
+    log(f"[serve] plain: {plain['frames_per_s']:.1f} frames/s")
+    log(f"[serve] closed loop, {args.serve_slots} slots")
+    batched, _ = run(args.serve_slots, closed, 0.2)
+    log(f"[serve] batched: {batched['frames_per_s']:.1f} frames/s")
+    reqs = W.serve_requests(sw)
+    log(f"[serve] open loop: {len(reqs)} requests at {sw.rate:.0f}/s over {sw.seconds:.0f}s, {args.serve_slots} slots")
+    live, st = run(args.serve_slots, reqs, 1.0)
+    log(f"[serve] p50 {live['p50_ms']:.2f} ms p99 {live['p99_ms']:.2f} ms, {live['frames_per_s']:.1f} frames/s")
+    return {"workload": "c5: resident 1M fp32 / 10M fp32 / 100M compact scenes; Poisson arrivals "
+                        f"{sw.rate:.0f}/s for {sw.seconds:.0f} s; 854x480 / 1280x720 / 1920x1080, spp 1 / 4, ss 2, "
+                        "random sphere cameras; frustum block culling, no occlusion culling",
+            "slots": args.serve_slots, "requests": live["requests"], "frames_per_s": round(live["frames_per_s"], 2),
+            "p50_ms": round(live["p50_ms"], 3), "p99_ms": round(live["p99_ms"], 3), "max_ms": round(live["max_ms"], 3),
+            "closed_loop_requests": len(closed), "plain_frames_per_s": round(plain["frames_per_s"], 2),
+            "batched_frames_per_s": round(batched["frames_per_s"], 2),
+            "batched_over_plain": round(batched["frames_per_s"] / plain["frames_per_s"], 3),
+            "closed_loop_p99_ms": round(batched["p99_ms"], 3), "max_batch": int(st["max_batch"]),
+            "hbm_used_gb": round(st["hbm_used_bytes"] / 1e9, 2)}
+
+
+def batched(G, W, args):
+    """Batched views: the 10M SH3 scene in compact form, 64 views per batch
+    on a sphere of radius 150, resolution and spp drawn per view, ss 2.
+    Plain form: the views rendered one after another through one pass.
+    Batched form: one gps_batch_render per batch (autotuned splat variant,
+    replayed from a HIP graph).  Both render the same frames."""
+    bw = W.replace(W.BATCH, views=args.batch_views)
+    log(f"[batch] building {bw.scene.name} compact scene ({bw.scene.n} Gaussians)")
+    scene = W.device_scene(bw.scene, log=lambda m: log("[batch] " + m))
+    nb = args.batch_count
+    frames = [W.batch_views(bw, b) for b in range(nb + 1)]
+    stream = G.Stream()
+    cols = [G.DeviceArray((f.height, f.width, 3), np.float32) for f in frames[0]]
+    deps = [G.DeviceArray((f.height, f.width), np.float32) for f in frames[0]]
+    mw = max(r[0] for r in bw.resolutions)
+    mh = max(r[1] for r in bw.resolutions)
+    single = G.Pass(G.RenderSettings(mw, mh, max(bw.spps), bw.ss, flexible_frame_size=True), scene)
+    for f, c, d in zip(frames[0], cols, deps):
+        single.render(f, c, d, stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for b in range(1, nb + 1):
+        for f, c, d in zip(frames[b], cols, deps):
+            single.render(f, c, d, stream)
+    stream.synchronize()
+    plain_s = time.perf_counter() - t0
+    single.close()
+    log(f"[batch] plain: {nb * bw.views} frames in {plain_s * 1e3:.1f} ms")
+    batch = G.Batch(scene, bw.views, mw, mh, max(bw.spps), bw.ss)
+    tuned = batch.autotune(frames[0], cols, deps, stream, repeats=2)
+    log(f"[batch] autotune: {tuned}")
+    batch.set_tuning(tuned["splat_threads"], tuned["splat_points_per_thread"], True)
+    batch.render(frames[0], cols, deps, stream)  # capture
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for b in range(1, nb + 1):
+        batch.render(frames[b], cols, deps, stream)
+    stream.synchronize()
+    batch_s = time.perf_counter() - t0
+    samples = sum(batch.view_stats(v, stream)["samples"] for v in range(bw.views))
+    log(f"[batch] batched: {nb * bw.views} frames in {batch_s * 1e3:.1f} ms")
+    out = {"workload": f"c4: {bw.scene.n} clustered SH3 Gaussians, compact format, {bw.views} views per batch on a "
+                       f"sphere of radius {bw.radius:.0f}, resolutions 854x480 / 1280x720 / 1920x1080 and spp 1 / 2 / 4 "
+                       f"per view, ss {bw.ss}, block frustum culling",
+           "views_per_batch": bw.views, "batches": nb,
+           "plain_frames_per_s": round(nb * bw.views / plain_s, 2),
+           "batched_frames_per_s": round(nb * bw.views / batch_s, 2),
+           "batched_over_plain": round(plain_s / batch_s, 3),
+           "batched_ms_per_batch": round(batch_s * 1e3 / nb, 3),
+           "points_per_batch": samples, "batched_gpoints_per_s": round(samples * nb / batch_s / 1e9, 3),
+           "tuning": tuned, "graph": True, "device_gb": round(batch.device_bytes / 1e9, 2)}
+    batch.close()
+    scene.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +225,11 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-sort", action="store_true",
                     help="keep generation order instead of the load-time Morton order")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="serving measurement (0 = skip)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-closed", type=int, default=300, help="requests of the closed-loop comparison")
+    ap.add_argument("--batch-views", type=int, default=64, help="views per batch (0 = skip the batched form)")
+    ap.add_argument("--batch-count", type=int, default=5, help="timed batches")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
@@ -190,6 +314,12 @@ def main() -> int:
         sample_n = args.cpu_sample or min(w.n, 2_000_000)
         log(f"[bench] cpu baseline on {sample_n} Gaussians, {threads} threads")
         cpu = cpu_baseline(G, W, w, sample_n, threads, perm)
+    batch_report = None
+    if args.batch_views > 0:
+        batch_report = batched(G, W, args)
+    serve = None
+    if args.serve_seconds > 0:
+        serve = serving(G, W, args, {w.name: scene})
     passes = [st["passes"] for st in per_frame]
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Gpoints/s", "n_gpus": 1, "steps": K, "warmup": Wm,
@@ -208,6 +338,8 @@ def main() -> int:
         "culled_blocks_per_frame": round(sum(sum(st["culled_blocks"][:st["passes"]]) for st in per_frame) / K),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batch_report,
+        "serving": serve,
         "stages": stage_report,
     }
     print(json.dumps(out), flush=True)

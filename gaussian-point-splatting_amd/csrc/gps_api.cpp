@@ -63,14 +63,6 @@ namespace {
 
 }  // namespace
 
-struct gps_scene {
-    DevScene view{};
-    void* planes = nullptr;  // FP32 planes or HALF_FP8 means
-    void* geo = nullptr;
-    void* shrec = nullptr;
-    uint64_t bytes = 0;
-};
-
 struct gps_pass {
     gps_scene* scene = nullptr;
     gps_render_settings settings{};
@@ -90,7 +82,8 @@ struct gps_pass {
     // framebuffers
     uint64_t* fbA = nullptr;
     uint64_t* fbB = nullptr;
-    uint64_t fb_words = 0;
+    uint64_t fb_words = 0;  // words of the current frame
+    uint64_t fb_cap = 0;    // words allocated per buffer
     int fb_ss = -1;
     uint64_t* cur_fb = nullptr;
     // occlusion
@@ -135,10 +128,15 @@ uint64_t mip_total(HiZView& h, int w, int hgt) {
     return total;
 }
 
-gps_status ensure_fb(gps_pass* p, int ss) {
+// Allocates FB A/B for frames of supersampling `ss` up to the pass's size and
+// spp (reallocated only when ss changes); `frame_words` is this frame's part.
+gps_status ensure_fb(gps_pass* p, int ss, uint64_t frame_words) {
     const int uw = p->settings.width * ss, uh = p->settings.height * ss;
     const uint64_t words = fb_words(uw, uh, p->settings.render_pass_count);
-    if (words == p->fb_words && p->fbA) return GPS_OK;
+    if (words == p->fb_cap && p->fbA) {
+        p->fb_words = frame_words;
+        return GPS_OK;
+    }
     if (words >= (1ull << 31)) {
         g_last_error = "framebuffer exceeds 2^31 words";
         return GPS_ERR_UNSUPPORTED;
@@ -149,7 +147,8 @@ gps_status ensure_fb(gps_pass* p, int ss) {
     if (s != GPS_OK) return s;
     s = dalloc(&p->fbB, words, false);
     if (s != GPS_OK) return s;
-    p->fb_words = words;
+    p->fb_cap = words;
+    p->fb_words = frame_words;
     p->fb_ss = ss;
     return GPS_OK;
 }
@@ -415,7 +414,10 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     }
     p->cull.nodes = p->nodes;
     p->cull.mask = p->cull_mask;
-    if (s == GPS_OK) s = ensure_fb(p, st->supersampling_factor);
+    if (s == GPS_OK)
+        s = ensure_fb(p, st->supersampling_factor,
+                      fb_words(st->width * st->supersampling_factor, st->height * st->supersampling_factor,
+                               st->render_pass_count));
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t prop;
@@ -436,6 +438,36 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     }
     *out = p;
     return GPS_OK;
+}
+
+uint64_t gps_pass_estimate_bytes(const gps_render_settings* st, const gps_scene* scene) {
+    if (!st || !scene) return 0;
+    const uint64_t N = scene->view.N;
+    const uint64_t ntiles = (N + GPS_TILE - 1) / GPS_TILE, npad = ntiles * GPS_TILE, nwords = (N + 31) / 32 + 2;
+    const uint64_t budget = (uint64_t)std::min<int64_t>(st->max_number_of_points, 0x7FFFFFFFll);
+    HiZView h{};
+    const uint64_t hiz = mip_total(h, st->width, st->height);
+    const uint64_t fb = fb_words(st->width * st->supersampling_factor, st->height * st->supersampling_factor,
+                                 st->render_pass_count);
+    uint64_t b = npad * 4 + 2 * nwords * 4 + std::max<uint64_t>(N, 1) * (sizeof(SplatRecord) + 8) +
+                 16 * ntiles + (budget / GPS_SPLAT_CHUNK + 2) * 4 + sizeof(FrameCounters) + hiz * 4 + 2 * fb * 8;
+    if (!st->fully_disable_hierarchical_culling) {
+        const uint64_t gpb = st->gaussians_per_block > 0 ? (uint64_t)st->gaussians_per_block
+                                                         : (uint64_t)std::max(4.0f, std::floor((float)N / 1e6f));
+        const uint64_t nb = (N + gpb - 1) / gpb;
+        b += nb * sizeof(CullNode) + ((nb + 31) / 32 + 2) * 4;
+    }
+    return b;
+}
+
+uint64_t gps_pass_device_bytes(const gps_pass* p) {
+    if (!p) return 0;
+    const uint64_t N = p->N, npad = (uint64_t)p->ntiles * GPS_TILE, nwords = (N + 31) / 32 + 2;
+    uint64_t b = npad * 4 + 2 * nwords * 4 + std::max<uint64_t>(N, 1) * (sizeof(SplatRecord) + 8) +
+                 4ull * 4 * p->ntiles + ((uint64_t)p->budget / GPS_SPLAT_CHUNK + 2) * 4 + sizeof(FrameCounters) +
+                 (uint64_t)p->hiz_size * 4 + 2 * p->fb_cap * 8 + p->dbg_indices_cap * 4;
+    if (p->hier_available) b += (uint64_t)p->cull.node_count * sizeof(CullNode) + (p->cull.mask_words + 2) * 4ull;
+    return b;
 }
 
 void gps_pass_destroy(gps_pass* p) {
@@ -462,26 +494,16 @@ void gps_pass_destroy(gps_pass* p) {
     delete p;
 }
 
-gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_color, float* d_depth, void* stream) {
-    if (!p || !fp || !d_color || !d_depth) return GPS_ERR_INVALID_ARG;
-    if (fp->width != p->settings.width || fp->height != p->settings.height) {
-        g_last_error = "runtime resolution change is not supported (main.cu:82-86)";
-        return GPS_ERR_INVALID_ARG;
-    }
-    if (fp->supersampling_factor < 1 || !(fp->far_plane > fp->near_plane)) return GPS_ERR_INVALID_ARG;
+extern "C++" {
+namespace gps {
+void make_frame_consts(const gps_frame_params* fp, int spp, FrameConsts* out) {
+    FrameConsts& f = *out;
     const int ss = fp->supersampling_factor;
-    if ((int64_t)fp->width * ss >= 65536 || (int64_t)fp->height * ss >= 65536) return GPS_ERR_UNSUPPORTED;
-    hipStream_t st = (hipStream_t)stream;
-    gps_status s = ensure_fb(p, ss);
-    if (s != GPS_OK) return s;
-
-    // Frame constants (gaussian_point_splatting.cu:605-615).
     hm::M4 V, P;
     std::memcpy(V.m, fp->view, 64);
     std::memcpy(P.m, fp->proj, 64);
     const hm::M4 cam = hm::affine_inverse(V);
     const hm::M4 VP = hm::mul(P, V);
-    FrameConsts f;
     std::memcpy(f.V, V.m, 64);
     std::memcpy(f.VP, VP.m, 64);
     f.campos[0] = cam.at(3, 0);
@@ -501,16 +523,56 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
     f.up_w = fp->width * ss;
     f.up_h = fp->height * ss;
     f.ss = ss;
-    f.spp = p->settings.render_pass_count;
+    f.spp = spp;
     f.Ki = fp->reduce_point_count ? ss * ss : 1;
     f.K = fp->reduce_point_count ? (float)(ss * ss) : 1.0f;
     f.point_cap = det_f2u_rz((float)(f.up_w * f.up_h) / (2.0f * f.K));
     f.occl = fp->enable_occlusion_culling ? 1 : 0;
-    const bool hier = fp->enable_hierarchical_culling && p->hier_available;
-    f.hier = hier ? 1 : 0;
+    f.hier = fp->enable_hierarchical_culling ? 1 : 0;
     f.cull_small = fp->cull_small_gaussians ? 1 : 0;
     f.unbiased = fp->use_unbiased_2d_splatting ? 1 : 0;
     f.reduce = fp->reduce_point_count ? 1 : 0;
+    f.frame_seed = 0;
+    f.only_prev = 0;
+}
+}  // namespace gps
+}
+
+gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_color, float* d_depth, void* stream) {
+    if (!p || !fp || !d_color || !d_depth) return GPS_ERR_INVALID_ARG;
+    const int spp = fp->samples_per_pixel > 0 ? fp->samples_per_pixel : p->settings.render_pass_count;
+    if (!p->settings.flexible_frame_size) {
+        if (fp->width != p->settings.width || fp->height != p->settings.height ||
+            spp != p->settings.render_pass_count) {
+            g_last_error = "runtime resolution / spp change is not supported (main.cu:82-86)";
+            return GPS_ERR_INVALID_ARG;
+        }
+    } else {
+        if (fp->width < 1 || fp->height < 1 || fp->width > p->settings.width || fp->height > p->settings.height ||
+            spp > p->settings.render_pass_count) {
+            g_last_error = "frame larger than the pass's width x height x spp";
+            return GPS_ERR_INVALID_ARG;
+        }
+        if (fp->enable_occlusion_culling && (fp->width != p->settings.width || fp->height != p->settings.height)) {
+            g_last_error = "occlusion culling needs frames of the pass's size (Hi-Z chain)";
+            return GPS_ERR_UNSUPPORTED;
+        }
+    }
+    if (fp->supersampling_factor < 1 || !(fp->far_plane > fp->near_plane)) return GPS_ERR_INVALID_ARG;
+    const int ss = fp->supersampling_factor;
+    if ((int64_t)fp->width * ss >= 65536 || (int64_t)fp->height * ss >= 65536) return GPS_ERR_UNSUPPORTED;
+    hipStream_t st = (hipStream_t)stream;
+    gps_status s = ensure_fb(p, ss, fb_words(fp->width * ss, fp->height * ss, spp));
+    if (s != GPS_OK) return s;
+
+    FrameConsts f;
+    make_frame_consts(fp, spp, &f);
+    const bool hier = fp->enable_hierarchical_culling && p->hier_available;
+    f.hier = hier ? 1 : 0;
+    hm::M4 V, P;
+    std::memcpy(V.m, fp->view, 64);
+    std::memcpy(P.m, fp->proj, 64);
+    const hm::M4 VP = hm::mul(P, V);
 
     const uint64_t bg = background_word(fp->background[0], fp->background[1], fp->background[2]);
     p->cur_fb = (fp->frame % 2 == 0) ? p->fbA : p->fbB;

@@ -97,6 +97,24 @@ struct FrameCounters {
     uint32_t pad[3];
 };
 
+// One view of a multi-view batch (device table entry).  The host fills the
+// frame part before every batch; the device writes the per-view results.
+struct BatchView {
+    FrameConsts f;          // f.frame_seed = 4 * frame (preprocess); the splat uses + 1
+    float planes[24];       // frustum planes for block culling
+    uint64_t fb_offset;     // first framebuffer word of this view
+    uint64_t fb_words;      // words of this view's framebuffer
+    uint64_t background;    // background word
+    uint32_t budget;        // point budget of the view
+    // written on the device
+    uint32_t base;          // first point of the view in the batch's point list
+    uint32_t points;        // points splatted (min(total, budget))
+    uint32_t total;         // uncapped sum of point counts
+    uint32_t valid;         // valid Gaussians
+    uint32_t culled_blocks;
+};
+#define GPS_BATCH_MAX_VIEWS 256
+
 // Splat work decomposition: one workgroup processes CHUNK consecutive points,
 // PPT consecutive points per thread.
 #define GPS_SPLAT_THREADS 256
@@ -107,4 +125,17 @@ struct FrameCounters {
 #define GPS_TILE (1u << GPS_TILE_SHIFT)
 #define GPS_PRE_THREADS 256
 
+// Frame constants of gps_pass_render for a frame with `spp` samples per pixel
+// (gaussian_point_splatting.cu:605-615).
+void make_frame_consts(const gps_frame_params* fp, int spp, FrameConsts* f);
+
 }  // namespace gps
+
+// Resident scene behind the gps_scene handle.
+struct gps_scene {
+    gps::DevScene view{};
+    void* planes = nullptr;  // FP32 planes or HALF_FP8 means
+    void* geo = nullptr;
+    void* shrec = nullptr;
+    uint64_t bytes = 0;
+};

@@ -51,6 +51,39 @@ struct SplatArgs {
     int32_t xcds;    // accelerator complex dies (workgroups round-robin over them)
 };
 
+// Multi-view batch (see the batch kernels): all arrays are concatenated per view.
+struct BatchArgs {
+    DevScene scene;
+    BatchView* views;        // device table, nviews entries
+    int32_t nviews;
+    uint32_t npad;           // per-view stride of weights / records (multiple of the tile)
+    uint32_t tiles_per_view;
+    uint32_t* weights;
+    SplatRecord* recs;
+    uint32_t *tile_sum, *tile_cnt, *tile_off, *tile_voff;
+    uint32_t *cidx, *coff;
+    uint8_t* cview;
+    uint32_t* chunk_start;
+    PassCounters* ctr;
+    uint32_t* iteration;     // scratch word (the scan flips it)
+    CullView cull;           // mask = first view's mask
+    uint32_t mask_stride;
+    int32_t hier;
+    uint64_t* fb;            // pool; views address it by fb_offset
+    uint64_t max_fb_words;   // largest view
+    uint32_t grid;
+    int32_t xcds;
+    int32_t threads, ppt;    // splat variant
+    uint32_t* atomics;
+};
+extern const int kBatchSplatVariants[][2];
+extern const int kNumBatchSplatVariants;
+void launch_batch_clear(const BatchArgs& a, hipStream_t st);
+void launch_batch_cull(const BatchArgs& a, hipStream_t st);
+void launch_batch_preprocess(const BatchArgs& a, hipStream_t st);
+void launch_batch_distribute(const BatchArgs& a, hipStream_t st);
+bool launch_batch_splat(const BatchArgs& a, hipStream_t st);
+
 void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st);
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
                  uint32_t* culled_count, hipStream_t st);

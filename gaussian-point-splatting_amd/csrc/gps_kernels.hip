@@ -249,6 +249,57 @@ __global__ __launch_bounds__(256) void cull_kernel(CullView cv, Planes pl, Mat4 
 }
 
 // ------------------------------------------------------------------ preprocess
+// One Gaussian under one frame (gaussian_point_splatting.cu:147-307, steps 2-11
+// of its cull order): returns the point count, 0 if the Gaussian is not
+// splatted, and writes its 64-byte splat record when the count is > 0.
+// `mask_prev` is read only when f.only_prev is set; `h` only when f.occl.
+template <int FORMAT, int DEG>
+__device__ __forceinline__ uint32_t preprocess_one(const DevScene& s, const FrameConsts& f, uint32_t g,
+                                                   const uint32_t* __restrict__ mask_prev, const HiZView& h,
+                                                   SplatRecord* __restrict__ rec_out) {
+    u32x2 seed = make_seed(g, f.frame_seed);
+    if (f.only_prev && mask_bit(mask_prev, g)) return 0u;
+    const Geo ge = load_geo<FORMAT>(s, g);
+    const float op = glm_min(ge.op, 1.0f);
+    if (op <= 0.0f) return 0u;
+    Projected pr;
+    if (!project_gaussian(f, ge.mx, ge.my, ge.mz, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx, ge.sy, ge.sz, pr)) return 0u;
+    if (pr.minx >= f.up_w || pr.miny >= f.up_h || pr.maxx < 0 || pr.maxy < 0) return 0u;
+    if (f.cull_small) {
+        const float tr = pr.ca + pr.cc;
+        const float t2 = 0.5f * tr;
+        const float term2 = det_sqrt(det_fmax(0.1f, t2 * t2 - pr.det));
+        const float e1 = t2 + term2, e2 = t2 - term2;
+        if (e1 < 0.02f || e2 < 0.02f) return 0u;
+    }
+    const float s_det = det_sqrt(pr.det);
+    const float dl = dilog(op);
+    const float importance = f.unbiased ? GPS_TWO_PI_F * s_det * dl : op * GPS_TWO_PI_F * s_det;
+    if (f.occl && hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss)) return 0u;
+    uint32_t cnt = f.unbiased ? poisson(seed, importance) : (uint32_t)stochastic_round(importance, pcg2d(seed).x);
+    if (f.K > 1.0f) cnt = (uint32_t)stochastic_round((float)cnt / f.K, pcg2d(seed).x);
+    if (cnt == 0u) return 0u;
+    cnt = min(cnt, f.point_cap);
+    float dx, dy, dz;
+    view_dir(f, ge.mx, ge.my, ge.mz, dx, dy, dz);
+    float rgb[3];
+    ShEval<FORMAT, DEG>::eval(s, g, dx, dy, dz, rgb);
+    const float c0 = det_sqrt(pr.ca);
+    const float c1 = pr.cb / c0;
+    const float c2 = det_sqrt(pr.cc - c1 * c1);
+    const float inv_det = 1.0f / pr.det;
+    const uint64_t packed = (depth_quantize(pr.view_z, f.near_plane, f.far_plane) << GPS_DEPTH_SHIFT) |
+                            pack_color36(rgb[0], rgb[1], rgb[2]);
+    uint4* r = reinterpret_cast<uint4*>(rec_out);
+    r[0] = make_uint4(__float_as_uint(pr.pmx), __float_as_uint(pr.pmy), __float_as_uint(c0), __float_as_uint(c1));
+    r[1] = make_uint4(__float_as_uint(c2), __float_as_uint(pr.cc * inv_det), __float_as_uint(-pr.cb * inv_det),
+                      __float_as_uint(pr.ca * inv_det));
+    r[2] = make_uint4(__float_as_uint(op), __float_as_uint(corrected_inv_alpha(op)), __float_as_uint(dl),
+                      (uint32_t)pr.minx | ((uint32_t)pr.miny << 16));
+    r[3] = make_uint4((uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16), (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
+    return cnt;
+}
+
 template <int FORMAT, int DEG>
 __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
@@ -263,51 +314,7 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     // Gaussians of culled blocks leave their weight slot untouched: the
     // distributor reads the same cull mask and counts them as zero.
     if (g < s.N && !(f.hier && block_culled(cv, g))) {
-        u32x2 seed = make_seed(g, f.frame_seed);
-        do {
-            if (f.only_prev && mask_bit(mask_prev, g)) break;
-            const Geo ge = load_geo<FORMAT>(s, g);
-            const float op = glm_min(ge.op, 1.0f);
-            if (op <= 0.0f) break;
-            Projected pr;
-            if (!project_gaussian(f, ge.mx, ge.my, ge.mz, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx, ge.sy, ge.sz, pr)) break;
-            if (pr.minx >= f.up_w || pr.miny >= f.up_h || pr.maxx < 0 || pr.maxy < 0) break;
-            if (f.cull_small) {
-                const float tr = pr.ca + pr.cc;
-                const float t2 = 0.5f * tr;
-                const float term2 = det_sqrt(det_fmax(0.1f, t2 * t2 - pr.det));
-                const float e1 = t2 + term2, e2 = t2 - term2;
-                if (e1 < 0.02f || e2 < 0.02f) break;
-            }
-            const float s_det = det_sqrt(pr.det);
-            const float dl = dilog(op);
-            const float importance = f.unbiased ? GPS_TWO_PI_F * s_det * dl : op * GPS_TWO_PI_F * s_det;
-            if (f.occl && hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss)) break;
-            uint32_t cnt = f.unbiased ? poisson(seed, importance)
-                                      : (uint32_t)stochastic_round(importance, pcg2d(seed).x);
-            if (f.K > 1.0f) cnt = (uint32_t)stochastic_round((float)cnt / f.K, pcg2d(seed).x);
-            if (cnt == 0u) break;
-            cnt = min(cnt, f.point_cap);
-            float dx, dy, dz;
-            view_dir(f, ge.mx, ge.my, ge.mz, dx, dy, dz);
-            float rgb[3];
-            ShEval<FORMAT, DEG>::eval(s, g, dx, dy, dz, rgb);
-            const float c0 = det_sqrt(pr.ca);
-            const float c1 = pr.cb / c0;
-            const float c2 = det_sqrt(pr.cc - c1 * c1);
-            const float inv_det = 1.0f / pr.det;
-            const uint64_t packed = (depth_quantize(pr.view_z, f.near_plane, f.far_plane) << GPS_DEPTH_SHIFT) |
-                                    pack_color36(rgb[0], rgb[1], rgb[2]);
-            uint4* r = reinterpret_cast<uint4*>(recs + g);
-            r[0] = make_uint4(__float_as_uint(pr.pmx), __float_as_uint(pr.pmy), __float_as_uint(c0), __float_as_uint(c1));
-            r[1] = make_uint4(__float_as_uint(c2), __float_as_uint(pr.cc * inv_det), __float_as_uint(-pr.cb * inv_det),
-                              __float_as_uint(pr.ca * inv_det));
-            r[2] = make_uint4(__float_as_uint(op), __float_as_uint(corrected_inv_alpha(op)), __float_as_uint(dl),
-                              (uint32_t)pr.minx | ((uint32_t)pr.miny << 16));
-            r[3] = make_uint4((uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16), (uint32_t)packed,
-                              (uint32_t)(packed >> 32), 0u);
-            n = cnt;
-        } while (false);
+        n = preprocess_one<FORMAT, DEG>(s, f, g, mask_prev, h, recs + g);
         weights[g] = n;
     }
     // Two 32-bit valid-mask words per wave.
@@ -404,30 +411,41 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
 // Tiles without valid Gaussians are skipped whole; with block culling on,
 // weights of culled blocks are taken as zero without being read (preprocess
 // does not write them).
+//
+// Several views can be concatenated (multi-view batches): view v owns tiles
+// [v * tiles_per_view, (v + 1) * tiles_per_view), its Gaussian g sits at
+// v * npad + g, its cull mask at cv.mask + v * mask_stride; `cview`, when not
+// null, receives the view of every compacted entry.  Chunks are
+// 2^chunk_shift points.
 __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restrict__ weights,
                                                          const uint32_t* __restrict__ tile_off,
                                                          const uint32_t* __restrict__ tile_voff, uint32_t ntiles,
                                                          const PassCounters* __restrict__ ctr, uint32_t* __restrict__ cidx,
                                                          uint32_t* __restrict__ coff, uint32_t* __restrict__ chunk_start,
-                                                         CullView cv, int hier, uint32_t N) {
+                                                         CullView cv, int hier, uint32_t N, uint32_t tiles_per_view,
+                                                         uint32_t npad, uint32_t mask_stride,
+                                                         uint8_t* __restrict__ cview, uint32_t chunk_shift) {
     __shared__ uint32_t s_a[4], s_c[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t tile = blockIdx.x;
     const uint32_t vend = tile + 1 < ntiles ? tile_voff[tile + 1] : ctr->valid;
     if (vend == tile_voff[tile]) return;  // uniform over the workgroup
+    const uint32_t view = tile / tiles_per_view;
+    const uint32_t* cmask = cv.mask + (uint64_t)view * mask_stride;
     const uint32_t g0 = (tile << GPS_TILE_SHIFT) + (uint32_t)tid * 16u;
+    const uint32_t l0 = g0 - view * npad;  // index within the view's scene
     // bit k set: element g0 + k counts (inside the scene and not culled)
-    uint32_t live = g0 >= N ? 0u : (N - g0 >= 16u ? 0xFFFFu : ((1u << (N - g0)) - 1u));
+    uint32_t live = l0 >= N ? 0u : (N - l0 >= 16u ? 0xFFFFu : ((1u << (N - l0)) - 1u));
     if (hier && live) {
-        uint32_t b = g0 / cv.gpb;
+        uint32_t b = l0 / cv.gpb;
         uint32_t next = (b + 1u) * cv.gpb;
-        bool c = mask_bit(cv.mask, b);
-        const int kmax = N - g0 >= 16u ? 16 : (int)(N - g0);
+        bool c = mask_bit(cmask, b);
+        const int kmax = N - l0 >= 16u ? 16 : (int)(N - l0);
         for (int k = 0; k < kmax; ++k) {
-            if (g0 + (uint32_t)k == next) {
+            if (l0 + (uint32_t)k == next) {
                 ++b;
                 next += cv.gpb;
-                c = mask_bit(cv.mask, b);
+                c = mask_bit(cmask, b);
             }
             if (c) live &= ~(1u << k);
         }
@@ -468,9 +486,11 @@ __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restr
         if (w[k]) {
             cidx[r] = g0 + (uint32_t)k;
             coff[r] = off;
+            if (cview) cview[r] = (uint8_t)view;
             const uint64_t lo = off;
             const uint64_t hi = (lo + w[k]) < P ? (lo + w[k]) : P;
-            for (uint64_t c = (lo + GPS_SPLAT_CHUNK - 1) / GPS_SPLAT_CHUNK; c * GPS_SPLAT_CHUNK < hi; ++c)
+            const uint64_t csz = 1ull << chunk_shift;
+            for (uint64_t c = (lo + csz - 1) >> chunk_shift; (c << chunk_shift) < hi; ++c)
                 chunk_start[c] = r;
             ++r;
             off += w[k];
@@ -479,6 +499,67 @@ __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restr
 }
 
 // ------------------------------------------------------------------ splat
+__device__ __forceinline__ SplatRecord load_record(const SplatRecord* __restrict__ recs, uint32_t g) {
+    const uint4* rp = reinterpret_cast<const uint4*>(recs + g);
+    const uint4 a = rp[0], b = rp[1], cc = rp[2], d = rp[3];
+    SplatRecord rec;
+    rec.pmx = __uint_as_float(a.x);
+    rec.pmy = __uint_as_float(a.y);
+    rec.c0 = __uint_as_float(a.z);
+    rec.c1 = __uint_as_float(a.w);
+    rec.c2 = __uint_as_float(b.x);
+    rec.conx = __uint_as_float(b.y);
+    rec.cony = __uint_as_float(b.z);
+    rec.conz = __uint_as_float(b.w);
+    rec.opacity = __uint_as_float(cc.x);
+    rec.inv_alpha = __uint_as_float(cc.y);
+    rec.dilog_alpha = __uint_as_float(cc.z);
+    rec.bmin = cc.w;
+    rec.bmax = d.x;
+    rec.packed_lo = d.y;
+    rec.packed_hi = d.z;
+    return rec;
+}
+
+// The K * spp samples of point p (seed stream p of RNG frame `frame_seed`)
+// of one Gaussian (splatGaussianPoint, gaussian_point_splatting.cu:67-141):
+// sample q = layer * K + k lands in layer q / K.  Returns the atomics issued.
+__device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t p, uint32_t frame_seed, int K,
+                                                int spp, int up_w, int unbiased, unsigned long long* __restrict__ fb) {
+    const int bminx = (int)(rec.bmin & 0xFFFFu), bminy = (int)(rec.bmin >> 16);
+    const int bmaxx = (int)(rec.bmax & 0xFFFFu), bmaxy = (int)(rec.bmax >> 16);
+    const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
+    const unsigned long long depth_bits = packed & ~(unsigned long long)GPS_COLOR_MASK;
+    const int samples = spp * K;
+    uint32_t issued = 0;
+    u32x2 seed = make_seed(p, frame_seed);
+    for (int layer = 0, k = 0, q = 0; q < samples; ++q) {
+        if (k == K) {
+            k = 0;
+            ++layer;
+        }
+        ++k;
+        const f32x2 u = pcg2d(seed);
+        const f32x2 xy = unbiased ? corrected_box_muller_pre(u.x, u.y, rec.inv_alpha, rec.dilog_alpha)
+                                  : box_muller(u.x, u.y);
+        const float fx = det_floor(rec.pmx + rec.c0 * xy.x);
+        const float fy = det_floor((rec.pmy + rec.c1 * xy.x) + rec.c2 * xy.y);
+        const int x = det_f2i_rz(fx);
+        const int y = det_f2i_rz(fy);
+        if (x < bminx || x >= bmaxx || y < bminy || y >= bmaxy) continue;
+        const float dx = rec.pmx - fx, dy = rec.pmy - fy;
+        const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
+        const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
+        const float alpha = rec.opacity * gw;
+        if (alpha < 1.0f / 255.0f) continue;
+        const int idx = fb_index(x, y, layer, spp, up_w);
+        if (fb[idx] <= depth_bits) continue;
+        atomicMin(fb + idx, packed);
+        ++issued;
+    }
+    return issued;
+}
+
 struct SplatConsts {
     int32_t spp, K, up_w, unbiased;
     uint32_t frame_seed;
@@ -497,9 +578,6 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
     const uint32_t V = ctr->valid;
     const uint32_t nchunks = ctr->chunks;
     const int tid = threadIdx.x;
-    const int samples = sc.spp * sc.K;
-    const int tiles_per_row = fb_tiles_per_row(sc.up_w);
-    (void)tiles_per_row;
     uint32_t issued = 0;
     // Workgroups are dispatched round-robin over the XCDs (blockIdx % xcds);
     // give every XCD one contiguous range of chunks so the framebuffer lines
@@ -536,54 +614,9 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
                 const uint32_t g = cidx[r_lo + (uint32_t)j];
                 if (g != cur_g) {
                     cur_g = g;
-                    const uint4* rp = reinterpret_cast<const uint4*>(recs + g);
-                    const uint4 a = rp[0], b = rp[1], cc = rp[2], d = rp[3];
-                    rec.pmx = __uint_as_float(a.x);
-                    rec.pmy = __uint_as_float(a.y);
-                    rec.c0 = __uint_as_float(a.z);
-                    rec.c1 = __uint_as_float(a.w);
-                    rec.c2 = __uint_as_float(b.x);
-                    rec.conx = __uint_as_float(b.y);
-                    rec.cony = __uint_as_float(b.z);
-                    rec.conz = __uint_as_float(b.w);
-                    rec.opacity = __uint_as_float(cc.x);
-                    rec.inv_alpha = __uint_as_float(cc.y);
-                    rec.dilog_alpha = __uint_as_float(cc.z);
-                    rec.bmin = cc.w;
-                    rec.bmax = d.x;
-                    rec.packed_lo = d.y;
-                    rec.packed_hi = d.z;
+                    rec = load_record(recs, g);
                 }
-                const int bminx = (int)(rec.bmin & 0xFFFFu), bminy = (int)(rec.bmin >> 16);
-                const int bmaxx = (int)(rec.bmax & 0xFFFFu), bmaxy = (int)(rec.bmax >> 16);
-                const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
-                const unsigned long long depth_bits = packed & ~(unsigned long long)GPS_COLOR_MASK;
-                u32x2 seed = make_seed(p, sc.frame_seed);
-                // sample q = layer * K + k lands in layer q / K
-                for (int layer = 0, k = 0, q = 0; q < samples; ++q) {
-                    if (k == sc.K) {
-                        k = 0;
-                        ++layer;
-                    }
-                    ++k;
-                    const f32x2 u = pcg2d(seed);
-                    const f32x2 xy = sc.unbiased ? corrected_box_muller_pre(u.x, u.y, rec.inv_alpha, rec.dilog_alpha)
-                                                 : box_muller(u.x, u.y);
-                    const float fx = det_floor(rec.pmx + rec.c0 * xy.x);
-                    const float fy = det_floor((rec.pmy + rec.c1 * xy.x) + rec.c2 * xy.y);
-                    const int x = det_f2i_rz(fx);
-                    const int y = det_f2i_rz(fy);
-                    if (x < bminx || x >= bmaxx || y < bminy || y >= bmaxy) continue;
-                    const float dx = rec.pmx - fx, dy = rec.pmy - fy;
-                    const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
-                    const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
-                    const float alpha = rec.opacity * gw;
-                    if (alpha < 1.0f / 255.0f) continue;
-                    const int idx = fb_index(x, y, layer, sc.spp, sc.up_w);
-                    if (fb[idx] <= depth_bits) continue;
-                    atomicMin(fb + idx, packed);
-                    ++issued;
-                }
+                issued += splat_point(rec, p, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb);
             }
         }
         __syncthreads();
@@ -923,6 +956,207 @@ __global__ __launch_bounds__(256) void materialize_indices_kernel(const uint32_t
     out[p] = cidx[lo];
 }
 
+// ------------------------------------------------------------------ multi-view batches
+// A batch renders up to GPS_BATCH_MAX_VIEWS independent frames of one scene:
+// view v is exactly the single-view frame (same seeds, same records, same
+// framebuffer words), with its weights / records / tiles at offset v of
+// concatenated arrays, so the distributor and a persistent splat cover all
+// views in one launch each.
+
+__global__ __launch_bounds__(256) void batch_clear_kernel(const BatchView* __restrict__ views, uint4* __restrict__ fb) {
+    const BatchView& bv = views[blockIdx.y];
+    const uint64_t n2 = bv.fb_words / 2;
+    const uint64_t value = bv.background;
+    const uint4 v = make_uint4((uint32_t)value, (uint32_t)(value >> 32), (uint32_t)value, (uint32_t)(value >> 32));
+    uint4* dst = fb + bv.fb_offset / 2;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+// Frustum part of the block cull (gaussian_bvh.cu:215-234) per view.
+__global__ __launch_bounds__(256) void batch_cull_kernel(BatchView* __restrict__ views, CullView cv,
+                                                         uint32_t mask_stride) {
+    const uint32_t v = blockIdx.y;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* pl = views[v].planes;
+    bool culled = false;
+    if (idx < cv.node_count && views[v].f.hier) {  // views without block culling get an empty mask
+        const CullNode n = cv.nodes[idx];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const float nx = pl[4 * i], ny = pl[4 * i + 1], nz = pl[4 * i + 2], d = pl[4 * i + 3];
+            const float vx = nx >= 0.0f ? n.mx[0] : n.mn[0];
+            const float vy = ny >= 0.0f ? n.mx[1] : n.mn[1];
+            const float vz = nz >= 0.0f ? n.mx[2] : n.mn[2];
+            if (((nx * vx + ny * vy) + nz * vz) + d < 0.0f) culled = true;
+        }
+    }
+    const uint64_t b = __ballot(culled);
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        uint32_t* mask = cv.mask + (uint64_t)v * mask_stride;
+        const uint32_t w0 = (idx - lane) >> 5;
+        if (w0 < cv.mask_words) mask[w0] = (uint32_t)b;
+        if (w0 + 1 < cv.mask_words) mask[w0 + 1] = (uint32_t)(b >> 32);
+        const uint32_t c = (uint32_t)__popcll(b);
+        if (c) atomicAdd(&views[v].culled_blocks, c);
+    }
+}
+
+// One workgroup per tile of 4096 Gaussians; each thread evaluates its 16
+// Gaussians under every view (view loop inner, so a Gaussian's attributes
+// are fetched from HBM once per batch and from the caches afterwards).  The
+// per-(view, tile) sums are complete when the workgroup ends, so they are
+// stored, not accumulated.
+template <int FORMAT, int DEG>
+__global__ __launch_bounds__(256) void batch_preprocess_kernel(DevScene s, const BatchView* __restrict__ views,
+                                                               int nviews, uint32_t npad, uint32_t tiles_per_view,
+                                                               uint32_t* __restrict__ weights,
+                                                               SplatRecord* __restrict__ recs,
+                                                               uint32_t* __restrict__ tile_sum,
+                                                               uint32_t* __restrict__ tile_cnt, CullView cv,
+                                                               uint32_t mask_stride) {
+    __shared__ uint32_t s_sum[GPS_BATCH_MAX_VIEWS], s_cnt[GPS_BATCH_MAX_VIEWS];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t tile = blockIdx.x;
+    for (int v = tid; v < nviews; v += 256) {
+        s_sum[v] = 0u;
+        s_cnt[v] = 0u;
+    }
+    __syncthreads();
+    const HiZView no_hiz{};
+    for (int i = 0; i < GPS_TILE / 256; ++i) {
+        const uint32_t g = (tile << GPS_TILE_SHIFT) + (uint32_t)(i * 256 + tid);
+        for (int v = 0; v < nviews; ++v) {
+            const FrameConsts& f = views[v].f;
+            uint32_t n = 0;
+            if (g < s.N) {
+                const uint32_t blk = g / cv.gpb;
+                const bool culled = f.hier && mask_bit(cv.mask + (uint64_t)v * mask_stride, blk);
+                if (!culled) {
+                    n = preprocess_one<FORMAT, DEG>(s, f, g, nullptr, no_hiz, recs + (uint64_t)v * npad + g);
+                    weights[(uint64_t)v * npad + g] = n;
+                }
+            }
+            const uint64_t b = __ballot(n != 0u);
+            if (b) {
+                const uint32_t sum = wave_reduce_add(n);
+                if (lane == 0) {
+                    atomicAdd(&s_sum[v], sum);
+                    atomicAdd(&s_cnt[v], (uint32_t)__popcll(b));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int v = tid; v < nviews; v += 256) {
+        tile_sum[(uint64_t)v * tiles_per_view + tile] = s_sum[v];
+        tile_cnt[(uint64_t)v * tiles_per_view + tile] = s_cnt[v];
+    }
+}
+
+// Per-view point ranges of the batch from the concatenated tile scan.
+__global__ __launch_bounds__(256) void batch_views_kernel(BatchView* __restrict__ views, int nviews,
+                                                          const uint32_t* __restrict__ tile_off,
+                                                          const uint32_t* __restrict__ tile_voff,
+                                                          uint32_t tiles_per_view, const PassCounters* __restrict__ ctr) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nviews) return;
+    const uint64_t t0 = (uint64_t)v * tiles_per_view, t1 = t0 + tiles_per_view;
+    const uint32_t base = tile_off[t0];
+    const uint32_t end = v + 1 < nviews ? tile_off[t1] : ctr->total;
+    const uint32_t vbase = tile_voff[t0];
+    const uint32_t vend = v + 1 < nviews ? tile_voff[t1] : ctr->valid;
+    views[v].base = base;
+    views[v].total = end - base;
+    views[v].points = min(end - base, views[v].budget);
+    views[v].valid = vend - vbase;
+}
+
+// Persistent splat over the points of all views (same schedule as
+// splat_kernel, THREADS x PPT points per chunk).  Point p of view v is point
+// p - base(v) of that view's own frame.
+template <int THREADS, int PPT>
+__global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord* __restrict__ recs,
+                                                              const uint32_t* __restrict__ cidx,
+                                                              const uint32_t* __restrict__ coff,
+                                                              const uint8_t* __restrict__ cview,
+                                                              const uint32_t* __restrict__ chunk_start,
+                                                              const PassCounters* __restrict__ ctr,
+                                                              const BatchView* __restrict__ views,
+                                                              unsigned long long* __restrict__ fb, int32_t xcds,
+                                                              uint32_t* __restrict__ atomics_count) {
+    constexpr int CHUNK = THREADS * PPT;
+    __shared__ uint32_t s_off[CHUNK + 1];
+    const uint32_t P = ctr->points;
+    const uint32_t V = ctr->valid;
+    const uint32_t nchunks = (uint32_t)(((uint64_t)P + CHUNK - 1) / CHUNK);
+    const int tid = threadIdx.x;
+    uint32_t issued = 0;
+    const uint32_t nx = (uint32_t)xcds;
+    const uint32_t xcd = blockIdx.x % nx;
+    const uint32_t per_xcd = gridDim.x / nx;
+    const uint32_t seg_lo = (uint32_t)(((uint64_t)nchunks * xcd) / nx);
+    const uint32_t seg_hi = (uint32_t)(((uint64_t)nchunks * (xcd + 1)) / nx);
+    for (uint32_t c = seg_lo + blockIdx.x / nx; c < seg_hi; c += per_xcd) {
+        const uint32_t r_lo = chunk_start[c];
+        for (int j = tid; j <= CHUNK; j += THREADS) {
+            const uint32_t r = r_lo + (uint32_t)j;
+            s_off[j] = r < V ? coff[r] : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        const uint32_t p0 = c * CHUNK + (uint32_t)tid * PPT;
+        if (p0 < P) {
+            int lo = 0, hi = CHUNK;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= p0) lo = mid;
+                else hi = mid - 1;
+            }
+            int j = lo;
+            uint32_t cur_g = 0xFFFFFFFFu;
+            int cur_v = -1;
+            SplatRecord rec;
+            uint32_t vbase = 0, vpoints = 0, vseed = 0;
+            int vK = 1, vspp = 1, vup_w = 1, vunb = 1;
+            uint64_t voff = 0;
+            for (int k = 0; k < PPT; ++k) {
+                const uint32_t p = p0 + (uint32_t)k;
+                if (p >= P) break;
+                while (j < CHUNK - 1 && s_off[j + 1] <= p) ++j;
+                if (r_lo + (uint32_t)j >= V) break;
+                const uint32_t r = r_lo + (uint32_t)j;
+                const uint32_t g = cidx[r];
+                if (g != cur_g) {
+                    cur_g = g;
+                    rec = load_record(recs, g);
+                    const int v = cview[r];
+                    if (v != cur_v) {
+                        cur_v = v;
+                        const BatchView& bv = views[v];
+                        vbase = bv.base;
+                        vpoints = bv.points;
+                        vseed = bv.f.frame_seed + 1u;
+                        vK = bv.f.Ki;
+                        vspp = bv.f.spp;
+                        vup_w = bv.f.up_w;
+                        vunb = bv.f.unbiased;
+                        voff = bv.fb_offset;
+                    }
+                }
+                const uint32_t pl = p - vbase;
+                if (pl >= vpoints) continue;  // beyond the view's point budget
+                issued += splat_point(rec, pl, vseed, vK, vspp, vup_w, vunb, fb + voff);
+            }
+        }
+        __syncthreads();
+    }
+    if (atomics_count) {
+        const uint32_t tot = wave_reduce_add(issued);
+        if ((tid & 63) == 0 && tot) atomicAdd(atomics_count, tot);
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 template <int FORMAT, int DEG>
 static void launch_pre(const PreprocessArgs& a, hipStream_t st) {
@@ -969,7 +1203,8 @@ void launch_distribute(const DistributeArgs& a, hipStream_t st) {
                        a.tile_voff, a.ctr, a.budget, a.iteration);
     if (a.ntiles == 0) return;
     hipLaunchKernelGGL(distribute_kernel, dim3(a.ntiles), dim3(256), 0, st, a.weights, a.tile_off, a.tile_voff,
-                       a.ntiles, a.ctr, a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.N);
+                       a.ntiles, a.ctr, a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.N, a.ntiles, 0u, 0u,
+                       (uint8_t*)nullptr, 10u);
 }
 
 void launch_splat(const SplatArgs& a, hipStream_t st) {
@@ -1050,6 +1285,81 @@ void launch_materialize_indices(const uint32_t* cidx, const uint32_t* coff, cons
                                 uint32_t max_points, hipStream_t st) {
     if (max_points == 0) return;
     hipLaunchKernelGGL(materialize_indices_kernel, dim3((max_points + 255) / 256), dim3(256), 0, st, cidx, coff, ctr, out);
+}
+
+// ------------------------------------------------------------------ batch launchers
+void launch_batch_clear(const BatchArgs& a, hipStream_t st) {
+    uint32_t bx = (uint32_t)std::min<uint64_t>((a.max_fb_words / 2 + 255) / 256, 2048);
+    if (bx == 0) bx = 1;
+    hipLaunchKernelGGL(batch_clear_kernel, dim3(bx, a.nviews), dim3(256), 0, st, a.views,
+                       reinterpret_cast<uint4*>(a.fb));
+}
+
+void launch_batch_cull(const BatchArgs& a, hipStream_t st) {
+    if (!a.cull.node_count) return;
+    hipLaunchKernelGGL(batch_cull_kernel, dim3((a.cull.node_count + 255) / 256, a.nviews), dim3(256), 0, st, a.views,
+                       a.cull, a.mask_stride);
+}
+
+template <int FORMAT, int DEG> static void launch_batch_pre_t(const BatchArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((batch_preprocess_kernel<FORMAT, DEG>), dim3(a.tiles_per_view), dim3(256), 0, st, a.scene,
+                       a.views, a.nviews, a.npad, a.tiles_per_view, a.weights, a.recs, a.tile_sum, a.tile_cnt, a.cull,
+                       a.mask_stride);
+}
+
+void launch_batch_preprocess(const BatchArgs& a, hipStream_t st) {
+    if (a.scene.N == 0) return;
+    const int d = a.scene.sh_degree;
+    if (a.scene.format == GPS_FORMAT_FP32) {
+        if (d == 0) launch_batch_pre_t<GPS_FORMAT_FP32, 0>(a, st);
+        else if (d == 1) launch_batch_pre_t<GPS_FORMAT_FP32, 1>(a, st);
+        else if (d == 2) launch_batch_pre_t<GPS_FORMAT_FP32, 2>(a, st);
+        else launch_batch_pre_t<GPS_FORMAT_FP32, 3>(a, st);
+    } else {
+        if (d == 0) launch_batch_pre_t<GPS_FORMAT_HALF_FP8, 0>(a, st);
+        else if (d == 1) launch_batch_pre_t<GPS_FORMAT_HALF_FP8, 1>(a, st);
+        else if (d == 2) launch_batch_pre_t<GPS_FORMAT_HALF_FP8, 2>(a, st);
+        else launch_batch_pre_t<GPS_FORMAT_HALF_FP8, 3>(a, st);
+    }
+}
+
+void launch_batch_distribute(const BatchArgs& a, hipStream_t st) {
+    const uint32_t ntiles = a.tiles_per_view * (uint32_t)a.nviews;
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, st, a.tile_sum, a.tile_cnt, ntiles, a.tile_off,
+                       a.tile_voff, a.ctr, 0xFFFFFFFFu, a.iteration);
+    hipLaunchKernelGGL(batch_views_kernel, dim3((a.nviews + 255) / 256), dim3(256), 0, st, a.views, a.nviews,
+                       (const uint32_t*)a.tile_off, (const uint32_t*)a.tile_voff, a.tiles_per_view,
+                       (const PassCounters*)a.ctr);
+    if (ntiles == 0) return;
+    uint32_t shift = 0;
+    while ((1u << shift) < (uint32_t)(a.threads * a.ppt)) ++shift;
+    hipLaunchKernelGGL(distribute_kernel, dim3(ntiles), dim3(256), 0, st, (const uint32_t*)a.weights,
+                       (const uint32_t*)a.tile_off, (const uint32_t*)a.tile_voff, ntiles, (const PassCounters*)a.ctr,
+                       a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.scene.N, a.tiles_per_view, a.npad,
+                       a.mask_stride, a.cview, shift);
+}
+
+template <int THREADS, int PPT> static void launch_batch_splat_t(const BatchArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((batch_splat_kernel<THREADS, PPT>), dim3(a.grid), dim3(THREADS), 0, st,
+                       (const SplatRecord*)a.recs, (const uint32_t*)a.cidx, (const uint32_t*)a.coff,
+                       (const uint8_t*)a.cview, (const uint32_t*)a.chunk_start, (const PassCounters*)a.ctr,
+                       (const BatchView*)a.views, reinterpret_cast<unsigned long long*>(a.fb), a.xcds, a.atomics);
+}
+
+// Splat variants the batch autotuner chooses from: (threads, points per thread).
+const int kBatchSplatVariants[][2] = {{256, 4}, {256, 2}, {256, 8}, {128, 4}, {512, 2}, {512, 4}};
+const int kNumBatchSplatVariants = 6;
+
+bool launch_batch_splat(const BatchArgs& a, hipStream_t st) {
+    const int t = a.threads, q = a.ppt;
+    if (t == 256 && q == 4) launch_batch_splat_t<256, 4>(a, st);
+    else if (t == 256 && q == 2) launch_batch_splat_t<256, 2>(a, st);
+    else if (t == 256 && q == 8) launch_batch_splat_t<256, 8>(a, st);
+    else if (t == 128 && q == 4) launch_batch_splat_t<128, 4>(a, st);
+    else if (t == 512 && q == 2) launch_batch_splat_t<512, 2>(a, st);
+    else if (t == 512 && q == 4) launch_batch_splat_t<512, 4>(a, st);
+    else return false;
+    return true;
 }
 
 }  // namespace gps

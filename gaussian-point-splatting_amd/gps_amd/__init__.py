@@ -69,7 +69,8 @@ class SceneDesc(C.Structure):
 class RenderSettingsC(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("render_pass_count", C.c_int32),
                 ("supersampling_factor", C.c_int32), ("max_number_of_points", C.c_int64),
-                ("fully_disable_hierarchical_culling", C.c_int32), ("gaussians_per_block", C.c_int32)]
+                ("fully_disable_hierarchical_culling", C.c_int32), ("gaussians_per_block", C.c_int32),
+                ("flexible_frame_size", C.c_int32)]
 
 
 class FrameParams(C.Structure):
@@ -79,7 +80,8 @@ class FrameParams(C.Structure):
                 ("supersampling_factor", C.c_int32), ("frame", C.c_int32),
                 ("enable_occlusion_culling", C.c_int32), ("enable_hierarchical_culling", C.c_int32),
                 ("cull_small_gaussians", C.c_int32), ("use_unbiased_2d_splatting", C.c_int32),
-                ("reduce_point_count", C.c_int32), ("occlusion_culling_min_reduce_count", C.c_int32)]
+                ("reduce_point_count", C.c_int32), ("occlusion_culling_min_reduce_count", C.c_int32),
+                ("samples_per_pixel", C.c_int32)]
 
     def copy(self) -> "FrameParams":
         f = FrameParams()
@@ -450,6 +452,7 @@ class RenderSettings:
     max_number_of_points: int = 250_000_000
     fully_disable_hierarchical_culling: bool = False
     gaussians_per_block: int = 0
+    flexible_frame_size: bool = False
 
     def c(self) -> RenderSettingsC:
         s = RenderSettingsC()
@@ -459,6 +462,7 @@ class RenderSettings:
         s.max_number_of_points = self.max_number_of_points
         s.fully_disable_hierarchical_culling = int(self.fully_disable_hierarchical_culling)
         s.gaussians_per_block = self.gaussians_per_block
+        s.flexible_frame_size = int(self.flexible_frame_size)
         return s
 
 
@@ -552,7 +556,8 @@ def taa_reprojection(prev_vp, view, proj) -> np.ndarray:
 def frame_params(eye, center, up=(0.0, 1.0, 0.0), fovy: float = 45.0, width: int = 1280, height: int = 720,
                  ss: int = 1, near: float = 0.2, far: float = 1e4, frame: int = 1, occlusion: bool = False,
                  hierarchical: bool = False, unbiased: bool = True, reduce_point_count: bool = True,
-                 cull_small: bool = False, background=(0.0, 0.0, 0.0), min_reduce_count: int = 0) -> FrameParams:
+                 cull_small: bool = False, background=(0.0, 0.0, 0.0), min_reduce_count: int = 0,
+                 spp: int = 0) -> FrameParams:
     """Frame inputs for a look-at camera (view = glm::lookAt, proj = glm::perspective)."""
     f = FrameParams()
     e, ep = _f(eye)
@@ -566,6 +571,7 @@ def frame_params(eye, center, up=(0.0, 1.0, 0.0), fovy: float = 45.0, width: int
     f.reduce_point_count = int(reduce_point_count)
     f.cull_small_gaussians = int(cull_small)
     f.occlusion_culling_min_reduce_count = min_reduce_count
+    f.samples_per_pixel = spp
     for i in range(3):
         f.background[i] = background[i]
     return f
@@ -585,3 +591,208 @@ def resolve_progressive(color: DeviceArray, accum: DeviceArray, width: int, heig
                         out_rgb8: DeviceArray, stream: Optional[Stream] = None) -> None:
     check(_lib.gps_resolve_progressive(color.ptr, accum.ptr, width, height, num_frames, out_rgb8.ptr,
                                        stream.handle if stream else None), "progressive")
+
+
+# ----------------------------------------------------------------- serving
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_width", C.c_int32), ("max_height", C.c_int32), ("max_spp", C.c_int32),
+                ("max_supersampling", C.c_int32), ("tick_us", C.c_int32), ("device", C.c_int32),
+                ("hbm_budget_bytes", C.c_uint64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("reserved", C.c_int32), ("frame", FrameParams), ("out_rgb8", C.c_void_p),
+                ("out_color", C.c_void_p), ("arrival_s", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("slot", C.c_int32), ("arrival_s", C.c_double),
+                ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("passes_created", C.c_uint64),
+                ("passes_evicted", C.c_uint64), ("max_queue", C.c_uint64), ("max_batch", C.c_uint64),
+                ("ticks_with_admissions", C.c_uint64), ("hbm_budget_bytes", C.c_uint64),
+                ("hbm_used_bytes", C.c_uint64), ("clock_s", C.c_double), ("slot_busy_s", C.c_double)]
+
+
+_sig("gps_pass_device_bytes", C.c_uint64, _vp)
+_sig("gps_pass_estimate_bytes", C.c_uint64, C.POINTER(RenderSettingsC), _vp)
+_sig("gps_server_create", C.c_int32, C.POINTER(ServerConfig), C.POINTER(_vp))
+_sig("gps_server_add_scene", C.c_int32, _vp, _vp, C.POINTER(C.c_int32))
+_sig("gps_server_prepare", C.c_int32, _vp)
+_sig("gps_server_start", C.c_int32, _vp)
+_sig("gps_server_clock", C.c_double, _vp)
+_sig("gps_server_submit", C.c_int32, _vp, C.POINTER(Request), C.POINTER(C.c_uint64))
+_sig("gps_server_poll", C.c_int32, _vp, C.POINTER(RequestResult), C.c_int32, C.c_int32)
+_sig("gps_server_drain", C.c_int32, _vp, C.c_int32)
+_sig("gps_server_get_stats", C.c_int32, _vp, C.POINTER(ServerStats))
+_sig("gps_server_last_error", C.c_char_p, _vp)
+_sig("gps_server_destroy", None, _vp)
+
+
+class Server:
+    """Serving loop over resident scenes (gps_server_*): independent frames of
+    mixed size / spp, admitted by a native scheduler thread into HIP-stream
+    slots as they arrive and memory allows."""
+
+    def __init__(self, slots: int = 4, max_width: int = 1920, max_height: int = 1080, max_spp: int = 4,
+                 max_supersampling: int = 2, tick_us: int = 200, hbm_budget_bytes: int = 0):
+        cfg = ServerConfig(slots, max_width, max_height, max_spp, max_supersampling, tick_us, 0, hbm_budget_bytes)
+        h = C.c_void_p()
+        check(_lib.gps_server_create(C.byref(cfg), C.byref(h)), "server create")
+        self.handle = h
+        self.max_width, self.max_height = max_width, max_height
+        self.scenes = []
+        self._keep = {}
+
+    def add_scene(self, scene: Scene) -> int:
+        i = C.c_int32(0)
+        check(_lib.gps_server_add_scene(self.handle, scene.handle, C.byref(i)), "add scene")
+        self.scenes.append(scene)
+        return int(i.value)
+
+    def prepare(self) -> None:
+        """Create every slot's pass for every scene now (within the budget)."""
+        check(_lib.gps_server_prepare(self.handle), "server prepare")
+
+    def start(self) -> None:
+        check(_lib.gps_server_start(self.handle), "server start")
+
+    def clock(self) -> float:
+        return float(_lib.gps_server_clock(self.handle))
+
+    def submit(self, scene: int, frame: FrameParams, arrival_s: float = -1.0, out_rgb8: Optional[np.ndarray] = None,
+               out_color: Optional[np.ndarray] = None) -> int:
+        r = Request()
+        r.scene = scene
+        r.frame = frame
+        r.arrival_s = arrival_s
+        n = frame.width * frame.height
+        if out_rgb8 is not None:
+            assert out_rgb8.dtype == np.uint8 and out_rgb8.size == 3 * n and out_rgb8.flags.c_contiguous
+            r.out_rgb8 = out_rgb8.ctypes.data
+        if out_color is not None:
+            assert out_color.dtype == np.float32 and out_color.size == 3 * n and out_color.flags.c_contiguous
+            r.out_color = out_color.ctypes.data
+        t = C.c_uint64(0)
+        check(_lib.gps_server_submit(self.handle, C.byref(r), C.byref(t)), "submit")
+        self._keep[int(t.value)] = (out_rgb8, out_color)  # alive until the result is polled
+        return int(t.value)
+
+    def poll(self, max_results: int = 1024, wait_ms: int = 0) -> list:
+        buf = (RequestResult * max_results)()
+        n = _lib.gps_server_poll(self.handle, buf, max_results, wait_ms)
+        if n < 0:
+            raise GpsError(1, "poll")
+        out = []
+        for i in range(n):
+            r = buf[i]
+            self._keep.pop(int(r.ticket), None)
+            out.append({"ticket": int(r.ticket), "status": int(r.status), "slot": int(r.slot),
+                        "arrival_s": r.arrival_s, "admit_s": r.admit_s, "done_s": r.done_s})
+        return out
+
+    def drain(self, timeout_ms: int = 600_000) -> None:
+        check(_lib.gps_server_drain(self.handle, timeout_ms), "drain")
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(_lib.gps_server_get_stats(self.handle, C.byref(s)), "server stats")
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.gps_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- batches
+class BatchSettings(C.Structure):
+    _fields_ = [("max_views", C.c_int32), ("max_width", C.c_int32), ("max_height", C.c_int32),
+                ("max_spp", C.c_int32), ("max_supersampling", C.c_int32),
+                ("fully_disable_hierarchical_culling", C.c_int32), ("gaussians_per_block", C.c_int32),
+                ("reserved", C.c_int32), ("max_number_of_points", C.c_int64)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("splat_threads", C.c_int32), ("splat_points_per_thread", C.c_int32), ("use_graph", C.c_int32),
+                ("variants", C.c_int32), ("variant_threads", C.c_int32 * 8),
+                ("variant_points_per_thread", C.c_int32 * 8), ("variant_ms", C.c_float * 8)]
+
+
+_sig("gps_batch_create", C.c_int32, C.POINTER(BatchSettings), _vp, C.POINTER(_vp))
+_sig("gps_batch_render", C.c_int32, _vp, C.c_int32, C.POINTER(FrameParams), C.POINTER(_vp), C.POINTER(_vp), _vp)
+_sig("gps_batch_autotune", C.c_int32, _vp, C.c_int32, C.POINTER(FrameParams), C.POINTER(_vp), C.POINTER(_vp), _vp,
+     C.c_int32, C.POINTER(BatchTuning))
+_sig("gps_batch_set_tuning", C.c_int32, _vp, C.POINTER(BatchTuning))
+_sig("gps_batch_view_stats", C.c_int32, _vp, C.c_int32, C.POINTER(Stats), _vp)
+_sig("gps_batch_device_bytes", C.c_uint64, _vp)
+_sig("gps_batch_destroy", None, _vp)
+
+
+class Batch:
+    """Many views of one scene per launch sequence (gps_batch_*)."""
+
+    def __init__(self, scene: Scene, max_views: int, max_width: int, max_height: int, max_spp: int = 4,
+                 max_supersampling: int = 2, gaussians_per_block: int = 0, hierarchical: bool = True,
+                 max_number_of_points: int = 250_000_000):
+        st = BatchSettings(max_views, max_width, max_height, max_spp, max_supersampling, int(not hierarchical),
+                           gaussians_per_block, 0, max_number_of_points)
+        h = C.c_void_p()
+        check(_lib.gps_batch_create(C.byref(st), scene.handle, C.byref(h)), "batch create")
+        self.handle = h
+        self.scene = scene
+
+    @staticmethod
+    def _arrays(frames, colors, depths):
+        n = len(frames)
+        fa = (FrameParams * n)(*frames)
+        ca = (_vp * n)(*[c.ptr for c in colors])
+        da = (_vp * n)(*[d.ptr for d in depths])
+        return n, fa, ca, da
+
+    def render(self, frames, colors, depths, stream: Optional[Stream] = None) -> None:
+        n, fa, ca, da = self._arrays(frames, colors, depths)
+        check(_lib.gps_batch_render(self.handle, n, fa, ca, da, stream.handle if stream else None), "batch render")
+
+    def autotune(self, frames, colors, depths, stream: Optional[Stream] = None, repeats: int = 3) -> dict:
+        n, fa, ca, da = self._arrays(frames, colors, depths)
+        t = BatchTuning()
+        check(_lib.gps_batch_autotune(self.handle, n, fa, ca, da, stream.handle if stream else None, repeats,
+                                      C.byref(t)), "batch autotune")
+        return {"splat_threads": t.splat_threads, "splat_points_per_thread": t.splat_points_per_thread,
+                "variants": [{"threads": t.variant_threads[i], "points_per_thread": t.variant_points_per_thread[i],
+                              "ms": round(float(t.variant_ms[i]), 4)} for i in range(t.variants)]}
+
+    def set_tuning(self, threads: int, points_per_thread: int, graph: bool) -> None:
+        t = BatchTuning()
+        t.splat_threads, t.splat_points_per_thread, t.use_graph = threads, points_per_thread, int(graph)
+        check(_lib.gps_batch_set_tuning(self.handle, C.byref(t)), "batch tuning")
+
+    def view_stats(self, view: int, stream: Optional[Stream] = None) -> dict:
+        s = Stats()
+        check(_lib.gps_batch_view_stats(self.handle, view, C.byref(s), stream.handle if stream else None), "stats")
+        return s.as_dict()
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.gps_batch_device_bytes(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.gps_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

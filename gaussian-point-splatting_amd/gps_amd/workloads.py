@@ -272,3 +272,103 @@ class FrameLoop:
 
     def close(self) -> None:
         self.pass_.close()
+
+
+# ----------------------------------------------------------------- serving
+@dataclass(frozen=True)
+class ServeWorkload:
+    """Render requests against resident scenes: Poisson arrivals at `rate`
+    per second for `seconds`; each request draws a uniform scene, a camera
+    on a sphere of that scene's radius around its centre (looking at the
+    centre), a resolution and an spp, supersampling `ss`.  Occlusion
+    culling is off for independent frames; block frustum culling is on."""
+    scenes: tuple                       # Workload per resident scene
+    radii: tuple
+    centers: tuple
+    rate: float = 200.0
+    seconds: float = 60.0
+    resolutions: tuple = ((854, 480), (1280, 720), (1920, 1080))
+    spps: tuple = (1, 4)
+    ss: int = 2
+    seed: int = 0x5EED0005
+
+
+SERVE = ServeWorkload(scenes=(WORKLOADS["c1"], WORKLOADS["c2"], WORKLOADS["c3"]), radii=(30.0, 350.0, 600.0),
+                      centers=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (375.0, 0.0, 0.0)))
+
+
+def serve_requests(sw: ServeWorkload, count: Optional[int] = None, rate: Optional[float] = None):
+    """[(scene index, FrameParams, arrival seconds)] for the workload (or its
+    first `count` requests), seeded and reproducible."""
+    rng = np.random.default_rng(sw.seed)
+    lam = sw.rate if rate is None else rate
+    out = []
+    t = 0.0
+    i = 0
+    while True:
+        t += rng.exponential(1.0 / lam) if lam > 0 else 0.0
+        if count is None and t > sw.seconds:
+            break
+        if count is not None and i >= count:
+            break
+        k = int(rng.integers(len(sw.scenes)))
+        w, h = sw.resolutions[int(rng.integers(len(sw.resolutions)))]
+        spp = int(sw.spps[int(rng.integers(len(sw.spps)))])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        c = np.array(sw.centers[k])
+        eye = tuple(c + sw.radii[k] * d)
+        f = frame_params(eye, tuple(c), (0.0, 1.0, 0.0), 45.0, w, h, ss=sw.ss, frame=i + 1, occlusion=False,
+                         hierarchical=True, spp=spp)
+        out.append((k, f, t))
+        i += 1
+    return out
+
+
+def serve_stats(results, t_first=None) -> dict:
+    """Latency percentiles and throughput of completed requests."""
+    lat = np.array([r["done_s"] - r["arrival_s"] for r in results])
+    done = np.array([r["done_s"] for r in results])
+    arr = np.array([r["arrival_s"] for r in results])
+    t0 = arr.min() if t_first is None else t_first
+    span = max(done.max() - t0, 1e-9)
+    return {"requests": len(results), "frames_per_s": len(results) / span, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "max_ms": float(lat.max() * 1e3), "span_s": span}
+
+
+# ----------------------------------------------------------------- batches
+@dataclass(frozen=True)
+class BatchWorkload:
+    """Many camera views of one scene per batch: `views` cameras on a sphere
+    of `radius` around the origin looking at it, resolution and spp drawn
+    per view, supersampling `ss`, block frustum culling."""
+    scene: Workload
+    views: int = 64
+    radius: float = 150.0
+    resolutions: tuple = ((854, 480), (1280, 720), (1920, 1080))
+    spps: tuple = (1, 2, 4)
+    ss: int = 2
+    seed: int = 0x5EED0004
+
+
+BATCH = BatchWorkload(scene=replace(WORKLOADS["c2"], fmt=FORMAT_HALF_FP8))
+
+
+def batch_views(bw: BatchWorkload, batch_index: int = 0, layout_seed: Optional[int] = None):
+    """FrameParams of one batch.  The size / spp layout depends only on the
+    workload seed (so a captured graph can be replayed); the camera poses and
+    RNG frames change with `batch_index`."""
+    lay = np.random.default_rng(bw.seed if layout_seed is None else layout_seed)
+    sizes = [bw.resolutions[int(lay.integers(len(bw.resolutions)))] for _ in range(bw.views)]
+    spps = [int(bw.spps[int(lay.integers(len(bw.spps)))]) for _ in range(bw.views)]
+    rng = np.random.default_rng((bw.seed << 16) + batch_index)
+    out = []
+    for v in range(bw.views):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = tuple(bw.radius * d)
+        w, h = sizes[v]
+        out.append(frame_params(eye, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, w, h, ss=bw.ss,
+                                frame=batch_index * bw.views + v + 1, occlusion=False, hierarchical=True,
+                                spp=spps[v]))
+    return out

@@ -80,6 +80,12 @@ typedef struct gps_render_settings {
     int32_t fully_disable_hierarchical_culling;
     int32_t gaussians_per_block;         /* culling block size; 0 = reference
                                             rule max(4, floor(N / 1e6)) */
+    int32_t flexible_frame_size;         /* 0: every frame is width x height with
+                                            render_pass_count spp, as in the
+                                            reference (main.cu:82-86); 1: frames
+                                            up to that size and spp are accepted
+                                            (serving); occlusion culling still
+                                            needs the full size */
 } gps_render_settings;
 
 /* Per-frame inputs; the RenderSettings fields read by
@@ -101,6 +107,8 @@ typedef struct gps_frame_params {
     int32_t use_unbiased_2d_splatting;
     int32_t reduce_point_count;
     int32_t occlusion_culling_min_reduce_count;
+    int32_t samples_per_pixel; /* 0 = the pass's render_pass_count; other values
+                                  need flexible_frame_size */
 } gps_frame_params;
 
 /* Counters of the last rendered frame (Statistics, render_settings.cuh:23-34,
@@ -146,6 +154,10 @@ uint64_t gps_scene_device_bytes(const gps_scene* scene);
  * must outlive the pass; several passes may share one scene. */
 gps_status gps_pass_create(const gps_render_settings* settings, gps_scene* scene, gps_pass** out);
 void gps_pass_destroy(gps_pass* pass);
+/* Device memory held by the pass (scene not included), and what
+ * gps_pass_create would allocate for these settings. */
+uint64_t gps_pass_device_bytes(const gps_pass* pass);
+uint64_t gps_pass_estimate_bytes(const gps_render_settings* settings, const gps_scene* scene);
 
 /* Replaces RenderPass::render(glm::vec3* d_image, float* d_depth, const
  * RenderSettings&) (src/core/rendering/render_pass.cuh:17-24) as implemented
@@ -283,6 +295,112 @@ gps_status gps_encode_half_fp8(const gps_scene_desc* desc, uint16_t* rotations, 
                                uint16_t* sh_dc, uint8_t* sh_rest, float* sh_scale);
 uint16_t gps_float_to_half(float f);
 uint8_t gps_float_to_fp8e4m3(float f);
+
+/* ------------------------------------------------------------- batches */
+/* Many camera views of one scene per launch sequence.  No reference
+ * counterpart (the reference renders one camera per frame,
+ * gaussian_point_splatting.cu:599-737).  Up to max_views frames of uneven
+ * size / spp / ss go through one block-cull launch, one preprocess launch
+ * (each Gaussian fetched once for all views), one scan + distribution over
+ * the concatenated per-view tiles, one persistent splat over all views'
+ * points, and a combine per view.  View k's colour / depth are bit-identical
+ * to the same frame rendered alone by gps_pass_render; occlusion culling is
+ * not available in a batch (it needs the previous frame of the same camera).
+ * With use_graph the launches are captured once per view layout (sizes,
+ * output pointers, stream) into a HIP graph and replayed for new cameras;
+ * a graph needs a non-null stream. */
+typedef struct gps_batch gps_batch;
+typedef struct gps_batch_settings {
+    int32_t max_views;                   /* 1..256 */
+    int32_t max_width, max_height;
+    int32_t max_spp;
+    int32_t max_supersampling;
+    int32_t fully_disable_hierarchical_culling;
+    int32_t gaussians_per_block;         /* 0 = max(4, floor(N / 1e6)) */
+    int32_t reserved;
+    int64_t max_number_of_points;        /* point budget per view */
+} gps_batch_settings;
+typedef struct gps_batch_tuning {
+    int32_t splat_threads;               /* workgroup size of the splat */
+    int32_t splat_points_per_thread;     /* points per thread (x 64 = points per wave) */
+    int32_t use_graph;
+    int32_t variants;                    /* filled by gps_batch_autotune */
+    int32_t variant_threads[8];
+    int32_t variant_points_per_thread[8];
+    float variant_ms[8];                 /* batch time of each variant */
+} gps_batch_tuning;
+gps_status gps_batch_create(const gps_batch_settings* settings, gps_scene* scene, gps_batch** out);
+gps_status gps_batch_render(gps_batch* batch, int32_t num_views, const gps_frame_params* views,
+                            float* const* d_colors, float* const* d_depths, void* stream);
+/* Times every splat variant on this batch (`repeats` batches each), keeps
+ * the fastest and reports all; the graph setting is kept. */
+gps_status gps_batch_autotune(gps_batch* batch, int32_t num_views, const gps_frame_params* views,
+                              float* const* d_colors, float* const* d_depths, void* stream, int32_t repeats,
+                              gps_batch_tuning* out);
+gps_status gps_batch_set_tuning(gps_batch* batch, const gps_batch_tuning* tuning);
+/* Counters of view `view` of the last batch (atomics: whole batch). */
+gps_status gps_batch_view_stats(gps_batch* batch, int32_t view, gps_stats* out, void* stream);
+uint64_t gps_batch_device_bytes(const gps_batch* batch);
+void gps_batch_destroy(gps_batch* batch);
+
+/* ------------------------------------------------------------- serving */
+/* Serving loop with continuous batching over resident scenes.  No reference
+ * counterpart: the reference renders one camera per frame from a single
+ * host loop (RenderServer::renderFrame, render_server.cu:86-160).  Requests
+ * are independent frames (camera, size <= max, spp <= max, RNG frame) of one
+ * of the added scenes; occlusion culling is off for them (it needs the
+ * previous frame of the same camera), block frustum culling stays on, and
+ * the output is the progressive resolve of that one frame.  A scheduler
+ * thread admits every arrived request that finds an idle slot (HIP stream)
+ * with a pass for its scene, or HBM budget to create one (evicting idle
+ * passes least-recently-used); work of different slots overlaps on the
+ * GPU.  Results are bit-identical whatever ran beside them. */
+typedef struct gps_server gps_server;
+typedef struct gps_server_config {
+    int32_t slots;                 /* requests in flight on the GPU, 1..64 */
+    int32_t max_width, max_height; /* largest request */
+    int32_t max_spp;
+    int32_t max_supersampling;
+    int32_t tick_us;               /* scheduler period */
+    int32_t device;
+    uint64_t hbm_budget_bytes;     /* for slot buffers + passes; 0 = 90% of free HBM */
+} gps_server_config;
+typedef struct gps_request {
+    int32_t scene;                 /* id from gps_server_add_scene */
+    int32_t reserved;
+    gps_frame_params frame;        /* width/height/samples_per_pixel/frame per request */
+    uint8_t* out_rgb8;             /* host, width*height*3, written on completion (may be NULL) */
+    float* out_color;              /* host, width*height*3 floats (may be NULL) */
+    double arrival_s;              /* server clock time the request arrives; < 0 = on submit */
+} gps_request;
+typedef struct gps_request_result {
+    uint64_t ticket;
+    int32_t status;                /* gps_status of the request */
+    int32_t slot;
+    double arrival_s, admit_s, done_s;  /* server clock, seconds */
+} gps_request_result;
+typedef struct gps_server_stats {
+    uint64_t submitted, completed;
+    uint64_t passes_created, passes_evicted;
+    uint64_t max_queue, max_batch, ticks_with_admissions;
+    uint64_t hbm_budget_bytes, hbm_used_bytes;
+    double clock_s, slot_busy_s;
+} gps_server_stats;
+gps_status gps_server_create(const gps_server_config* config, gps_server** out);
+/* The scene must outlive the server. */
+gps_status gps_server_add_scene(gps_server* server, gps_scene* scene, int32_t* scene_id);
+/* Optional, before start: create the pass of every (slot, scene) pair that
+ * fits the budget, so no request pays for an allocation. */
+gps_status gps_server_prepare(gps_server* server);
+gps_status gps_server_start(gps_server* server);
+double gps_server_clock(const gps_server* server);
+gps_status gps_server_submit(gps_server* server, const gps_request* request, uint64_t* ticket);
+/* Completed requests (up to max), waiting up to wait_ms for the first. */
+int32_t gps_server_poll(gps_server* server, gps_request_result* out, int32_t max, int32_t wait_ms);
+gps_status gps_server_drain(gps_server* server, int32_t timeout_ms);
+gps_status gps_server_get_stats(gps_server* server, gps_server_stats* out);
+const char* gps_server_last_error(const gps_server* server);
+void gps_server_destroy(gps_server* server);
 
 /* ---------------------------------------------------------- self-tests */
 /* Device evaluation of the shared deterministic scalar math and RNG, used to

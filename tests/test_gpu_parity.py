@@ -291,3 +291,156 @@ def test_resolve_taa_and_progressive(gpu):
         assert_same(f"prog color {n}", dc.download(), c2)
         assert_same(f"prog accum {n}", dp.download(), a2)
         assert_same(f"prog rgb {n}", rgb.download(), r2)
+
+
+# ------------------------------------------------------------------ serving
+def test_flexible_pass_matches_exact_pass_and_oracle(gpu):
+    """A pass created at 320x200 spp 4 renders a 160x96 spp 1 frame exactly
+    like a pass (and the oracle) created at that size."""
+    import orc
+    arrays = H.scene(0x5EED0501, 20_000, H.C2, clusters=40)
+    big = gpu.RenderSettings(320, 200, 4, 2, flexible_frame_size=True)
+    small = gpu.RenderSettings(160, 96, 1, 2)
+    f = gpu.frame_params((0, 80, -300), (0, 0, 0), (0, 1, 0), 45.0, 160, 96, ss=2, frame=3, hierarchical=True,
+                         spp=1)
+    scene = gpu.Scene(arrays)
+    flex = gpu.Pass(big, scene)
+    color = gpu.DeviceArray((96, 160, 3), np.float32, zero=True)
+    depth = gpu.DeviceArray((96, 160), np.float32, zero=True)
+    flex.render(f, color, depth)
+    fb_flex = flex.debug(gpu.DBG_FRAMEBUFFER)
+    rs = H.GpuRunner(arrays, small).render(f, [gpu.DBG_FRAMEBUFFER])
+    o = orc.OraclePass(small, arrays, 0, 8)
+    c, d = o.render(f)
+    assert_same("flex fb", fb_flex, rs.debug[gpu.DBG_FRAMEBUFFER])
+    assert_same("flex fb vs oracle", fb_flex, o.debug(gpu.DBG_FRAMEBUFFER))
+    assert_same("flex color", color.download(), c)
+    assert_same("flex depth", depth.download(), d)
+    # frames larger than the pass are refused
+    with pytest.raises(gpu.GpsError):
+        flex.render(gpu.frame_params((0, 0, -30), (0, 0, 0), (0, 1, 0), 45.0, 400, 100, ss=2), color, depth)
+
+
+def test_serving_results_identical_alone_and_in_the_mix(gpu):
+    """Requests of three resident scenes with mixed size / spp, submitted
+    together to a 4-slot server, give the same images (colour floats and
+    RGB8, bit for bit) as the same requests rendered one at a time on a
+    1-slot server, and the same colour as the oracle."""
+    import orc
+    from gps_amd import workloads as W
+    scenes = (W.scaled(W.WORKLOADS["c1"], 20_000), W.scaled(W.WORKLOADS["c2"], 30_000),
+              W.scaled(W.WORKLOADS["c3"], 10_000))
+    sw = W.ServeWorkload(scenes=scenes, radii=(30.0, 350.0, 600.0),
+                         centers=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (375.0, 0.0, 0.0)),
+                         resolutions=((128, 72), (200, 120), (320, 180)), spps=(1, 4))
+    arrays = [W.host_scene(w) for w in scenes]
+    dev = [gpu.Scene(a, w.fmt) for a, w in zip(arrays, scenes)]
+    reqs = W.serve_requests(sw, count=24, rate=0.0)
+
+    def run(slots, one_at_a_time):
+        srv = gpu.Server(slots=slots, max_width=320, max_height=180, max_spp=4, max_supersampling=2)
+        ids = [srv.add_scene(s) for s in dev]
+        srv.start()
+        outs = []
+        for k, f, _ in reqs:
+            rgb = np.zeros((f.height, f.width, 3), np.uint8)
+            col = np.zeros((f.height, f.width, 3), np.float32)
+            srv.submit(ids[k], f, -1.0, rgb, col)
+            outs.append((rgb, col))
+            if one_at_a_time:
+                srv.drain(120_000)
+        srv.drain(120_000)
+        res = srv.poll(1024)
+        st = srv.stats()
+        srv.close()
+        assert len(res) == len(reqs) and all(r["status"] == 0 for r in res)
+        return outs, res, st
+
+    flush("serving: 4 slots, all requests at once")
+    mixed, res_m, st_m = run(4, False)
+    flush(f"serving stats {st_m}")
+    flush("serving: 1 slot, one at a time")
+    alone, _, _ = run(1, True)
+    assert len({r["slot"] for r in res_m}) > 1  # the mix really used several slots
+    for i, ((a_rgb, a_col), (m_rgb, m_col)) in enumerate(zip(alone, mixed)):
+        assert_same(f"request {i} colour", m_col, a_col)
+        assert_same(f"request {i} rgb", m_rgb, a_rgb)
+    # every fourth request against the oracle at its exact size
+    for i in range(0, len(reqs), 4):
+        k, f, _ = reqs[i]
+        st = gpu.RenderSettings(f.width, f.height, f.samples_per_pixel, 2)
+        o = orc.OraclePass(st, arrays[k], scenes[k].fmt, 8)
+        c, _ = o.render(f)
+        assert_same(f"request {i} vs oracle", mixed[i][1], c)
+
+
+# ------------------------------------------------------------------ batches
+def test_batch_views_identical_to_single_renders(gpu):
+    """A batch of views of uneven size / spp / ss (one without block culling)
+    gives every view's colour and depth bit for bit as a pass rendering that
+    frame alone, for every splat variant, eagerly and replayed from a HIP
+    graph with new cameras (each replay checked, eager work in between)."""
+    import orc
+    from gps_amd import workloads as W
+    w = W.scaled(W.WORKLOADS["c2"], 30_000, fmt=gpu.FORMAT_HALF_FP8)
+    arrays = W.host_scene(w)
+    scene = gpu.Scene(arrays, w.fmt)
+    bw = W.BatchWorkload(scene=w, views=6, radius=150.0, resolutions=((96, 64), (160, 90), (200, 120)),
+                         spps=(1, 2, 4), ss=2)
+    stream = gpu.Stream()
+    batch = gpu.Batch(scene, 6, 200, 120, max_spp=4, max_supersampling=2)
+    single = gpu.Pass(gpu.RenderSettings(200, 120, 4, 2, flexible_frame_size=True), scene)
+
+    def outputs(frames):
+        cols = [gpu.DeviceArray((f.height, f.width, 3), np.float32, zero=True) for f in frames]
+        deps = [gpu.DeviceArray((f.height, f.width), np.float32, zero=True) for f in frames]
+        return cols, deps
+
+    def alone(frames):
+        ref = []
+        for f in frames:
+            c = gpu.DeviceArray((f.height, f.width, 3), np.float32)
+            d = gpu.DeviceArray((f.height, f.width), np.float32)
+            single.render(f, c, d, stream)
+            stream.synchronize()
+            ref.append((c.download(), d.download(), single.stats(stream)))
+        return ref
+
+    def check_batch(frames, cols, deps, ref, what):
+        stream.synchronize()
+        for v, f in enumerate(frames):
+            assert_same(f"{what} view {v} colour", cols[v].download(), ref[v][0])
+            assert_same(f"{what} view {v} depth", deps[v].download(), ref[v][1])
+            st = batch.view_stats(v, stream)
+            for key in ("valid_gaussians", "num_points", "total_weight"):
+                assert st[key][0] == ref[v][2][key][0], (what, v, key)
+
+    frames = W.batch_views(bw, 0)
+    frames[1].enable_hierarchical_culling = 0
+    frames[2].supersampling_factor = 1
+    ref = alone(frames)
+    cols, deps = outputs(frames)
+    for t, q in ((256, 4), (256, 2), (256, 8), (128, 4), (512, 2), (512, 4)):
+        flush(f"batch eager, splat variant {t}x{q}")
+        batch.set_tuning(t, q, False)
+        batch.render(frames, cols, deps, stream)
+        check_batch(frames, cols, deps, ref, f"eager {t}x{q}")
+    # oracle for two views at their exact sizes
+    for v in (0, 3):
+        f = frames[v]
+        o = orc.OraclePass(gpu.RenderSettings(f.width, f.height, f.samples_per_pixel, f.supersampling_factor),
+                           arrays, w.fmt, 8)
+        c, _ = o.render(f)
+        assert_same(f"view {v} vs oracle", ref[v][0], c)
+    tuned = batch.autotune(frames, cols, deps, stream, repeats=1)
+    flush(f"autotune: {tuned}")
+    batch.set_tuning(tuned["splat_threads"], tuned["splat_points_per_thread"], True)
+    for k in range(3):
+        fr = W.batch_views(bw, k + 1)
+        fr[1].enable_hierarchical_culling = 0
+        fr[2].supersampling_factor = 1
+        flush(f"batch graph replay {k}")
+        batch.render(fr, cols, deps, stream)
+        stream.synchronize()
+        r = alone(fr)  # eager single-view work between replays
+        check_batch(fr, cols, deps, r, f"graph {k}")
