@@ -37,6 +37,7 @@ struct Options {
     float background[3] = {0, 0, 0};
     float rotation[3] = {0, 0, 0}, translation[3] = {0, 0, 0}, scaling[3] = {1, 1, 1};
     std::string model_path;
+    bool sort_morton = false;
     // headless
     std::string synthetic = "c1";
     long long num_gaussians = -1;
@@ -85,10 +86,11 @@ Options parse(int argc, char** argv) {
         else if (a == "--taa") o.taa = true;
         else if (a == "--run_tests" || a == "--run-tests") o.run_tests = true;
         else if (a == "--model-path") o.model_path = next();
+        else if (a == "--sort_morton_order" || a == "--sort-morton-order") o.sort_morton = true;
         else if (a == "--max-eval-count" || a == "--max_eval_count" || a == "--cameras-path" || a == "--camera-path" ||
                  a == "--sorted-model-output-path" || a == "--sorted_model_output_path") next();
         else if (a == "--freeze-culling" || a == "--enable_profiling" || a == "--disable_gui" || a == "--disable-gui" ||
-                 a == "--sort_morton_order" || a == "--sort-morton-order" || a == "--play_camera_path" ||
+                 a == "--play_camera_path" ||
                  a == "--play-camera-path" || a == "--snap_window_to_corner" || a == "--snap-window-to-corner" ||
                  a == "--create-evaluation-screenshots" || a == "--create_evaluation_screenshots") {
         }
@@ -173,11 +175,7 @@ void write_pfm(const std::string& path, const std::vector<float>& c, int w, int 
 
 int main(int argc, char** argv) {
     Options o = parse(argc, argv);
-    if (!o.model_path.empty()) {
-        std::fprintf(stderr, "--model-path: PLY scenes are not supported by this driver; use --synthetic\n");
-        return 2;
-    }
-    // ---- scene
+    // ---- scene: a 3DGS PLY file (gaussian_loader.cu) or a synthetic scene
     gps_synth_params sp;
     std::memset(&sp, 0, sizeof(sp));
     sp.seed = o.seed ? o.seed : 0x5EED0001ull;
@@ -205,10 +203,36 @@ int main(int argc, char** argv) {
         sp.dc_sigma = 0.3;
     }
     if (o.num_gaussians >= 0) sp.num_gaussians = (uint64_t)o.num_gaussians;
+    if (!o.model_path.empty()) {
+        gps_ply_info info;
+        CHECK(gps_ply_info_read(o.model_path.c_str(), &info));
+        sp.num_gaussians = info.num_gaussians;
+        sp.sh_degree = 3;
+    }
     const uint64_t N = sp.num_gaussians;
     const int nc = (sp.sh_degree + 1) * (sp.sh_degree + 1);
     std::vector<float> means(3 * N), rots(4 * N), scales(3 * N), ops(N), shs((size_t)3 * nc * N);
-    CHECK(gps_synth_generate(&sp, 0, N, means.data(), rots.data(), scales.data(), ops.data(), shs.data()));
+    if (!o.model_path.empty()) {
+        uint64_t got = 0;
+        CHECK(gps_ply_load(o.model_path.c_str(), o.sort_morton ? 1 : 0, N, means.data(), rots.data(), scales.data(),
+                           ops.data(), shs.data(), &got));
+    } else {
+        CHECK(gps_synth_generate(&sp, 0, N, means.data(), rots.data(), scales.data(), ops.data(), shs.data()));
+        if (o.sort_morton && N > 1) {  // gaussian_loader.cu:363-426
+            std::vector<uint32_t> perm(N);
+            CHECK(gps_morton_order(means.data(), N, perm.data()));
+            auto permute = [&](std::vector<float>& a, size_t w) {
+                std::vector<float> t(a.size());
+                for (uint64_t i = 0; i < N; ++i) std::memcpy(&t[w * i], &a[w * perm[i]], 4 * w);
+                a.swap(t);
+            };
+            permute(means, 3);
+            permute(rots, 4);
+            permute(scales, 3);
+            permute(ops, 1);
+            permute(shs, (size_t)3 * nc);
+        }
+    }
     gps_scene_desc d;
     d.num_gaussians = N;
     d.sh_degree = sp.sh_degree;
@@ -318,8 +342,9 @@ int main(int argc, char** argv) {
     if (o.run_tests) {
         // src/main.cu:211-241 envelope
         std::printf("---BEGIN_TIMINGS_JSON---\n{\n  \"algorithm\": \"GaussianPointSplatting\",\n");
-        std::printf("  \"model_path\": \"synthetic:%s\",\n  \"timings\": [\n    { \"camera\": 0, \"samples\": [",
-                    o.synthetic.c_str());
+        std::printf("  \"model_path\": \"%s%s\",\n  \"timings\": [\n    { \"camera\": 0, \"samples\": [",
+                    o.model_path.empty() ? "synthetic:" : "",
+                    o.model_path.empty() ? o.synthetic.c_str() : o.model_path.c_str());
         for (size_t i = 0; i < times.size(); ++i) std::printf("%s%.4f", i ? ", " : "", times[i]);
         std::printf("] }\n  ]\n}\n---END_TIMINGS_JSON---\n");
     }

@@ -796,3 +796,38 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ----------------------------------------------------------------- PLY files
+class PlyInfo(C.Structure):
+    _fields_ = [("num_gaussians", C.c_uint64), ("record_bytes", C.c_uint64), ("has_rest", C.c_int32),
+                ("bvh_sorted", C.c_int32)]
+
+
+_sig("gps_ply_info_read", C.c_int32, C.c_char_p, C.POINTER(PlyInfo))
+_sig("gps_ply_load", C.c_int32, C.c_char_p, C.c_int32, C.c_uint64, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_uint64))
+_sig("gps_ply_write", C.c_int32, C.c_char_p, C.POINTER(SceneDesc), C.c_int32)
+
+
+def ply_info(path: str) -> dict:
+    i = PlyInfo()
+    check(_lib.gps_ply_info_read(os.fsencode(path), C.byref(i)), f"ply header {path}")
+    return {"num_gaussians": int(i.num_gaussians), "record_bytes": int(i.record_bytes),
+            "has_rest": bool(i.has_rest), "bvh_sorted": bool(i.bvh_sorted)}
+
+
+def load_ply(path: str, sort_morton_order: bool = False) -> SceneArrays:
+    """A 3DGS PLY file as SceneArrays (SH degree 3), with the reference
+    loader's activations and optional load-time Morton order."""
+    n = ply_info(path)["num_gaussians"]
+    a = SceneArrays(np.empty((n, 3), np.float32), np.empty((n, 4), np.float32), np.empty((n, 3), np.float32),
+                    np.empty((n,), np.float32), np.empty((n, 48), np.float32), 3)
+    got = C.c_uint64(0)
+    check(_lib.gps_ply_load(os.fsencode(path), int(sort_morton_order), n, _ptr(a.means), _ptr(a.rotations),
+                            _ptr(a.scales), _ptr(a.opacities), _ptr(a.shs), C.byref(got)), f"ply load {path}")
+    return a
+
+
+def write_ply(path: str, s: SceneArrays, mark_sorted: bool = False) -> None:
+    d = s.desc(FORMAT_FP32)
+    check(_lib.gps_ply_write(os.fsencode(path), C.byref(d), int(mark_sorted)), f"ply write {path}")

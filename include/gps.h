@@ -285,6 +285,28 @@ gps_status gps_synth_generate_indices(const gps_synth_params* p, const uint64_t*
  * a stable sort of the 63-bit Morton codes of the bbox-normalized positions. */
 gps_status gps_morton_order(const float* means, uint64_t n, uint32_t* perm);
 
+/* ---------------------------------------------------------- PLY files */
+/* 3D Gaussian Splatting PLY (binary little-endian), as the reference's
+ * GaussianLoader reads and writes it (src/core/gaussians/gaussian_loader.cu:
+ * 105-427 load, 455-562 write).  Loading applies the reference's
+ * activations: opacity clamp(sigmoid), scale exp, rotation normalized and
+ * kept (w, x, y, z), SH DC + f_rest de-interleaved to coefficient-major RGB
+ * (shs has 48 floats per Gaussian; all higher bands 0 unless f_rest_44 is
+ * declared).  sort_morton_order reorders like --sort-morton-order unless the
+ * file has the "bvh_sorted_order" comment. */
+typedef struct gps_ply_info {
+    uint64_t num_gaussians;
+    uint64_t record_bytes;
+    int32_t has_rest;      /* f_rest_0..44 declared */
+    int32_t bvh_sorted;    /* "bvh_sorted_order" comment present */
+} gps_ply_info;
+gps_status gps_ply_info_read(const char* path, gps_ply_info* out);
+gps_status gps_ply_load(const char* path, int32_t sort_morton_order, uint64_t capacity, float* means,
+                        float* rotations, float* scales, float* opacities, float* shs, uint64_t* loaded);
+/* Writes the reference's full record layout (zero normals, opacity logit,
+ * log scale); mark_sorted adds the "bvh_sorted_order" comment. */
+gps_status gps_ply_write(const char* path, const gps_scene_desc* desc, int32_t mark_sorted);
+
 /* ----------------------------------------------------- compact encoding */
 /* Host encoder of GPS_FORMAT_HALF_FP8 (the same code gps_scene_create uses):
  * fp16 RNE rotation (N*4), scale (N*3), opacity (N), SH DC (N*3); the
