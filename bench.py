@@ -280,10 +280,28 @@ def main() -> int:
     # Work counters: replay the same deterministic frame sequence on a fresh
     # pass (same initial state), reading the counters after every frame.
     loop = W.FrameLoop(w, scene, stream)
+    loop.pass_.set_counters(True)
     per_frame = []
     run(loop, range(Wm))
     run(loop, range(Wm, Wm + K), per_frame)
     loop.close()
+    # The same frames with each splat form forced (binned vs point-parallel
+    # with one global atomic per sample): the chosen and the plain form.
+    splat_forms = {}
+    for name, mode in (("point_parallel", G.Pass.SPLAT_POINT_PARALLEL), ("binned", G.Pass.SPLAT_BINNED)):
+        loop = W.FrameLoop(w, scene, stream)
+        loop.pass_.set_splat_mode(mode)
+        run(loop, range(Wm))
+        stream.synchronize()
+        loop.pass_.set_timing(K + 1)
+        t0 = time.perf_counter()
+        run(loop, range(Wm, Wm + K))
+        stream.synchronize()
+        sp = loop.pass_.timing_summary()["stages"]["splat"]["ms"] / K
+        splat_forms[name] = {"ms_per_frame": round((time.perf_counter() - t0) * 1e3 / K, 4),
+                             "splat_ms_per_frame": round(sp, 4)}
+        loop.close()
+    log(f"[bench] splat forms: {splat_forms}")
     samples = sum(st["samples"] for st in per_frame)
     atomics = sum(st["atomics"] for st in per_frame)
     elapsed_s = max(gpu_ms, wall * 1e3) / 1e3
@@ -336,6 +354,7 @@ def main() -> int:
         "two_pass_frames": sum(1 for p in passes if p == 2),
         "valid_gaussians_per_frame": round(sum(sum(st["valid_gaussians"][:st["passes"]]) for st in per_frame) / K),
         "culled_blocks_per_frame": round(sum(sum(st["culled_blocks"][:st["passes"]]) for st in per_frame) / K),
+        "splat_forms": splat_forms,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batch_report,

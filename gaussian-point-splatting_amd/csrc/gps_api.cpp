@@ -78,6 +78,21 @@ struct gps_pass {
     uint32_t *cidx = nullptr, *coff = nullptr, *chunk_start = nullptr;
     FrameCounters* ctr = nullptr;
     uint32_t* dbg_indices = nullptr;
+    // binned splat
+    int32_t splat_mode = GPS_SPLAT_AUTO;
+    int32_t count_atomics = 0;
+    uint32_t cus = 256;
+    uint2* erect = nullptr;
+    uint32_t* ew = nullptr;
+    uint32_t *reg_cnt = nullptr, *reg_pts = nullptr, *reg_ecnt = nullptr, *reg_ptot = nullptr, *reg_eoff = nullptr,
+             *reg_poff = nullptr, *reg_cursor = nullptr, *reg_jobs = nullptr;
+    uint32_t *job_reg = nullptr, *job_p0 = nullptr;
+    uint32_t *list = nullptr, *lw = nullptr, *lpre = nullptr;
+    uint32_t *tsum = nullptr, *toff = nullptr;
+    uint32_t* big = nullptr;
+    uint32_t* bin_state = nullptr;
+    uint32_t list_cap = 0, job_cap = 0;
+    int last_binned = 0;
     uint64_t dbg_indices_cap = 0;
     // framebuffers
     uint64_t* fbA = nullptr;
@@ -392,6 +407,27 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     chk(dalloc(&p->cidx, N, false));
     chk(dalloc(&p->coff, N, false));
     chk(dalloc(&p->chunk_start, (uint64_t)p->budget / GPS_SPLAT_CHUNK + 2, true));
+    // Binned splat lists: up to 2 regions per valid Gaussian on average
+    // (overflow falls back to the point-parallel splat on the device).
+    {
+        const uint64_t cap = round_up(std::min<uint64_t>(2 * std::max<uint64_t>(N, 1) + 65536, 0xFFFFF000ull), GPS_TILE);
+        p->list_cap = (uint32_t)cap;
+        p->job_cap = GPS_BIN_HIST + p->budget / GPS_BIN_JOB_POINTS + 1;
+        chk(dalloc(&p->erect, std::max<uint64_t>(N, 1), false));
+        chk(dalloc(&p->ew, std::max<uint64_t>(N, 1), false));
+        chk(dalloc(&p->big, std::max<uint64_t>(N, 1), false));
+        for (uint32_t** a : {&p->reg_cnt, &p->reg_pts, &p->reg_ecnt, &p->reg_ptot, &p->reg_eoff, &p->reg_poff,
+                             &p->reg_cursor, &p->reg_jobs})
+            chk(dalloc(a, GPS_BIN_HIST, true));
+        chk(dalloc(&p->job_reg, p->job_cap, false));
+        chk(dalloc(&p->job_p0, p->job_cap, false));
+        chk(dalloc(&p->list, cap, false));
+        chk(dalloc(&p->lw, cap, true));
+        chk(dalloc(&p->lpre, cap, false));
+        chk(dalloc(&p->tsum, cap / GPS_TILE + 1, false));
+        chk(dalloc(&p->toff, cap / GPS_TILE + 1, false));
+        chk(dalloc(&p->bin_state, 8, true));
+    }
     chk(dalloc(&p->ctr, 1, true));
     // Hi-Z pyramid at output resolution, initialised to +inf (nothing occluded).
     p->hiz_size = (uint32_t)mip_total(p->hiz, st->width, st->height);
@@ -428,6 +464,7 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     // chunk schedule only needs the count to be a divisor of the grid.
     p->xcds = (cus % 8 == 0) ? 8 : 1;
     p->splat_grid = (uint32_t)cus * 8u;
+    p->cus = (uint32_t)cus;
     if (s == GPS_OK) {
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) s = hip_fail(e, "pass setup");
@@ -437,6 +474,19 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
         return s;
     }
     *out = p;
+    return GPS_OK;
+}
+
+gps_status gps_pass_set_splat_mode(gps_pass* p, int32_t mode) {
+    if (!p || (mode != GPS_SPLAT_AUTO && mode != GPS_SPLAT_POINT_PARALLEL && mode != GPS_SPLAT_BINNED))
+        return GPS_ERR_INVALID_ARG;
+    p->splat_mode = mode;
+    return GPS_OK;
+}
+
+gps_status gps_pass_set_counters(gps_pass* p, int32_t count_atomics) {
+    if (!p) return GPS_ERR_INVALID_ARG;
+    p->count_atomics = count_atomics ? 1 : 0;
     return GPS_OK;
 }
 
@@ -451,6 +501,11 @@ uint64_t gps_pass_estimate_bytes(const gps_render_settings* st, const gps_scene*
                                  st->render_pass_count);
     uint64_t b = npad * 4 + 2 * nwords * 4 + std::max<uint64_t>(N, 1) * (sizeof(SplatRecord) + 8) +
                  16 * ntiles + (budget / GPS_SPLAT_CHUNK + 2) * 4 + sizeof(FrameCounters) + hiz * 4 + 2 * fb * 8;
+    {
+        const uint64_t cap = round_up(std::min<uint64_t>(2 * std::max<uint64_t>(N, 1) + 65536, 0xFFFFF000ull), GPS_TILE);
+        b += std::max<uint64_t>(N, 1) * 16 + 3 * cap * 4 + 2 * (cap / GPS_TILE + 1) * 4 + 8ull * 4 * GPS_BIN_HIST +
+             2ull * 4 * (GPS_BIN_HIST + budget / GPS_BIN_JOB_POINTS + 1) + 32;
+    }
     if (!st->fully_disable_hierarchical_culling) {
         const uint64_t gpb = st->gaussians_per_block > 0 ? (uint64_t)st->gaussians_per_block
                                                          : (uint64_t)std::max(4.0f, std::floor((float)N / 1e6f));
@@ -467,6 +522,8 @@ uint64_t gps_pass_device_bytes(const gps_pass* p) {
                  4ull * 4 * p->ntiles + ((uint64_t)p->budget / GPS_SPLAT_CHUNK + 2) * 4 + sizeof(FrameCounters) +
                  (uint64_t)p->hiz_size * 4 + 2 * p->fb_cap * 8 + p->dbg_indices_cap * 4;
     if (p->hier_available) b += (uint64_t)p->cull.node_count * sizeof(CullNode) + (p->cull.mask_words + 2) * 4ull;
+    b += std::max<uint64_t>(N, 1) * 16 + 3ull * p->list_cap * 4 + 2ull * (p->list_cap / GPS_TILE + 1) * 4 +
+         8ull * 4 * GPS_BIN_HIST + 2ull * 4 * p->job_cap + 32;
     return b;
 }
 
@@ -491,6 +548,13 @@ void gps_pass_destroy(gps_pass* p) {
     dfree(p->hiz.mip);
     dfree(p->nodes);
     dfree(p->cull_mask);
+    dfree(p->erect);
+    dfree(p->ew);
+    dfree(p->big);
+    for (uint32_t** a : {&p->reg_cnt, &p->reg_pts, &p->reg_ecnt, &p->reg_ptot, &p->reg_eoff, &p->reg_poff,
+                         &p->reg_cursor, &p->reg_jobs, &p->job_reg, &p->job_p0, &p->list, &p->lw, &p->lpre, &p->tsum,
+                         &p->toff, &p->bin_state})
+        dfree(*a);
     delete p;
 }
 
@@ -644,7 +708,7 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         sa.chunk_start = p->chunk_start;
         sa.ctr = &p->ctr->pass[slot];
         sa.fb = p->cur_fb;
-        sa.atomics_count = &p->ctr->atomics;
+        sa.atomics_count = p->count_atomics ? &p->ctr->atomics : nullptr;
         sa.spp = f.spp;
         sa.K = f.Ki;
         sa.up_w = f.up_w;
@@ -653,6 +717,57 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         sa.grid = p->splat_grid;
         sa.xcds = p->xcds;
         stage_mark(p, 4, slot, 0, st);
+        BinGeom bg_geom;
+        // Binned when a point has >= 8 samples (K * spp): below that the
+        // per-point atomics are few and the binning does not pay for itself.
+        const bool binned = p->splat_mode != GPS_SPLAT_POINT_PARALLEL &&
+                            (p->splat_mode == GPS_SPLAT_BINNED || f.Ki * f.spp >= 8) &&
+                            binned_geometry(f.up_w, f.up_h, f.spp, &bg_geom);
+        p->last_binned = binned ? 1 : 0;
+        if (binned) {
+            BinnedArgs ba{};
+            ba.recs = p->recs;
+            ba.cidx = p->cidx;
+            ba.coff = p->coff;
+            ba.ctr = &p->ctr->pass[slot];
+            ba.g = bg_geom;
+            ba.sc.spp = f.spp;
+            ba.sc.K = f.Ki;
+            ba.sc.up_w = f.up_w;
+            ba.sc.unbiased = f.unbiased;
+            ba.sc.frame_seed = frame_seed + 1u;
+            ba.sc.xcds = p->xcds;
+            ba.background = bg;
+            ba.fb = p->cur_fb;
+            ba.erect = p->erect;
+            ba.ew = p->ew;
+            ba.reg_cnt = p->reg_cnt;
+            ba.reg_pts = p->reg_pts;
+            ba.reg_ecnt = p->reg_ecnt;
+            ba.reg_ptot = p->reg_ptot;
+            ba.reg_eoff = p->reg_eoff;
+            ba.reg_poff = p->reg_poff;
+            ba.reg_cursor = p->reg_cursor;
+            ba.reg_jobs = p->reg_jobs;
+            ba.job_reg = p->job_reg;
+            ba.job_p0 = p->job_p0;
+            ba.list = p->list;
+            ba.lw = p->lw;
+            ba.lpre = p->lpre;
+            ba.tsum = p->tsum;
+            ba.toff = p->toff;
+            ba.big = p->big;
+            ba.state = p->bin_state;
+            ba.list_cap = p->list_cap;
+            ba.job_cap = p->job_cap;
+            ba.job_points = GPS_BIN_JOB_POINTS;
+            ba.cus = p->cus;
+            ba.atomics_count = p->count_atomics ? &p->ctr->atomics : nullptr;
+            launch_binned_splat(ba, st);
+            sa.only_if = p->bin_state + 3;  // overflow: the point-parallel splat takes the pass
+        } else {
+            sa.only_if = nullptr;
+        }
         launch_splat(sa, st);
         stage_mark(p, 4, slot, 1, st);
     };
@@ -740,6 +855,15 @@ gps_status gps_pass_stats(gps_pass* p, gps_stats* out, void* stream) {
         if (c.pass[i].total > c.pass[i].points) out->budget_truncated = 1;
     }
     out->atomics = c.atomics;
+    if (p->last_binned) {
+        uint32_t bs[8];
+        GPS_HIP(hipMemcpy(bs, p->bin_state, sizeof(bs), hipMemcpyDeviceToHost));
+        out->splat_binned = 1;
+        out->splat_fallback = bs[3] ? 1 : 0;
+        out->bin_entries = bs[0];
+        out->bin_jobs = bs[1];
+        out->bin_big = bs[2];
+    }
     if (p->hier_available) {
         out->gaussians_per_block = (int32_t)p->cull.gpb;
         out->num_blocks = p->cull.node_count;

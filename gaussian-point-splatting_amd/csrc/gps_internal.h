@@ -120,10 +120,46 @@ struct BatchView {
 #define GPS_SPLAT_THREADS 256
 #define GPS_SPLAT_PPT 4
 #define GPS_SPLAT_CHUNK (GPS_SPLAT_THREADS * GPS_SPLAT_PPT)
+// Binned splat: one 1024-thread workgroup renders a screen region of at most
+// GPS_BIN_WORDS framebuffer words in LDS; GPS_BIN_PPT points per thread per
+// step; Gaussians overlapping more regions than GPS_BIN_MAX_REGIONS take the
+// global-atomic path; at most GPS_BIN_HIST regions per frame.
+#define GPS_TILE_WORDS 4096u
+#define GPS_BIN_THREADS 1024
+#define GPS_BIN_PPT 4
+#define GPS_BIN_CHUNK (GPS_BIN_THREADS * GPS_BIN_PPT)
+#define GPS_BIN_WORDS 16384u
+#define GPS_BIN_MAX_REGIONS 4u
+#define GPS_BIN_HIST 8192u
+#define GPS_BIN_JOB_POINTS 16384u
+struct BinGeom {
+    int32_t rw_shift, rh_shift;  // log2 of the region width / height in pixels
+    int32_t nrx, nry;            // regions per row / column
+    int32_t up_w, up_h, spp;
+    int32_t tiles_per_row;       // framebuffer 64x64 tiles per row
+    int32_t rtx, rty;            // framebuffer tiles per region (x, y)
+    uint32_t nregions;
+};
+
 // Distributor tile: 256 threads x 16 Gaussians.
 #define GPS_TILE_SHIFT 12
 #define GPS_TILE (1u << GPS_TILE_SHIFT)
 #define GPS_PRE_THREADS 256
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Byte `sel` (a constant 0..3) of `word` as an OCP FP8 E4M3 value, converted
+// by the hardware (v_cvt_f32_fp8; exact for every code).
+__device__ __forceinline__ float fp8_e4m3_to_float_hw(uint32_t word, int sel) {
+    switch (sel) {
+        case 0: return __builtin_amdgcn_cvt_f32_fp8((int)word, 0);
+        case 1: return __builtin_amdgcn_cvt_f32_fp8((int)word, 1);
+        case 2: return __builtin_amdgcn_cvt_f32_fp8((int)word, 2);
+        default: return __builtin_amdgcn_cvt_f32_fp8((int)word, 3);
+    }
+}
+#elif defined(__HIP__)
+__device__ __forceinline__ float fp8_e4m3_to_float_hw(uint32_t, int) { return 0.0f; }
+#endif
 
 // Frame constants of gps_pass_render for a frame with `spp` samples per pixel
 // (gaussian_point_splatting.cu:605-615).

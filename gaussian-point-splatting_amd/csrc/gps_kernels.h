@@ -49,7 +49,39 @@ struct SplatArgs {
     uint32_t frame_seed;
     uint32_t grid;   // multiple of xcds
     int32_t xcds;    // accelerator complex dies (workgroups round-robin over them)
+    const uint32_t* only_if;  // when set, the splat runs only if *only_if != 0
 };
+
+// Splat constants shared by the point-parallel and the binned splat.
+struct SplatConsts {
+    int32_t spp, K, up_w, unbiased;
+    uint32_t frame_seed;
+    int32_t xcds;
+};
+
+struct BinnedArgs {
+    const SplatRecord* recs;
+    const uint32_t* cidx;
+    const uint32_t* coff;
+    const PassCounters* ctr;
+    BinGeom g;
+    SplatConsts sc;
+    uint64_t background;
+    uint64_t* fb;
+    uint2* erect;
+    uint32_t* ew;
+    uint32_t *reg_cnt, *reg_pts, *reg_ecnt, *reg_ptot, *reg_eoff, *reg_poff, *reg_cursor, *reg_jobs;
+    uint32_t *job_reg, *job_p0;
+    uint32_t *list, *lw, *lpre;
+    uint32_t *tsum, *toff;
+    uint32_t* big;
+    uint32_t* state;      // 8 words: list length, jobs, big count, overflow, job cursor, big cursor
+    uint32_t list_cap, job_cap, job_points;
+    uint32_t cus;
+    uint32_t* atomics_count;
+};
+bool binned_geometry(int up_w, int up_h, int spp, BinGeom* g);
+void launch_binned_splat(const BinnedArgs& a, hipStream_t st);
 
 // Multi-view batch (see the batch kernels): all arrays are concatenated per view.
 struct BatchArgs {

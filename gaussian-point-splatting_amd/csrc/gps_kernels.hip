@@ -7,7 +7,8 @@
 //   distribute_kernel   -> compacted (id, offset) lists + per-chunk start
 //   splat_kernel        point sampling + 64-bit atomicMin (gaussian_point_splatting.cu:67-141,383-432)
 //   combine_kernel      FB -> colour / NDC depth       (gaussian_point_splatting.cu:485-548)
-//   hiz_build_kernel    single-launch Hi-Z pyramid     (occlusion/depth_mip_chain.cu:107-145)
+//   hiz_build_kernel +  Hi-Z pyramid in two launches   (occlusion/depth_mip_chain.cu:107-145)
+//   hiz_top_kernel
 //   taa / progressive   resolve                        (passes/post_processing_kernels.cuh:108-285)
 //
 // Wave64: every ballot yields two 32-bit mask words so the reference's
@@ -34,6 +35,18 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v) {
         if (lane >= d) v += t;
     }
     return v;
+}
+
+// Adds a per-thread count to *counter with one atomic per workgroup (the
+// counter is a single word: per-wave atomics on it serialize).
+__device__ __forceinline__ void add_count(uint32_t* counter, uint32_t v) {
+    __shared__ uint32_t s_tot;
+    if (threadIdx.x == 0) s_tot = 0u;
+    __syncthreads();
+    const uint32_t w = wave_reduce_add(v);
+    if ((threadIdx.x & 63) == 0 && w) atomicAdd(&s_tot, w);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_tot) atomicAdd(counter, s_tot);
 }
 
 __device__ __forceinline__ bool mask_bit(const uint32_t* m, uint32_t i) { return ((m[i >> 5] >> (i & 31u)) & 1u) != 0u; }
@@ -86,16 +99,6 @@ __device__ __forceinline__ float plane(const DevScene& s, int p, uint32_t g) {
 __device__ __forceinline__ float half_bits_to_float(uint32_t h16) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)h16);
 }
-// OCP FP8 E4M3 -> float (exact).
-__device__ __forceinline__ float fp8_to_float(uint32_t q) {
-    const uint32_t s = (q & 0x80u) << 24;
-    const uint32_t em = q & 0x7Fu;
-    if (em == 0x7Fu) return __builtin_bit_cast(float, 0x7FC00000u);
-    if (em >= 8u) return __builtin_bit_cast(float, s | ((em << 20) + (120u << 23)));
-    const float v = (float)em * 0x1.0p-9f;
-    return s ? -v : v;
-}
-
 struct Geo {
     float mx, my, mz, qw, qx, qy, qz, sx, sy, sz, op;
 };
@@ -168,7 +171,7 @@ template <int DEG> struct ShEval<GPS_FORMAT_HALF_FP8, DEG> {
                 if (i == 0) return dc[ch];
                 const int b = 3 * (i - 1) + ch;           // byte index into the rest block
                 const uint32_t w = words[3 + (b >> 2)];    // rest bytes start at byte 12
-                return fp8_to_float((w >> (8 * (b & 3))) & 0xFFu) * scale;
+                return fp8_e4m3_to_float_hw(w, b & 3) * scale;
             });
         }
     }
@@ -257,8 +260,8 @@ template <int FORMAT, int DEG>
 __device__ __forceinline__ uint32_t preprocess_one(const DevScene& s, const FrameConsts& f, uint32_t g,
                                                    const uint32_t* __restrict__ mask_prev, const HiZView& h,
                                                    SplatRecord* __restrict__ rec_out) {
-    u32x2 seed = make_seed(g, f.frame_seed);
     if (f.only_prev && mask_bit(mask_prev, g)) return 0u;
+    u32x2 seed = make_seed(g, f.frame_seed);
     const Geo ge = load_geo<FORMAT>(s, g);
     const float op = glm_min(ge.op, 1.0f);
     if (op <= 0.0f) return 0u;
@@ -272,10 +275,12 @@ __device__ __forceinline__ uint32_t preprocess_one(const DevScene& s, const Fram
         const float e1 = t2 + term2, e2 = t2 - term2;
         if (e1 < 0.02f || e2 < 0.02f) return 0u;
     }
+    // (The Hi-Z test is a pure rejection, so it is taken before the
+    // importance terms it does not depend on.)
+    if (f.occl && hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss)) return 0u;
     const float s_det = det_sqrt(pr.det);
     const float dl = dilog(op);
     const float importance = f.unbiased ? GPS_TWO_PI_F * s_det * dl : op * GPS_TWO_PI_F * s_det;
-    if (f.occl && hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss)) return 0u;
     uint32_t cnt = f.unbiased ? poisson(seed, importance) : (uint32_t)stochastic_round(importance, pcg2d(seed).x);
     if (f.K > 1.0f) cnt = (uint32_t)stochastic_round((float)cnt / f.K, pcg2d(seed).x);
     if (cnt == 0u) return 0u;
@@ -352,6 +357,7 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
                                                          uint32_t ntiles, uint32_t* __restrict__ tile_off,
                                                          uint32_t* __restrict__ tile_voff, PassCounters* __restrict__ ctr,
                                                          uint32_t budget, uint32_t* __restrict__ iteration) {
+    constexpr int PER = 8;  // tiles per thread per step
     __shared__ uint32_t s_a[16], s_c[16];
     __shared__ uint32_t s_carry_a, s_carry_c;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -360,12 +366,19 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
         s_carry_c = 0;
     }
     __syncthreads();
-    for (uint32_t base = 0; base < ntiles; base += 1024) {
-        const uint32_t i = base + tid;
-        const uint32_t a = i < ntiles ? tile_sum[i] : 0u;
-        const uint32_t c = i < ntiles ? tile_cnt[i] : 0u;
-        const uint32_t ia = wave_scan_incl(a);
-        const uint32_t ic = wave_scan_incl(c);
+    for (uint32_t base = 0; base < ntiles; base += 1024 * PER) {
+        const uint32_t i0 = base + (uint32_t)tid * PER;
+        uint32_t a[PER], c[PER], sa = 0, sc = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = i0 + (uint32_t)k;
+            a[k] = i < ntiles ? tile_sum[i] : 0u;
+            c[k] = i < ntiles ? tile_cnt[i] : 0u;
+            sa += a[k];
+            sc += c[k];
+        }
+        const uint32_t ia = wave_scan_incl(sa);
+        const uint32_t ic = wave_scan_incl(sc);
         if (lane == 63) {
             s_a[wave] = ia;
             s_c[wave] = ic;
@@ -377,11 +390,18 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
             wc += s_c[w];
         }
         const uint32_t ca = s_carry_a, cc = s_carry_c;
-        if (i < ntiles) {
-            tile_off[i] = ca + wa + ia - a;
-            tile_voff[i] = cc + wc + ic - c;
-            tile_sum[i] = 0u;
-            tile_cnt[i] = 0u;
+        uint32_t oa = ca + wa + ia - sa, oc = cc + wc + ic - sc;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = i0 + (uint32_t)k;
+            if (i < ntiles) {
+                tile_off[i] = oa;
+                tile_voff[i] = oc;
+                tile_sum[i] = 0u;
+                tile_cnt[i] = 0u;
+            }
+            oa += a[k];
+            oc += c[k];
         }
         __syncthreads();
         if (tid == 1023) {
@@ -560,11 +580,6 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
     return issued;
 }
 
-struct SplatConsts {
-    int32_t spp, K, up_w, unbiased;
-    uint32_t frame_seed;
-    int32_t xcds;
-};
 
 __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRecord* __restrict__ recs,
                                                                   const uint32_t* __restrict__ cidx,
@@ -572,8 +587,10 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
                                                                   const uint32_t* __restrict__ chunk_start,
                                                                   const PassCounters* __restrict__ ctr,
                                                                   unsigned long long* __restrict__ fb, SplatConsts sc,
-                                                                  uint32_t* __restrict__ atomics_count) {
+                                                                  uint32_t* __restrict__ atomics_count,
+                                                                  const uint32_t* __restrict__ only_if) {
     __shared__ uint32_t s_off[GPS_SPLAT_CHUNK + 1];
+    if (only_if && *only_if == 0u) return;  // the binned splat covers this pass
     const uint32_t P = ctr->points;
     const uint32_t V = ctr->valid;
     const uint32_t nchunks = ctr->chunks;
@@ -621,10 +638,485 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
         }
         __syncthreads();
     }
-    if (atomics_count) {
-        const uint32_t tot = wave_reduce_add(issued);
-        if ((tid & 63) == 0 && tot) atomicAdd(atomics_count, tot);
+    if (atomics_count) add_count(atomics_count, issued);
+}
+
+// ------------------------------------------------------------------ binned splat
+// Screen regions of at most GPS_BIN_WORDS framebuffer words (128x128 px at
+// 1 spp, 128x64 at 2, 64x64 at 3-4) are rendered by one workgroup in LDS.
+// The valid Gaussians are binned to the regions their bounds overlap; a
+// region job replays the points of its Gaussians with their global point
+// index (hence exactly the samples of the point-parallel splat) and keeps
+// the samples that land inside the region, with LDS atomicMin.  A region
+// rendered by one job starts from the framebuffer and stores the region
+// back; a heavy region split into several jobs merges its changed words with
+// global atomicMin.  Gaussians overlapping more than GPS_BIN_MAX_REGIONS
+// regions are splatted afterwards with global atomics (no replay).  Since
+// min is order-independent the framebuffer is bit-identical to the
+// point-parallel splat.  If the region lists overflow their allocation the
+// binned kernels stand down and the point-parallel splat runs instead.
+
+__device__ __forceinline__ uint32_t bin_region_words(const BinGeom& g) {
+    return (uint32_t)(g.rtx * g.rty) * GPS_TILE_WORDS * (uint32_t)g.spp;
+}
+
+// Per entry: points (clipped to the pass budget), region rectangle, region
+// histogram in LDS (one contiguous entry range per workgroup, so the
+// Morton-ordered entries touch few regions), big entries to their list.
+__global__ __launch_bounds__(256) void bin_count_kernel(const SplatRecord* __restrict__ recs,
+                                                        const uint32_t* __restrict__ cidx,
+                                                        const uint32_t* __restrict__ coff,
+                                                        const PassCounters* __restrict__ ctr, BinGeom g,
+                                                        uint2* __restrict__ erect, uint32_t* __restrict__ ew,
+                                                        uint32_t* __restrict__ reg_cnt, uint32_t* __restrict__ reg_pts,
+                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ state) {
+    __shared__ uint32_t s_cnt[GPS_BIN_HIST], s_pts[GPS_BIN_HIST];
+    const int tid = threadIdx.x;
+    for (uint32_t i = tid; i < g.nregions; i += 256) {
+        s_cnt[i] = 0u;
+        s_pts[i] = 0u;
     }
+    __syncthreads();
+    const uint32_t V = ctr->valid, P = ctr->points, T = ctr->total;
+    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = blockIdx.x * per, r1 = min(V, r0 + per);
+    for (uint32_t r = r0 + tid; r < r1; r += 256) {
+        const uint32_t lo = coff[r];
+        const uint32_t hi = min(r + 1 < V ? coff[r + 1] : T, P);
+        uint32_t w = lo < hi ? hi - lo : 0u;
+        uint2 rect = make_uint2(0xFFFFFFFFu, 0u);
+        if (w) {
+            const uint32_t* rp = reinterpret_cast<const uint32_t*>(recs + cidx[r]);
+            const uint32_t bmin = rp[11], bmax = rp[12];
+            const uint32_t bx0 = bmin & 0xFFFFu, by0 = bmin >> 16, bx1 = bmax & 0xFFFFu, by1 = bmax >> 16;
+            if (bx1 > bx0 && by1 > by0) {
+                const uint32_t rx0 = bx0 >> g.rw_shift, rx1 = (bx1 - 1u) >> g.rw_shift;
+                const uint32_t ry0 = by0 >> g.rh_shift, ry1 = (by1 - 1u) >> g.rh_shift;
+                if ((rx1 - rx0 + 1u) * (ry1 - ry0 + 1u) > GPS_BIN_MAX_REGIONS) {
+                    big[atomicAdd(&state[2], 1u)] = r;
+                } else {
+                    rect = make_uint2(rx0 | (rx1 << 16), ry0 | (ry1 << 16));
+                    for (uint32_t ry = ry0; ry <= ry1; ++ry)
+                        for (uint32_t rx = rx0; rx <= rx1; ++rx) {
+                            const uint32_t reg = ry * (uint32_t)g.nrx + rx;
+                            atomicAdd(&s_cnt[reg], 1u);
+                            atomicAdd(&s_pts[reg], w);
+                        }
+                }
+            } else {
+                w = 0u;  // empty bounds: every sample would be rejected
+            }
+        }
+        erect[r] = rect;
+        ew[r] = w;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < g.nregions; i += 256)
+        if (s_cnt[i]) {
+            atomicAdd(&reg_cnt[i], s_cnt[i]);
+            atomicAdd(&reg_pts[i], s_pts[i]);
+        }
+}
+
+// One workgroup: region offsets, jobs of at most `job_points` points per
+// region, the job table, and the list length (overflow flag).  Clears the
+// region counters for the next pass.
+__global__ __launch_bounds__(1024) void bin_scan_kernel(BinGeom g, uint32_t* __restrict__ reg_cnt,
+                                                        uint32_t* __restrict__ reg_pts,
+                                                        uint32_t* __restrict__ reg_ecnt,
+                                                        uint32_t* __restrict__ reg_ptot,
+                                                        uint32_t* __restrict__ reg_eoff,
+                                                        uint32_t* __restrict__ reg_poff,
+                                                        uint32_t* __restrict__ reg_cursor,
+                                                        uint32_t* __restrict__ reg_jobs,
+                                                        uint32_t* __restrict__ job_reg, uint32_t* __restrict__ job_p0,
+                                                        uint32_t* __restrict__ state, uint32_t list_cap,
+                                                        uint32_t job_points, uint32_t job_cap) {
+    __shared__ uint32_t s_a[16], s_b[16], s_c[16];
+    __shared__ uint32_t s_ca, s_cb, s_cc;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        s_ca = 0u;
+        s_cb = 0u;
+        s_cc = 0u;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < g.nregions; base += 1024) {
+        const uint32_t i = base + tid;
+        const uint32_t cnt = i < g.nregions ? reg_cnt[i] : 0u;
+        const uint32_t pts = i < g.nregions ? reg_pts[i] : 0u;
+        const uint32_t nj = pts ? (pts + job_points - 1u) / job_points : 0u;
+        const uint32_t ia = wave_scan_incl(cnt), ib = wave_scan_incl(pts), ic = wave_scan_incl(nj);
+        if (lane == 63) {
+            s_a[wave] = ia;
+            s_b[wave] = ib;
+            s_c[wave] = ic;
+        }
+        __syncthreads();
+        uint32_t wa = 0, wb = 0, wc = 0;
+        for (int w = 0; w < wave; ++w) {
+            wa += s_a[w];
+            wb += s_b[w];
+            wc += s_c[w];
+        }
+        const uint32_t ca = s_ca, cb = s_cb, cc = s_cc;
+        if (i < g.nregions) {
+            const uint32_t eoff = ca + wa + ia - cnt, joff = cc + wc + ic - nj;
+            reg_ecnt[i] = cnt;
+            reg_ptot[i] = pts;
+            reg_eoff[i] = eoff;
+            reg_poff[i] = cb + wb + ib - pts;
+            reg_cursor[i] = eoff;
+            reg_jobs[i] = nj;
+            for (uint32_t j = 0; j < nj && joff + j < job_cap; ++j) {
+                job_reg[joff + j] = i;
+                job_p0[joff + j] = j * job_points;
+            }
+            reg_cnt[i] = 0u;
+            reg_pts[i] = 0u;
+        }
+        __syncthreads();
+        if (tid == 1023) {
+            s_ca = ca + wa + ia;
+            s_cb = cb + wb + ib;
+            s_cc = cc + wc + ic;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        state[0] = s_ca;
+        state[1] = min(s_cc, job_cap);
+        state[3] = (s_ca > list_cap || s_cc > job_cap) ? 1u : 0u;
+        state[4] = 0u;
+        state[5] = 0u;
+    }
+}
+
+// Region lists: the same entry ranges as bin_count_kernel; the workgroup
+// counts its entries per region in LDS, reserves one range per region with a
+// single global atomic, and fills it.
+__global__ __launch_bounds__(256) void bin_scatter_kernel(const PassCounters* __restrict__ ctr, BinGeom g,
+                                                          const uint2* __restrict__ erect,
+                                                          const uint32_t* __restrict__ ew,
+                                                          uint32_t* __restrict__ reg_cursor,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ lw,
+                                                          const uint32_t* __restrict__ state) {
+    __shared__ uint32_t s_cnt[GPS_BIN_HIST], s_base[GPS_BIN_HIST];
+    if (state[3]) return;
+    const int tid = threadIdx.x;
+    for (uint32_t i = tid; i < g.nregions; i += 256) s_cnt[i] = 0u;
+    __syncthreads();
+    const uint32_t V = ctr->valid;
+    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = blockIdx.x * per, r1 = min(V, r0 + per);
+    for (uint32_t r = r0 + tid; r < r1; r += 256) {
+        const uint2 rect = erect[r];
+        if (rect.x == 0xFFFFFFFFu) continue;
+        for (uint32_t ry = rect.y & 0xFFFFu; ry <= (rect.y >> 16); ++ry)
+            for (uint32_t rx = rect.x & 0xFFFFu; rx <= (rect.x >> 16); ++rx)
+                atomicAdd(&s_cnt[ry * (uint32_t)g.nrx + rx], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < g.nregions; i += 256) {
+        if (s_cnt[i]) s_base[i] = atomicAdd(&reg_cursor[i], s_cnt[i]);
+        s_cnt[i] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t r = r0 + tid; r < r1; r += 256) {
+        const uint2 rect = erect[r];
+        if (rect.x == 0xFFFFFFFFu) continue;
+        const uint32_t w = ew[r];
+        for (uint32_t ry = rect.y & 0xFFFFu; ry <= (rect.y >> 16); ++ry)
+            for (uint32_t rx = rect.x & 0xFFFFu; rx <= (rect.x >> 16); ++rx) {
+                const uint32_t reg = ry * (uint32_t)g.nrx + rx;
+                const uint32_t slot = s_base[reg] + atomicAdd(&s_cnt[reg], 1u);
+                list[slot] = r;
+                lw[slot] = w;
+            }
+    }
+}
+
+// Exclusive scan of lw over the list (region-major) in tiles of 4096 slots:
+// per-tile sums, a one-workgroup scan of the tile sums, then the local scan.
+// The list length comes from the device (grid-stride over its tiles).
+__global__ __launch_bounds__(256) void bin_tile_sum_kernel(const uint32_t* __restrict__ lw,
+                                                           const uint32_t* __restrict__ state,
+                                                           uint32_t* __restrict__ tsum) {
+    __shared__ uint32_t s[4];
+    const uint32_t L = state[3] ? 0u : state[0];
+    const uint32_t nt = (L + GPS_TILE - 1) / GPS_TILE;
+    for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const uint32_t t0 = tile * GPS_TILE + threadIdx.x * 16u;
+        uint32_t sum = 0;
+        if (t0 < L) {
+            const uint4* p = reinterpret_cast<const uint4*>(lw + t0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint4 v = p[k];
+                const uint32_t b = t0 + 4u * k;
+                sum += (b < L ? v.x : 0u) + (b + 1 < L ? v.y : 0u) + (b + 2 < L ? v.z : 0u) + (b + 3 < L ? v.w : 0u);
+            }
+        }
+        sum = wave_reduce_add(sum);
+        if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) tsum[tile] = (s[0] + s[1]) + (s[2] + s[3]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(1024) void bin_tiles_scan_kernel(const uint32_t* __restrict__ tsum,
+                                                              const uint32_t* __restrict__ state,
+                                                              uint32_t* __restrict__ toff) {
+    __shared__ uint32_t s_a[16];
+    __shared__ uint32_t s_carry;
+    const uint32_t L = state[3] ? 0u : state[0];
+    const uint32_t nt = (L + GPS_TILE - 1) / GPS_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_carry = 0u;
+    __syncthreads();
+    for (uint32_t base = 0; base < nt; base += 1024) {
+        const uint32_t i = base + tid;
+        const uint32_t a = i < nt ? tsum[i] : 0u;
+        const uint32_t ia = wave_scan_incl(a);
+        if (lane == 63) s_a[wave] = ia;
+        __syncthreads();
+        uint32_t wa = 0;
+        for (int w = 0; w < wave; ++w) wa += s_a[w];
+        const uint32_t c = s_carry;
+        if (i < nt) toff[i] = c + wa + ia - a;
+        __syncthreads();
+        if (tid == 1023) s_carry = c + wa + ia;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void bin_local_scan_kernel(const uint32_t* __restrict__ lw,
+                                                             const uint32_t* __restrict__ state,
+                                                             const uint32_t* __restrict__ toff,
+                                                             uint32_t* __restrict__ lpre) {
+    __shared__ uint32_t s_a[4];
+    const uint32_t L = state[3] ? 0u : state[0];
+    const uint32_t nt = (L + GPS_TILE - 1) / GPS_TILE;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const uint32_t b0 = tile * GPS_TILE + (uint32_t)tid * 16u;
+        uint32_t w[16];
+        const uint4* p = reinterpret_cast<const uint4*>(lw + b0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 v = b0 < L ? p[k] : make_uint4(0u, 0u, 0u, 0u);
+            w[4 * k] = b0 + 4u * k < L ? v.x : 0u;
+            w[4 * k + 1] = b0 + 4u * k + 1u < L ? v.y : 0u;
+            w[4 * k + 2] = b0 + 4u * k + 2u < L ? v.z : 0u;
+            w[4 * k + 3] = b0 + 4u * k + 3u < L ? v.w : 0u;
+        }
+        uint32_t sa = 0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sa += w[k];
+        const uint32_t ia = wave_scan_incl(sa);
+        if (lane == 63) s_a[wave] = ia;
+        __syncthreads();
+        uint32_t wa = 0;
+        for (int q = 0; q < wave; ++q) wa += s_a[q];
+        uint32_t off = toff[tile] + wa + ia - sa;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (b0 + (uint32_t)k < L) lpre[b0 + k] = off;
+            off += w[k];
+        }
+        __syncthreads();
+    }
+}
+
+struct BinSplat {
+    BinGeom g;
+    int32_t K, unbiased;
+    uint32_t frame_seed;
+    uint64_t background;
+};
+
+// Samples of point p of one Gaussian that land in the LDS region.
+__device__ __forceinline__ void splat_point_region(const SplatRecord& rec, uint32_t p, const BinSplat& b, int rx0,
+                                                   int ry0, unsigned long long* s_fb) {
+    const int bminx = (int)(rec.bmin & 0xFFFFu), bminy = (int)(rec.bmin >> 16);
+    const int bmaxx = (int)(rec.bmax & 0xFFFFu), bmaxy = (int)(rec.bmax >> 16);
+    const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
+    const unsigned long long depth_bits = packed & ~(unsigned long long)GPS_COLOR_MASK;
+    const int rw = 1 << b.g.rw_shift, rh = 1 << b.g.rh_shift;
+    const int K = b.K, spp = b.g.spp, samples = spp * K;
+    u32x2 seed = make_seed(p, b.frame_seed);
+    for (int layer = 0, k = 0, q = 0; q < samples; ++q) {
+        if (k == K) {
+            k = 0;
+            ++layer;
+        }
+        ++k;
+        const f32x2 u = pcg2d(seed);
+        const f32x2 xy = b.unbiased ? corrected_box_muller_pre(u.x, u.y, rec.inv_alpha, rec.dilog_alpha)
+                                    : box_muller(u.x, u.y);
+        const float fx = det_floor(rec.pmx + rec.c0 * xy.x);
+        const float fy = det_floor((rec.pmy + rec.c1 * xy.x) + rec.c2 * xy.y);
+        const int x = det_f2i_rz(fx);
+        const int y = det_f2i_rz(fy);
+        if (x < bminx || x >= bmaxx || y < bminy || y >= bmaxy) continue;
+        const int lx = x - rx0, ly = y - ry0;
+        if ((unsigned)lx >= (unsigned)rw || (unsigned)ly >= (unsigned)rh) continue;  // another region's sample
+        const float dx = rec.pmx - fx, dy = rec.pmy - fy;
+        const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
+        const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
+        const float alpha = rec.opacity * gw;
+        if (alpha < 1.0f / 255.0f) continue;
+        const uint32_t t = (uint32_t)((ly >> GPS_TILE_BITS) * b.g.rtx + (lx >> GPS_TILE_BITS));
+        const uint32_t i = (t * GPS_TILE_WORDS + morton_encode((uint32_t)(x & (GPS_TILE_SIZE - 1)),
+                                                                (uint32_t)(y & (GPS_TILE_SIZE - 1)))) *
+                               (uint32_t)spp +
+                           (uint32_t)layer;
+        if (s_fb[i] <= depth_bits) continue;
+        atomicMin(s_fb + i, packed);
+    }
+}
+
+// Persistent, one 1024-thread workgroup per CU (the region buffer is most of
+// the LDS); jobs are taken from a device counter.
+__global__ __launch_bounds__(GPS_BIN_THREADS) void bin_splat_kernel(
+    const SplatRecord* __restrict__ recs, const uint32_t* __restrict__ cidx, const uint32_t* __restrict__ coff,
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ lpre, const uint32_t* __restrict__ reg_ecnt,
+    const uint32_t* __restrict__ reg_ptot, const uint32_t* __restrict__ reg_eoff, const uint32_t* __restrict__ reg_poff,
+    const uint32_t* __restrict__ reg_jobs, const uint32_t* __restrict__ job_reg, const uint32_t* __restrict__ job_p0,
+    uint32_t* __restrict__ state, unsigned long long* __restrict__ fb, BinSplat b, uint32_t job_points,
+    uint32_t* __restrict__ atomics_count) {
+    __shared__ unsigned long long s_fb[GPS_BIN_WORDS];
+    __shared__ uint32_t s_off[GPS_BIN_CHUNK + 1];
+    __shared__ uint32_t s_job, s_slot0;
+    if (state[3]) return;
+    const int tid = threadIdx.x;
+    const uint32_t njobs = state[1];
+    const uint32_t words = bin_region_words(b.g);
+    const uint32_t tile_words = GPS_TILE_WORDS * (uint32_t)b.g.spp;
+    const int tiles_rows = (b.g.up_h + GPS_TILE_SIZE - 1) >> GPS_TILE_BITS;
+    uint32_t merged = 0;
+    for (;;) {
+        if (tid == 0) s_job = atomicAdd(&state[4], 1u);
+        __syncthreads();
+        const uint32_t job = s_job;
+        if (job >= njobs) break;
+        const uint32_t reg = job_reg[job];
+        const uint32_t ptot = reg_ptot[reg];
+        const uint32_t p_lo = job_p0[job];
+        const uint32_t p_hi = min(ptot, p_lo + job_points);
+        const bool single = reg_jobs[reg] == 1u;
+        const int rxi = (int)(reg % (uint32_t)b.g.nrx), ryi = (int)(reg / (uint32_t)b.g.nrx);
+        const int rx0 = rxi << b.g.rw_shift, ry0 = ryi << b.g.rh_shift;
+        const int tx0 = rx0 >> GPS_TILE_BITS, ty0 = ry0 >> GPS_TILE_BITS;
+        // region words <- framebuffer (single job) or background (merged later)
+        for (uint32_t w = (uint32_t)tid * 2u; w < words; w += 2u * GPS_BIN_THREADS) {
+            const uint32_t t = w / tile_words, within = w - t * tile_words;
+            const int tx = tx0 + (int)(t % (uint32_t)b.g.rtx), ty = ty0 + (int)(t / (uint32_t)b.g.rtx);
+            uint4 v = make_uint4((uint32_t)b.background, (uint32_t)(b.background >> 32), (uint32_t)b.background,
+                                 (uint32_t)(b.background >> 32));
+            if (single && tx < b.g.tiles_per_row && ty < tiles_rows)
+                v = *reinterpret_cast<const uint4*>(fb + (uint64_t)(ty * b.g.tiles_per_row + tx) * tile_words + within);
+            s_fb[w] = ((unsigned long long)v.y << 32) | v.x;
+            s_fb[w + 1] = ((unsigned long long)v.w << 32) | v.z;
+        }
+        const uint32_t e0 = reg_eoff[reg], ecnt = reg_ecnt[reg], pbase = reg_poff[reg];
+        for (uint32_t q0 = p_lo; q0 < p_hi; q0 += GPS_BIN_CHUNK) {
+            // last slot whose region-local prefix is <= q0
+            if (tid == 0) {
+                uint32_t lo = 0, hi = ecnt - 1u;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1u) >> 1;
+                    if (lpre[e0 + mid] - pbase <= q0) lo = mid;
+                    else hi = mid - 1u;
+                }
+                s_slot0 = lo;
+            }
+            __syncthreads();
+            const uint32_t j0 = s_slot0;
+            for (int j = tid; j <= GPS_BIN_CHUNK; j += GPS_BIN_THREADS)
+                s_off[j] = j0 + (uint32_t)j < ecnt ? lpre[e0 + j0 + (uint32_t)j] - pbase : 0xFFFFFFFFu;
+            __syncthreads();
+            const uint32_t qa = q0 + (uint32_t)tid * GPS_BIN_PPT;
+            if (qa < p_hi) {
+                int lo = 0, hi = GPS_BIN_CHUNK;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= qa) lo = mid;
+                    else hi = mid - 1;
+                }
+                int j = lo;
+                uint32_t cur = 0xFFFFFFFFu, pstart = 0;
+                SplatRecord rec;
+                for (int k = 0; k < GPS_BIN_PPT; ++k) {
+                    const uint32_t q = qa + (uint32_t)k;
+                    if (q >= p_hi) break;
+                    while (j < GPS_BIN_CHUNK - 1 && s_off[j + 1] <= q) ++j;
+                    const uint32_t slot = j0 + (uint32_t)j;
+                    if (slot >= ecnt) break;
+                    if (slot != cur) {
+                        cur = slot;
+                        const uint32_t r = list[e0 + slot];
+                        pstart = coff[r];
+                        rec = load_record(recs, cidx[r]);
+                    }
+                    splat_point_region(rec, pstart + (q - s_off[j]), b, rx0, ry0, s_fb);
+                }
+            }
+            __syncthreads();
+        }
+        // region -> framebuffer
+        for (uint32_t w = (uint32_t)tid * 2u; w < words; w += 2u * GPS_BIN_THREADS) {
+            const uint32_t t = w / tile_words, within = w - t * tile_words;
+            const int tx = tx0 + (int)(t % (uint32_t)b.g.rtx), ty = ty0 + (int)(t / (uint32_t)b.g.rtx);
+            if (tx >= b.g.tiles_per_row || ty >= tiles_rows) continue;
+            unsigned long long* dst = fb + (uint64_t)(ty * b.g.tiles_per_row + tx) * tile_words + within;
+            const unsigned long long a0 = s_fb[w], a1 = s_fb[w + 1];
+            if (single) {
+                *reinterpret_cast<uint4*>(dst) = make_uint4((uint32_t)a0, (uint32_t)(a0 >> 32), (uint32_t)a1,
+                                                            (uint32_t)(a1 >> 32));
+            } else {
+                if (a0 != b.background) {
+                    atomicMin(dst, a0);
+                    ++merged;
+                }
+                if (a1 != b.background) {
+                    atomicMin(dst + 1, a1);
+                    ++merged;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (atomics_count) add_count(atomics_count, merged);
+}
+
+// Gaussians too large to bin: all their points with global atomics, after
+// the regions are written.
+__global__ __launch_bounds__(256) void bin_big_kernel(const SplatRecord* __restrict__ recs,
+                                                      const uint32_t* __restrict__ cidx,
+                                                      const uint32_t* __restrict__ coff,
+                                                      const uint32_t* __restrict__ ew, const uint32_t* __restrict__ big,
+                                                      uint32_t* __restrict__ state, unsigned long long* __restrict__ fb,
+                                                      SplatConsts sc, uint32_t* __restrict__ atomics_count) {
+    __shared__ uint32_t s_e;
+    if (state[3]) return;
+    const int tid = threadIdx.x;
+    const uint32_t nbig = state[2];
+    uint32_t issued = 0;
+    for (;;) {
+        if (tid == 0) s_e = atomicAdd(&state[5], 1u);
+        __syncthreads();
+        const uint32_t e = s_e;
+        __syncthreads();
+        if (e >= nbig) break;
+        const uint32_t r = big[e];
+        const SplatRecord rec = load_record(recs, cidx[r]);
+        const uint32_t p0 = coff[r], w = ew[r];
+        for (uint32_t k = (uint32_t)tid; k < w; k += 256u)
+            issued += splat_point(rec, p0 + k, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb);
+    }
+    if (atomics_count) add_count(atomics_count, issued);
 }
 
 // ------------------------------------------------------------------ combine
@@ -741,13 +1233,10 @@ __global__ __launch_bounds__(256) void combine_staged_kernel(const uint4* __rest
 }
 
 // ------------------------------------------------------------------ Hi-Z
-// Single launch: each workgroup reduces a 32x32 block of the finest level
-// through five levels in LDS; the last workgroup to finish (device-scope
-// ticket) reduces the remaining coarse levels.
-__global__ __launch_bounds__(256) void hiz_build_kernel(HiZView h, const float* __restrict__ depth, int kmin,
-                                                        uint32_t* __restrict__ ticket) {
+// Each workgroup reduces a 32x32 block of the finest level through five
+// levels in LDS; hiz_top_kernel then reduces the remaining coarse levels.
+__global__ __launch_bounds__(256) void hiz_build_kernel(HiZView h, const float* __restrict__ depth, int kmin) {
     __shared__ float s0[1024], s1[1024];
-    __shared__ int s_last;
     const int L = h.levels, fl = L - 1;
     const int tid = threadIdx.x;
     const int bx = blockIdx.x, by = blockIdx.y;
@@ -791,17 +1280,14 @@ __global__ __launch_bounds__(256) void hiz_build_kernel(HiZView h, const float* 
         cur = nxt;
         nxt = t;
     }
-    if (last_local == 0) return;
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const uint32_t prev = atomicAdd(ticket, 1u);
-        s_last = (prev == gridDim.x * gridDim.y - 1u) ? 1 : 0;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    for (int l = last_local - 1; l >= 0; --l) {
+}
+
+// Levels above the per-workgroup ones, in one workgroup (second launch; a
+// device-wide ticket on one word serialized the 2000 workgroups).
+__global__ __launch_bounds__(256) void hiz_top_kernel(HiZView h, int kmin, int top) {
+    const int L = h.levels;
+    const int tid = threadIdx.x;
+    for (int l = top; l >= 0; --l) {
         const bool use_min = ((L - 2) - l) < kmin;
         const int n = h.lw[l] * h.lh[l];
         for (int i = tid; i < n; i += 256) {
@@ -811,16 +1297,14 @@ __global__ __launch_bounds__(256) void hiz_build_kernel(HiZView h, const float* 
                 for (int dx = 0; dx < 2; ++dx) {
                     const int CX = 2 * X + dx, CY = 2 * Y + dy;
                     if (CX < h.lw[l + 1] && CY < h.lh[l + 1]) {
-                        const float c = __builtin_nontemporal_load(h.mip + h.off[l + 1] + (uint32_t)(CY * h.lw[l + 1] + CX));
+                        const float c = h.mip[h.off[l + 1] + (uint32_t)(CY * h.lw[l + 1] + CX)];
                         v = use_min ? det_fmin(v, c) : det_fmax(v, c);
                     }
                 }
             h.mip[h.off[l] + (uint32_t)i] = v;
         }
-        __threadfence_block();
         __syncthreads();
     }
-    if (tid == 0) *ticket = 0u;
 }
 
 __global__ __launch_bounds__(64) void zero_u32_kernel(uint32_t* __restrict__ p, uint32_t n) {
@@ -1151,10 +1635,7 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
         }
         __syncthreads();
     }
-    if (atomics_count) {
-        const uint32_t tot = wave_reduce_add(issued);
-        if ((tid & 63) == 0 && tot) atomicAdd(atomics_count, tot);
-    }
+    if (atomics_count) add_count(atomics_count, issued);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1216,7 +1697,8 @@ void launch_splat(const SplatArgs& a, hipStream_t st) {
     sc.frame_seed = a.frame_seed;
     sc.xcds = a.xcds;
     hipLaunchKernelGGL(splat_kernel, dim3(a.grid), dim3(GPS_SPLAT_THREADS), 0, st, a.recs, a.cidx, a.coff,
-                       a.chunk_start, a.ctr, reinterpret_cast<unsigned long long*>(a.fb), sc, a.atomics_count);
+                       a.chunk_start, a.ctr, reinterpret_cast<unsigned long long*>(a.fb), sc, a.atomics_count,
+                       a.only_if);
 }
 
 template <int SS, int SPP>
@@ -1246,7 +1728,10 @@ void launch_combine(const uint64_t* fb, float* color, float* depth, int W, int H
 
 void launch_hiz_build(const HiZView& h, const float* depth, int kmin, uint32_t* ticket, hipStream_t st) {
     dim3 grid((h.width + 31) / 32, (h.height + 31) / 32);
-    hipLaunchKernelGGL(hiz_build_kernel, grid, dim3(256), 0, st, h, depth, kmin, ticket);
+    (void)ticket;
+    hipLaunchKernelGGL(hiz_build_kernel, grid, dim3(256), 0, st, h, depth, kmin);
+    const int fl = h.levels - 1, last_local = fl - 5 > 0 ? fl - 5 : 0;
+    if (last_local > 0) hipLaunchKernelGGL(hiz_top_kernel, dim3(1), dim3(256), 0, st, h, kmin, last_local - 1);
 }
 
 void launch_fill_f32(float* p, uint32_t n, float v, hipStream_t st) {
@@ -1360,6 +1845,57 @@ bool launch_batch_splat(const BatchArgs& a, hipStream_t st) {
     else if (t == 512 && q == 4) launch_batch_splat_t<512, 4>(a, st);
     else return false;
     return true;
+}
+
+// ------------------------------------------------------------------ binned splat launcher
+void launch_binned_splat(const BinnedArgs& a, hipStream_t st) {
+    launch_zero_u32(a.state, 8, st);
+    const uint32_t grid = a.cus * 4u;
+    hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), 0, st, a.recs, a.cidx, a.coff, a.ctr, a.g, a.erect,
+                       a.ew, a.reg_cnt, a.reg_pts, a.big, a.state);
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, a.g, a.reg_cnt, a.reg_pts, a.reg_ecnt, a.reg_ptot,
+                       a.reg_eoff, a.reg_poff, a.reg_cursor, a.reg_jobs, a.job_reg, a.job_p0, a.state, a.list_cap,
+                       a.job_points, a.job_cap);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), 0, st, a.ctr, a.g, (const uint2*)a.erect,
+                       (const uint32_t*)a.ew, a.reg_cursor, a.list, a.lw, (const uint32_t*)a.state);
+    hipLaunchKernelGGL(bin_tile_sum_kernel, dim3(grid), dim3(256), 0, st, (const uint32_t*)a.lw,
+                       (const uint32_t*)a.state, a.tsum);
+    hipLaunchKernelGGL(bin_tiles_scan_kernel, dim3(1), dim3(1024), 0, st, (const uint32_t*)a.tsum,
+                       (const uint32_t*)a.state, a.toff);
+    hipLaunchKernelGGL(bin_local_scan_kernel, dim3(grid), dim3(256), 0, st, (const uint32_t*)a.lw,
+                       (const uint32_t*)a.state, (const uint32_t*)a.toff, a.lpre);
+    BinSplat b;
+    b.g = a.g;
+    b.K = a.sc.K;
+    b.unbiased = a.sc.unbiased;
+    b.frame_seed = a.sc.frame_seed;
+    b.background = a.background;
+    hipLaunchKernelGGL(bin_splat_kernel, dim3(a.cus), dim3(GPS_BIN_THREADS), 0, st, a.recs, a.cidx, a.coff,
+                       (const uint32_t*)a.list, (const uint32_t*)a.lpre, (const uint32_t*)a.reg_ecnt,
+                       (const uint32_t*)a.reg_ptot, (const uint32_t*)a.reg_eoff, (const uint32_t*)a.reg_poff,
+                       (const uint32_t*)a.reg_jobs, (const uint32_t*)a.job_reg, (const uint32_t*)a.job_p0, a.state,
+                       reinterpret_cast<unsigned long long*>(a.fb), b, a.job_points, a.atomics_count);
+    hipLaunchKernelGGL(bin_big_kernel, dim3(grid), dim3(256), 0, st, a.recs, a.cidx, a.coff, (const uint32_t*)a.ew,
+                       (const uint32_t*)a.big, a.state, reinterpret_cast<unsigned long long*>(a.fb), a.sc,
+                       a.atomics_count);
+}
+
+bool binned_geometry(int up_w, int up_h, int spp, BinGeom* g) {
+    if (spp < 1 || spp > 4) return false;
+    g->spp = spp;
+    g->up_w = up_w;
+    g->up_h = up_h;
+    g->rw_shift = spp <= 2 ? 7 : 6;
+    g->rh_shift = spp == 1 ? 7 : 6;
+    g->rtx = 1 << (g->rw_shift - GPS_TILE_BITS);
+    g->rty = 1 << (g->rh_shift - GPS_TILE_BITS);
+    g->nrx = (up_w + (1 << g->rw_shift) - 1) >> g->rw_shift;
+    g->nry = (up_h + (1 << g->rh_shift) - 1) >> g->rh_shift;
+    g->tiles_per_row = fb_tiles_per_row(up_w);
+    const uint64_t n = (uint64_t)g->nrx * (uint64_t)g->nry;
+    if (n == 0 || n > GPS_BIN_HIST) return false;
+    g->nregions = (uint32_t)n;
+    return (uint64_t)g->rtx * g->rty * GPS_TILE_WORDS * (uint64_t)spp <= GPS_BIN_WORDS;
 }
 
 }  // namespace gps

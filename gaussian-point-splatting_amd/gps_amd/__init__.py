@@ -94,7 +94,9 @@ class Stats(C.Structure):
                 ("total_weight", C.c_uint64 * 2), ("num_points", C.c_uint64 * 2), ("samples", C.c_uint64),
                 ("culled_blocks", C.c_uint64 * 2), ("passes", C.c_int32), ("budget_truncated", C.c_int32),
                 ("K", C.c_int32), ("gaussians_per_block", C.c_int32), ("atomics", C.c_uint64),
-                ("num_blocks", C.c_uint64)]
+                ("num_blocks", C.c_uint64), ("splat_binned", C.c_int32), ("splat_fallback", C.c_int32),
+                ("bin_entries", C.c_uint32), ("bin_jobs", C.c_uint32), ("bin_big", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {"num_gaussians": int(self.num_gaussians), "valid_gaussians": list(self.valid_gaussians),
@@ -102,7 +104,9 @@ class Stats(C.Structure):
                 "samples": int(self.samples), "culled_blocks": list(self.culled_blocks),
                 "passes": int(self.passes), "budget_truncated": int(self.budget_truncated), "K": int(self.K),
                 "gpb": int(self.gaussians_per_block), "num_blocks": int(self.num_blocks),
-                "atomics": int(self.atomics)}
+                "atomics": int(self.atomics), "splat_binned": int(self.splat_binned),
+                "splat_fallback": int(self.splat_fallback), "bin_entries": int(self.bin_entries),
+                "bin_jobs": int(self.bin_jobs), "bin_big": int(self.bin_big)}
 
 
 class SynthParams(C.Structure):
@@ -171,6 +175,8 @@ _sig("gps_event_record", C.c_int32, _vp, _vp)
 _sig("gps_event_elapsed_ms", C.c_int32, _vp, _vp, C.POINTER(C.c_float))
 
 _sig("gps_pass_set_timing", C.c_int32, _vp, C.c_int32)
+_sig("gps_pass_set_splat_mode", C.c_int32, _vp, C.c_int32)
+_sig("gps_pass_set_counters", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_timing_summary", C.c_int32, _vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
      C.POINTER(C.c_uint64))
 _sig("gps_selftest_math", C.c_int32, C.c_int32, _vp, _vp, C.c_uint64, _vp)
@@ -496,6 +502,17 @@ class Pass:
         return out
 
     STAGES = ("clear", "cull", "preprocess", "distribute", "splat", "combine", "hiz")
+
+    SPLAT_AUTO = 0
+    SPLAT_POINT_PARALLEL = 1
+    SPLAT_BINNED = 2
+
+    def set_splat_mode(self, mode: int) -> None:
+        """SPLAT_AUTO, SPLAT_POINT_PARALLEL or SPLAT_BINNED."""
+        check(_lib.gps_pass_set_splat_mode(self.handle, mode), "splat mode")
+
+    def set_counters(self, count_atomics: bool) -> None:
+        check(_lib.gps_pass_set_counters(self.handle, int(count_atomics)), "counters")
 
     def set_timing(self, capacity: int) -> None:
         check(_lib.gps_pass_set_timing(self.handle, capacity), "set timing")

@@ -126,6 +126,12 @@ typedef struct gps_stats {
     int32_t gaussians_per_block;   /* culling block size in use (0 when block culling is off) */
     uint64_t atomics;              /* framebuffer atomicMin operations issued (diagnostic) */
     uint64_t num_blocks;           /* culling blocks (0 when block culling is off) */
+    int32_t splat_binned;          /* 1: the last pass used the binned splat */
+    int32_t splat_fallback;        /* 1: its region lists overflowed, the point-parallel splat ran */
+    uint32_t bin_entries;          /* (Gaussian, region) pairs of the last pass */
+    uint32_t bin_jobs;             /* region jobs of the last pass */
+    uint32_t bin_big;              /* Gaussians splatted with global atomics (too many regions) */
+    uint32_t reserved;
 } gps_stats;
 
 /* -------------------------------------------------------------- versioning */
@@ -154,6 +160,20 @@ uint64_t gps_scene_device_bytes(const gps_scene* scene);
  * must outlive the pass; several passes may share one scene. */
 gps_status gps_pass_create(const gps_render_settings* settings, gps_scene* scene, gps_pass** out);
 void gps_pass_destroy(gps_pass* pass);
+/* Splat kernel choice.  GPS_SPLAT_BINNED: screen regions rendered in LDS,
+ * no per-sample global atomics (spp <= 4, at most 8192 regions per frame,
+ * else point-parallel).  GPS_SPLAT_POINT_PARALLEL: one 64-bit global
+ * atomicMin per accepted sample.  GPS_SPLAT_AUTO (default): binned when a
+ * point has at least 8 samples (K * spp).  All give the same framebuffer bit
+ * for bit.  No reference counterpart (the reference has only the
+ * point-parallel form, gaussian_point_splatting.cu:383-432). */
+#define GPS_SPLAT_AUTO 0
+#define GPS_SPLAT_POINT_PARALLEL 1
+#define GPS_SPLAT_BINNED 2
+gps_status gps_pass_set_splat_mode(gps_pass* pass, int32_t mode);
+/* Count the splat's global atomics into gps_stats.atomics (off by default:
+ * the count costs one atomic per workgroup on a single word). */
+gps_status gps_pass_set_counters(gps_pass* pass, int32_t count_atomics);
 /* Device memory held by the pass (scene not included), and what
  * gps_pass_create would allocate for these settings. */
 uint64_t gps_pass_device_bytes(const gps_pass* pass);

@@ -956,6 +956,7 @@ void orc_detmath(int32_t kind, const float* in, float* out, uint64_t n) {
             case 8: r = (float)gps::det_f2i_rz(std::nearbyint(x)); break;
             case 9: r = boxMuller(x, in[(i + 1) % n]).x; break;
             case 10: r = correctedBoxMuller(x, in[(i + 1) % n], 0.7f).y; break;
+            case 11: r = fp8e4m3_to_float((uint8_t)((uint32_t)(int)x & 0xFFu)); break;
         }
         out[i] = r;
     }

@@ -444,3 +444,62 @@ def test_batch_views_identical_to_single_renders(gpu):
         stream.synchronize()
         r = alone(fr)  # eager single-view work between replays
         check_batch(fr, cols, deps, r, f"graph {k}")
+
+
+# ------------------------------------------------------------------ binned splat
+@pytest.mark.parametrize("case", ["spp1_ss1", "spp2_ss2", "spp3_ss1", "spp4_ss2", "closeup_big", "budget",
+                                  "occlusion_two_pass"])
+def test_binned_splat_equals_point_parallel(gpu, case):
+    """The binned (LDS region) splat and the point-parallel splat with global
+    atomics write the same framebuffer bit for bit: single- and multi-job
+    regions, Gaussians too large to bin, the point budget, and the second
+    pass of occlusion culling (regions start from pass-1 contents)."""
+    spp, ss = {"spp1_ss1": (1, 1), "spp2_ss2": (2, 2), "spp3_ss1": (3, 1), "spp4_ss2": (4, 2)}.get(case, (1, 2))
+    cfg = dict(H.C1, half_extent=6.0, log_scale_mu=-2.0) if case != "closeup_big" else dict(H.C1, log_scale_mu=-0.7)
+    arrays = H.scene(0x5EED0601 + len(case), 30_000, cfg).morton_sorted()
+    w, h = 320, 200
+    st = gpu.RenderSettings(w, h, spp, ss, max_number_of_points=5_000 if case == "budget" else 250_000_000,
+                            gaussians_per_block=64)
+    occl = case == "occlusion_two_pass"
+    eye = (0.0, 2.0, -9.0) if case == "closeup_big" else (0.0, 3.0, -25.0)
+    frames = [gpu.frame_params((eye[0] + i, eye[1], eye[2]), (0, 0, 0), (0, 1, 0), 45.0, w, h, ss=ss, frame=i + 1,
+                               occlusion=occl, hierarchical=occl) for i in range(3 if occl else 1)]
+    runs = []
+    for mode in (gpu.Pass.SPLAT_BINNED, gpu.Pass.SPLAT_POINT_PARALLEL):
+        r = H.GpuRunner(arrays, st)
+        r.p.set_splat_mode(mode)
+        r.p.set_counters(True)
+        out = []
+        for k, f in enumerate(frames):
+            flush(f"binned {case}: mode {mode} frame {k}")
+            res = r.render(f, [gpu.DBG_FRAMEBUFFER])
+            out.append(res)
+        runs.append(out)
+    for k, (a, b) in enumerate(zip(*runs)):
+        assert_same(f"{case} frame {k} fb", a.debug[gpu.DBG_FRAMEBUFFER], b.debug[gpu.DBG_FRAMEBUFFER])
+        assert_same(f"{case} frame {k} colour", a.color, b.color)
+        assert a.stats["splat_binned"] == 1 and a.stats["splat_fallback"] == 0
+        assert b.stats["splat_binned"] == 0
+        assert a.stats["num_points"] == b.stats["num_points"]
+    s = runs[0][-1].stats
+    flush(f"{case}: entries {s['bin_entries']} jobs {s['bin_jobs']} big {s['bin_big']} points {s['num_points']}")
+    if case == "closeup_big":
+        assert s["bin_big"] > 0
+    if case == "spp1_ss1":
+        assert s["bin_jobs"] > 6  # 3 x 2 regions of 128 px: some are split into several jobs
+
+
+def test_device_fp8_decode_matches_oracle(gpu):
+    """Every FP8 E4M3 code through the hardware conversion the compact SH
+    path uses equals the oracle's software decode (NaN codes: both NaN)."""
+    import orc
+    x = np.arange(256, dtype=np.float32)
+    d_in = gpu.DeviceArray(x.shape, np.float32)
+    d_out = gpu.DeviceArray(x.shape, np.float32)
+    d_in.upload(x)
+    gpu.check(gpu.lib.gps_selftest_math(11, d_in.ptr, d_out.ptr, x.size, None), "selftest")
+    dev = d_out.download()
+    host = orc.detmath(11, x)
+    nan = np.isnan(host)
+    assert nan.sum() == 2 and np.isnan(dev[nan]).all()
+    assert_same("fp8 decode", dev[~nan], host[~nan])
