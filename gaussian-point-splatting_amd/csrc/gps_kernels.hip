@@ -305,44 +305,152 @@ __device__ __forceinline__ uint32_t preprocess_one(const DevScene& s, const Fram
     return cnt;
 }
 
+// The single-view preprocess runs the steps of preprocess_one in three
+// phases with the surviving Gaussians repacked onto the first lanes of the
+// workgroup between phases (LDS work lists), so the counting and the colour
+// steps run on full waves instead of the few lanes that need them:
+//   A  every lane: block cull, previous-mask skip, opacity, projection,
+//      bounds, small-eigenvalue cull, Hi-Z test -> list 1
+//   B  list 1: importance, Poisson count, / K, cap; weight; record geometry
+//      for counts > 0 -> list 2
+//   C  list 2: SH colour, packed depth | colour word of the record.
+// Per Gaussian the arithmetic is exactly preprocess_one's.
+struct PreItem {
+    uint32_t g;
+    float op, view_z, pmx, pmy, ca, cb, cc, det;
+    uint32_t bmin, bmax;
+};
+struct ColourItem {
+    uint32_t g, depth, bmax;
+};
+
 template <int FORMAT, int DEG>
 __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
     const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
     uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h) {
-    const uint32_t g = blockIdx.x * GPS_PRE_THREADS + threadIdx.x;
+    __shared__ PreItem s_item[GPS_PRE_THREADS];
+    __shared__ ColourItem s_col[GPS_PRE_THREADS];
+    __shared__ uint32_t s_count[GPS_PRE_THREADS];
+    __shared__ uint32_t s_n1, s_n2;
+    const int tid = threadIdx.x;
+    const uint32_t g0 = blockIdx.x * GPS_PRE_THREADS;
+    const uint32_t g = g0 + (uint32_t)tid;
     // workload_distributor.cu:32-40: current = A on even iterations.
     const uint32_t it = *iteration;
     uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
     const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
-    uint32_t n = 0;
-    // Gaussians of culled blocks leave their weight slot untouched: the
-    // distributor reads the same cull mask and counts them as zero.
-    if (g < s.N && !(f.hier && block_culled(cv, g))) {
-        n = preprocess_one<FORMAT, DEG>(s, f, g, mask_prev, h, recs + g);
-        weights[g] = n;
+    if (tid == 0) {
+        s_n1 = 0u;
+        s_n2 = 0u;
     }
-    // Two 32-bit valid-mask words per wave.
+    s_count[tid] = 0u;
+    __syncthreads();
+    // Phase A.  Gaussians of culled blocks leave their weight slot untouched:
+    // the distributor reads the same cull mask and counts them as zero.
+    if (g < s.N && !(f.hier && block_culled(cv, g))) {
+        bool survives = false;
+        if (!(f.only_prev && mask_bit(mask_prev, g))) {
+            const Geo ge = load_geo<FORMAT>(s, g);
+            const float op = glm_min(ge.op, 1.0f);
+            Projected pr;
+            if (op > 0.0f &&
+                project_gaussian(f, ge.mx, ge.my, ge.mz, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx, ge.sy, ge.sz, pr) &&
+                !(pr.minx >= f.up_w || pr.miny >= f.up_h || pr.maxx < 0 || pr.maxy < 0)) {
+                survives = true;
+                if (f.cull_small) {
+                    const float tr = pr.ca + pr.cc;
+                    const float t2 = 0.5f * tr;
+                    const float term2 = det_sqrt(det_fmax(0.1f, t2 * t2 - pr.det));
+                    const float e1 = t2 + term2, e2 = t2 - term2;
+                    if (e1 < 0.02f || e2 < 0.02f) survives = false;
+                }
+                if (survives && f.occl &&
+                    hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss))
+                    survives = false;
+                if (survives) {
+                    const uint32_t k = atomicAdd(&s_n1, 1u);
+                    PreItem& q = s_item[k];
+                    q.g = g;
+                    q.op = op;
+                    q.view_z = pr.view_z;
+                    q.pmx = pr.pmx;
+                    q.pmy = pr.pmy;
+                    q.ca = pr.ca;
+                    q.cb = pr.cb;
+                    q.cc = pr.cc;
+                    q.det = pr.det;
+                    q.bmin = (uint32_t)pr.minx | ((uint32_t)pr.miny << 16);
+                    q.bmax = (uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16);
+                }
+            }
+        }
+        if (!survives) weights[g] = 0u;
+    }
+    __syncthreads();
+    // Phase B
+    if ((uint32_t)tid < s_n1) {
+        const PreItem q = s_item[tid];
+        u32x2 seed = make_seed(q.g, f.frame_seed);
+        const float s_det = det_sqrt(q.det);
+        const float dl = dilog(q.op);
+        const float importance = f.unbiased ? GPS_TWO_PI_F * s_det * dl : q.op * GPS_TWO_PI_F * s_det;
+        uint32_t cnt = f.unbiased ? poisson(seed, importance) : (uint32_t)stochastic_round(importance, pcg2d(seed).x);
+        if (f.K > 1.0f) cnt = (uint32_t)stochastic_round((float)cnt / f.K, pcg2d(seed).x);
+        if (cnt) {
+            cnt = min(cnt, f.point_cap);
+            const float c0 = det_sqrt(q.ca);
+            const float c1 = q.cb / c0;
+            const float c2 = det_sqrt(q.cc - c1 * c1);
+            const float inv_det = 1.0f / q.det;
+            uint4* r = reinterpret_cast<uint4*>(recs + q.g);
+            r[0] = make_uint4(__float_as_uint(q.pmx), __float_as_uint(q.pmy), __float_as_uint(c0), __float_as_uint(c1));
+            r[1] = make_uint4(__float_as_uint(c2), __float_as_uint(q.cc * inv_det), __float_as_uint(-q.cb * inv_det),
+                              __float_as_uint(q.ca * inv_det));
+            r[2] = make_uint4(__float_as_uint(q.op), __float_as_uint(corrected_inv_alpha(q.op)), __float_as_uint(dl),
+                              q.bmin);
+            const uint32_t k = atomicAdd(&s_n2, 1u);
+            s_col[k].g = q.g;
+            s_col[k].depth = (uint32_t)depth_quantize(q.view_z, f.near_plane, f.far_plane);
+            s_col[k].bmax = q.bmax;
+            s_count[q.g - g0] = cnt;
+        }
+        weights[q.g] = cnt;
+    }
+    __syncthreads();
+    // Phase C
+    if ((uint32_t)tid < s_n2) {
+        const ColourItem q = s_col[tid];
+        const Geo ge = load_geo<FORMAT>(s, q.g);
+        float dx, dy, dz;
+        view_dir(f, ge.mx, ge.my, ge.mz, dx, dy, dz);
+        float rgb[3];
+        ShEval<FORMAT, DEG>::eval(s, q.g, dx, dy, dz, rgb);
+        const uint64_t packed = ((uint64_t)q.depth << GPS_DEPTH_SHIFT) | pack_color36(rgb[0], rgb[1], rgb[2]);
+        reinterpret_cast<uint4*>(recs + q.g)[3] = make_uint4(q.bmax, (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
+    }
+    // Two 32-bit valid-mask words per wave, tile sums.
+    const uint32_t n = s_count[tid];
     const uint64_t b = __ballot(n != 0u);
-    const int lane = threadIdx.x & 63;
+    const int lane = tid & 63;
     const uint32_t wave_first = g - (uint32_t)lane;
     const uint32_t sum = wave_reduce_add(n);
     __shared__ uint32_t s_sum[GPS_PRE_THREADS / 64], s_cnt[GPS_PRE_THREADS / 64];
     if (lane == 0) {
         if (wave_first < s.N) mask_cur[wave_first >> 5] = (uint32_t)b;
         if (wave_first + 32u < s.N) mask_cur[(wave_first >> 5) + 1] = (uint32_t)(b >> 32);
-        s_sum[threadIdx.x >> 6] = sum;
-        s_cnt[threadIdx.x >> 6] = (uint32_t)__popcll(b);
+        s_sum[tid >> 6] = sum;
+        s_cnt[tid >> 6] = (uint32_t)__popcll(b);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         uint32_t ts = 0, tc = 0;
 #pragma unroll
         for (int w = 0; w < GPS_PRE_THREADS / 64; ++w) {
             ts += s_sum[w];
             tc += s_cnt[w];
         }
-        const uint32_t tile = (blockIdx.x * GPS_PRE_THREADS) >> GPS_TILE_SHIFT;
+        const uint32_t tile = g0 >> GPS_TILE_SHIFT;
         if (tc) {
             atomicAdd(tile_sum + tile, ts);
             atomicAdd(tile_cnt + tile, tc);
