@@ -229,6 +229,8 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-closed", type=int, default=300, help="requests of the closed-loop comparison")
     ap.add_argument("--batch-views", type=int, default=64, help="views per batch (0 = skip the batched form)")
+    ap.add_argument("--no-splat-forms", action="store_true",
+                    help="skip re-timing the frames with each splat form forced")
     ap.add_argument("--batch-count", type=int, default=5, help="timed batches")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -288,7 +290,9 @@ def main() -> int:
     # The same frames with each splat form forced (binned vs point-parallel
     # with one global atomic per sample): the chosen and the plain form.
     splat_forms = {}
-    for name, mode in (("point_parallel", G.Pass.SPLAT_POINT_PARALLEL), ("binned", G.Pass.SPLAT_BINNED)):
+    forms = () if args.no_splat_forms else (("point_parallel", G.Pass.SPLAT_POINT_PARALLEL),
+                                            ("binned", G.Pass.SPLAT_BINNED))
+    for name, mode in forms:
         loop = W.FrameLoop(w, scene, stream)
         loop.pass_.set_splat_mode(mode)
         run(loop, range(Wm))

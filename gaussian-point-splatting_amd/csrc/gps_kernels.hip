@@ -778,7 +778,9 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const SplatRecord* __res
                                                         uint2* __restrict__ erect, uint32_t* __restrict__ ew,
                                                         uint32_t* __restrict__ reg_cnt, uint32_t* __restrict__ reg_pts,
                                                         uint32_t* __restrict__ big, uint32_t* __restrict__ state) {
-    __shared__ uint32_t s_cnt[GPS_BIN_HIST], s_pts[GPS_BIN_HIST];
+    extern __shared__ uint32_t s_dyn[];  // 2 * nregions words
+    uint32_t* s_cnt = s_dyn;
+    uint32_t* s_pts = s_dyn + g.nregions;
     const int tid = threadIdx.x;
     for (uint32_t i = tid; i < g.nregions; i += 256) {
         s_cnt[i] = 0u;
@@ -909,7 +911,9 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const PassCounters* __
                                                           uint32_t* __restrict__ reg_cursor,
                                                           uint32_t* __restrict__ list, uint32_t* __restrict__ lw,
                                                           const uint32_t* __restrict__ state) {
-    __shared__ uint32_t s_cnt[GPS_BIN_HIST], s_base[GPS_BIN_HIST];
+    extern __shared__ uint32_t s_dyn[];  // 2 * nregions words
+    uint32_t* s_cnt = s_dyn;
+    uint32_t* s_base = s_dyn + g.nregions;
     if (state[3]) return;
     const int tid = threadIdx.x;
     for (uint32_t i = tid; i < g.nregions; i += 256) s_cnt[i] = 0u;
@@ -1959,12 +1963,13 @@ bool launch_batch_splat(const BatchArgs& a, hipStream_t st) {
 void launch_binned_splat(const BinnedArgs& a, hipStream_t st) {
     launch_zero_u32(a.state, 8, st);
     const uint32_t grid = a.cus * 4u;
-    hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), 0, st, a.recs, a.cidx, a.coff, a.ctr, a.g, a.erect,
+    const size_t hist = 2u * a.g.nregions * sizeof(uint32_t);
+    hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), hist, st, a.recs, a.cidx, a.coff, a.ctr, a.g, a.erect,
                        a.ew, a.reg_cnt, a.reg_pts, a.big, a.state);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, a.g, a.reg_cnt, a.reg_pts, a.reg_ecnt, a.reg_ptot,
                        a.reg_eoff, a.reg_poff, a.reg_cursor, a.reg_jobs, a.job_reg, a.job_p0, a.state, a.list_cap,
                        a.job_points, a.job_cap);
-    hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), 0, st, a.ctr, a.g, (const uint2*)a.erect,
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), hist, st, a.ctr, a.g, (const uint2*)a.erect,
                        (const uint32_t*)a.ew, a.reg_cursor, a.list, a.lw, (const uint32_t*)a.state);
     hipLaunchKernelGGL(bin_tile_sum_kernel, dim3(grid), dim3(256), 0, st, (const uint32_t*)a.lw,
                        (const uint32_t*)a.state, a.tsum);
