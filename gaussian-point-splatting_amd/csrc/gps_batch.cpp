@@ -11,6 +11,9 @@
 // (occlusion culling off).  The launch sequence can be captured once into a
 // HIP graph and replayed for new cameras of the same view layout; the splat
 // variant (threads, points per thread) is picked by gps_batch_autotune.
+// Views with many samples per point use the binned splat (the regions of all
+// such views in one set of launches), the others the point-parallel splat,
+// under the same rule as a single pass.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,7 +32,7 @@ using namespace gps;
 namespace {
 
 struct Layout {  // what a captured graph depends on
-    std::vector<int32_t> shape;  // per view: w, h, ss, spp, hier
+    std::vector<int32_t> shape;  // per view: w, h, ss, spp, hier, regions
     std::vector<const void*> outs;
     void* stream = nullptr;
     int threads = 0, ppt = 0;
@@ -62,9 +65,14 @@ struct gps_batch {
     uint32_t mask_stride = 0;
     bool hier_available = false;
     uint64_t* fb = nullptr;
-    uint32_t grid = 0;
+    uint32_t grid = 0, cus = 256;
     int32_t xcds = 1;
     int threads = 256, ppt = 4;
+    int splat_mode = GPS_SPLAT_AUTO;
+    // binned splat buffers, allocated on the first batch with a binned view
+    bool bin_ready = false, bin_failed = false;
+    BatchBinArgs bin{};
+    uint32_t bin_regions_cap = 0;
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
@@ -82,6 +90,71 @@ template <typename T> gps_status balloc(gps_batch* b, T** p, uint64_t count, boo
     b->bytes += count * sizeof(T);
     if (zero && hipMemset(*p, 0, count * sizeof(T)) != hipSuccess) return GPS_ERR_HIP;
     return GPS_OK;
+}
+
+void free_binned(gps_batch* b) {
+    BatchBinArgs& a = b->bin;
+    for (void* p : {(void*)a.erect, (void*)a.ew, (void*)a.big, (void*)a.reg_cnt, (void*)a.reg_pts, (void*)a.reg_ecnt,
+                    (void*)a.reg_ptot, (void*)a.reg_eoff, (void*)a.reg_poff, (void*)a.reg_cursor, (void*)a.reg_jobs,
+                    (void*)a.job_reg, (void*)a.job_p0, (void*)a.list, (void*)a.lw, (void*)a.lpre, (void*)a.tsum,
+                    (void*)a.toff, (void*)a.state})
+        if (p) (void)hipFree(p);
+    a = BatchBinArgs{};
+}
+
+// Binned splat buffers for every compacted entry of a full batch and the
+// regions of max_views views of the largest size.  On failure the batch
+// keeps rendering every view with the point-parallel splat.
+bool ensure_binned(gps_batch* b) {
+    if (b->bin_ready) return true;
+    if (b->bin_failed) return false;
+    const uint64_t V = (uint64_t)b->settings.max_views;
+    const uint64_t entries = std::max<uint64_t>(V * b->N, 1);
+    uint32_t max_regions = 0;
+    for (int spp = 1; spp <= std::min(b->settings.max_spp, 4); ++spp) {
+        BinGeom g;
+        if (binned_geometry(b->settings.max_width * b->settings.max_supersampling,
+                            b->settings.max_height * b->settings.max_supersampling, spp, &g))
+            max_regions = std::max(max_regions, g.nregions);
+    }
+    if (max_regions == 0) {
+        b->bin_failed = true;
+        return false;
+    }
+    BatchBinArgs& a = b->bin;
+    const uint64_t regions = V * max_regions;
+    const uint64_t cap = std::min<uint64_t>(entries + entries / 2 + 65536, 0xFFFFF000ull);
+    const uint64_t list_cap = (cap + GPS_TILE - 1) / GPS_TILE * GPS_TILE;
+    a.list_cap = (uint32_t)list_cap;
+    a.job_cap = (uint32_t)(regions + (1ull << 32) / GPS_BIN_JOB_POINTS + 1);
+    a.job_points = GPS_BIN_JOB_POINTS;
+    gps_status s = GPS_OK;
+    auto chk = [&](gps_status r) {
+        if (s == GPS_OK) s = r;
+    };
+    chk(balloc(b, &a.erect, entries, false));
+    chk(balloc(b, &a.ew, entries, false));
+    chk(balloc(b, &a.big, entries, false));
+    for (uint32_t** p : {&a.reg_cnt, &a.reg_pts, &a.reg_ecnt, &a.reg_ptot, &a.reg_eoff, &a.reg_poff, &a.reg_cursor,
+                         &a.reg_jobs})
+        chk(balloc(b, p, regions, true));
+    chk(balloc(b, &a.job_reg, a.job_cap, false));
+    chk(balloc(b, &a.job_p0, a.job_cap, false));
+    chk(balloc(b, &a.list, list_cap, false));
+    chk(balloc(b, &a.lw, list_cap, true));
+    chk(balloc(b, &a.lpre, list_cap, false));
+    chk(balloc(b, &a.tsum, list_cap / GPS_TILE + 1, false));
+    chk(balloc(b, &a.toff, list_cap / GPS_TILE + 1, false));
+    chk(balloc(b, &a.state, 8, true));
+    if (s != GPS_OK) {
+        free_binned(b);
+        (void)hipGetLastError();
+        b->bin_failed = true;
+        return false;
+    }
+    b->bin_regions_cap = (uint32_t)regions;
+    b->bin_ready = true;
+    return true;
 }
 
 void drop_graph(gps_batch* b) {
@@ -121,6 +194,7 @@ BatchArgs args(gps_batch* b, int nviews) {
     a.threads = b->threads;
     a.ppt = b->ppt;
     a.atomics = b->atomics;
+    a.bin = b->bin;
     return a;
 }
 
@@ -133,7 +207,9 @@ gps_status enqueue(gps_batch* b, int nviews, const gps_frame_params* views, floa
     if (b->hier_available) launch_batch_cull(a, st);
     launch_batch_preprocess(a, st);
     launch_batch_distribute(a, st);
+    launch_batch_binned(a, st);
     if (!launch_batch_splat(a, st)) return GPS_ERR_INVALID_ARG;
+    launch_batch_bin_big(a, st);
     for (int v = 0; v < nviews; ++v) {
         const gps_frame_params& f = views[v];
         const int spp = f.samples_per_pixel > 0 ? f.samples_per_pixel : b->settings.max_spp;
@@ -216,6 +292,7 @@ gps_status gps_batch_create(const gps_batch_settings* st, gps_scene* scene, gps_
     }
     b->xcds = (cus % 8 == 0) ? 8 : 1;
     b->grid = (uint32_t)cus * 8u;
+    b->cus = (uint32_t)cus;
     if (s == GPS_OK && hipDeviceSynchronize() != hipSuccess) s = GPS_ERR_HIP;
     if (s != GPS_OK) {
         gps_batch_destroy(b);
@@ -235,6 +312,7 @@ gps_status gps_batch_render(gps_batch* b, int32_t nviews, const gps_frame_params
     lay.stream = stream;
     lay.threads = b->threads;
     lay.ppt = b->ppt;
+    uint32_t regions = 0, max_view_regions = 0;
     for (int v = 0; v < nviews; ++v) {
         const gps_frame_params& f = views[v];
         const int spp = f.samples_per_pixel > 0 ? f.samples_per_pixel : b->settings.max_spp;
@@ -256,10 +334,29 @@ gps_status gps_batch_render(gps_batch* b, int32_t nviews, const gps_frame_params
         bv.fb_words = fb_words(f.width * f.supersampling_factor, f.height * f.supersampling_factor, spp);
         bv.background = background_word(f.background[0], f.background[1], f.background[2]);
         bv.budget = (uint32_t)std::min<int64_t>(b->settings.max_number_of_points, 0x7FFFFFFFll);
-        lay.shape.insert(lay.shape.end(), {f.width, f.height, f.supersampling_factor, spp, bv.f.hier});
+        // Binned splat for the view under the same rule as a single pass:
+        // when a point has >= 8 samples (K * spp), or when forced.
+        BinGeom g;
+        if (b->splat_mode != GPS_SPLAT_POINT_PARALLEL &&
+            (b->splat_mode == GPS_SPLAT_BINNED || bv.f.Ki * bv.f.spp >= 8) &&
+            binned_geometry(bv.f.up_w, bv.f.up_h, bv.f.spp, &g) && ensure_binned(b) &&
+            regions + g.nregions <= b->bin_regions_cap) {
+            bv.bg = g;
+            bv.rbase = regions;
+            regions += g.nregions;
+            max_view_regions = std::max(max_view_regions, g.nregions);
+        } else {
+            bv.rbase = regions;  // bg.nregions == 0: point-parallel
+        }
+        lay.shape.insert(lay.shape.end(), {f.width, f.height, f.supersampling_factor, spp, bv.f.hier,
+                                           (int32_t)bv.bg.nregions});
         lay.outs.push_back(colors[v]);
         lay.outs.push_back(depths[v]);
     }
+    b->bin.any = regions > 0 ? 1 : 0;
+    b->bin.nregions = regions;
+    b->bin.max_view_regions = max_view_regions;
+    b->bin.cus = b->cus;
     if (hipMemcpyAsync(b->d_views, b->h_views, nviews * sizeof(BatchView), hipMemcpyHostToDevice, st) != hipSuccess)
         return GPS_ERR_HIP;
     b->last_views = nviews;
@@ -282,6 +379,13 @@ gps_status gps_batch_render(gps_batch* b, int32_t nviews, const gps_frame_params
         b->layout = lay;
     }
     return hipGraphLaunch(b->exec, st) == hipSuccess ? GPS_OK : GPS_ERR_HIP;
+}
+
+gps_status gps_batch_set_splat_mode(gps_batch* b, int32_t mode) {
+    if (!b || mode < GPS_SPLAT_AUTO || mode > GPS_SPLAT_BINNED) return GPS_ERR_INVALID_ARG;
+    if (b->splat_mode != mode) drop_graph(b);
+    b->splat_mode = mode;
+    return GPS_OK;
 }
 
 gps_status gps_batch_set_tuning(gps_batch* b, const gps_batch_tuning* t) {
@@ -362,6 +466,15 @@ gps_status gps_batch_view_stats(gps_batch* b, int32_t view, gps_stats* out, void
     uint32_t atomics = 0;
     if (hipMemcpy(&atomics, b->atomics, 4, hipMemcpyDeviceToHost) != hipSuccess) return GPS_ERR_HIP;
     out->atomics = atomics;  // whole batch
+    if (bv.bg.nregions) {
+        uint32_t st[8];
+        if (hipMemcpy(st, b->bin.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return GPS_ERR_HIP;
+        out->splat_binned = st[3] ? 0 : 1;
+        out->splat_fallback = st[3] ? 1 : 0;
+        out->bin_entries = st[0];  // whole batch
+        out->bin_jobs = st[1];
+        out->bin_big = st[2];
+    }
     return GPS_OK;
 }
 
@@ -385,6 +498,7 @@ void gps_batch_destroy(gps_batch* b) {
     (void)hipFree(b->nodes);
     (void)hipFree(b->cull_mask);
     (void)hipFree(b->fb);
+    free_binned(b);
     delete b;
 }
 

@@ -97,6 +97,16 @@ struct FrameCounters {
     uint32_t pad[3];
 };
 
+// Region geometry of the binned splat for one framebuffer.
+struct BinGeom {
+    int32_t rw_shift, rh_shift;  // log2 of the region width / height in pixels
+    int32_t nrx, nry;            // regions per row / column
+    int32_t up_w, up_h, spp;
+    int32_t tiles_per_row;       // framebuffer 64x64 tiles per row
+    int32_t rtx, rty;            // framebuffer tiles per region (x, y)
+    uint32_t nregions;
+};
+
 // One view of a multi-view batch (device table entry).  The host fills the
 // frame part before every batch; the device writes the per-view results.
 struct BatchView {
@@ -106,11 +116,14 @@ struct BatchView {
     uint64_t fb_words;      // words of this view's framebuffer
     uint64_t background;    // background word
     uint32_t budget;        // point budget of the view
+    BinGeom bg;             // binned splat regions (bg.nregions == 0: point-parallel splat)
+    uint32_t rbase;         // first region of the view in the batch's region numbering
     // written on the device
     uint32_t base;          // first point of the view in the batch's point list
     uint32_t points;        // points splatted (min(total, budget))
     uint32_t total;         // uncapped sum of point counts
     uint32_t valid;         // valid Gaussians
+    uint32_t ebase;         // first compacted entry of the view
     uint32_t culled_blocks;
 };
 #define GPS_BATCH_MAX_VIEWS 256
@@ -132,14 +145,6 @@ struct BatchView {
 #define GPS_BIN_MAX_REGIONS 4u
 #define GPS_BIN_HIST 8192u
 #define GPS_BIN_JOB_POINTS 16384u
-struct BinGeom {
-    int32_t rw_shift, rh_shift;  // log2 of the region width / height in pixels
-    int32_t nrx, nry;            // regions per row / column
-    int32_t up_w, up_h, spp;
-    int32_t tiles_per_row;       // framebuffer 64x64 tiles per row
-    int32_t rtx, rty;            // framebuffer tiles per region (x, y)
-    uint32_t nregions;
-};
 
 // Distributor tile: 256 threads x 16 Gaussians.
 #define GPS_TILE_SHIFT 12

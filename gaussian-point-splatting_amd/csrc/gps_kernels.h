@@ -83,6 +83,24 @@ struct BinnedArgs {
 bool binned_geometry(int up_w, int up_h, int spp, BinGeom* g);
 void launch_binned_splat(const BinnedArgs& a, hipStream_t st);
 
+// Binned splat buffers of a batch: entries are the batch's compacted entries,
+// regions those of all binned views (view v owns [rbase, rbase + nregions)).
+struct BatchBinArgs {
+    int32_t any;                // at least one view is binned
+    uint32_t nregions;          // regions of all binned views
+    uint32_t max_view_regions;  // largest view's regions (LDS histogram size)
+    uint2* erect;
+    uint32_t* ew;
+    uint32_t* big;
+    uint32_t *reg_cnt, *reg_pts, *reg_ecnt, *reg_ptot, *reg_eoff, *reg_poff, *reg_cursor, *reg_jobs;
+    uint32_t *job_reg, *job_p0;
+    uint32_t *list, *lw, *lpre;
+    uint32_t *tsum, *toff;
+    uint32_t* state;            // as BinnedArgs::state
+    uint32_t list_cap, job_cap, job_points;
+    uint32_t cus;
+};
+
 // Multi-view batch (see the batch kernels): all arrays are concatenated per view.
 struct BatchArgs {
     DevScene scene;
@@ -107,6 +125,7 @@ struct BatchArgs {
     int32_t xcds;
     int32_t threads, ppt;    // splat variant
     uint32_t* atomics;
+    BatchBinArgs bin;
 };
 extern const int kBatchSplatVariants[][2];
 extern const int kNumBatchSplatVariants;
@@ -115,6 +134,8 @@ void launch_batch_cull(const BatchArgs& a, hipStream_t st);
 void launch_batch_preprocess(const BatchArgs& a, hipStream_t st);
 void launch_batch_distribute(const BatchArgs& a, hipStream_t st);
 bool launch_batch_splat(const BatchArgs& a, hipStream_t st);
+void launch_batch_binned(const BatchArgs& a, hipStream_t st);
+void launch_batch_bin_big(const BatchArgs& a, hipStream_t st);
 
 void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st);
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,

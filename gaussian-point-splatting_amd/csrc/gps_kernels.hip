@@ -768,31 +768,28 @@ __device__ __forceinline__ uint32_t bin_region_words(const BinGeom& g) {
     return (uint32_t)(g.rtx * g.rty) * GPS_TILE_WORDS * (uint32_t)g.spp;
 }
 
-// Per entry: points (clipped to the pass budget), region rectangle, region
-// histogram in LDS (one contiguous entry range per workgroup, so the
-// Morton-ordered entries touch few regions), big entries to their list.
-__global__ __launch_bounds__(256) void bin_count_kernel(const SplatRecord* __restrict__ recs,
-                                                        const uint32_t* __restrict__ cidx,
-                                                        const uint32_t* __restrict__ coff,
-                                                        const PassCounters* __restrict__ ctr, BinGeom g,
-                                                        uint2* __restrict__ erect, uint32_t* __restrict__ ew,
-                                                        uint32_t* __restrict__ reg_cnt, uint32_t* __restrict__ reg_pts,
-                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ state) {
-    extern __shared__ uint32_t s_dyn[];  // 2 * nregions words
-    uint32_t* s_cnt = s_dyn;
-    uint32_t* s_pts = s_dyn + g.nregions;
+// Entries [r0, r1) of one frame whose entries end at v_end, whose points end
+// at t_end and whose point budget ends at p_end: points per entry (clipped to
+// the budget), region rectangle, region histogram in LDS (a workgroup takes
+// one contiguous entry range, so the Morton-ordered entries touch few
+// regions), big entries to their list.  The histogram is added to the
+// frame's regions, which start at rbase.
+__device__ __forceinline__ void bin_count_range(const SplatRecord* __restrict__ recs, const uint32_t* __restrict__ cidx,
+                                                const uint32_t* __restrict__ coff, const BinGeom& g, uint32_t r0,
+                                                uint32_t r1, uint32_t v_end, uint32_t t_end, uint32_t p_end,
+                                                uint32_t rbase, uint2* __restrict__ erect, uint32_t* __restrict__ ew,
+                                                uint32_t* __restrict__ reg_cnt, uint32_t* __restrict__ reg_pts,
+                                                uint32_t* __restrict__ big, uint32_t* __restrict__ state,
+                                                uint32_t* s_cnt, uint32_t* s_pts) {
     const int tid = threadIdx.x;
     for (uint32_t i = tid; i < g.nregions; i += 256) {
         s_cnt[i] = 0u;
         s_pts[i] = 0u;
     }
     __syncthreads();
-    const uint32_t V = ctr->valid, P = ctr->points, T = ctr->total;
-    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
-    const uint32_t r0 = blockIdx.x * per, r1 = min(V, r0 + per);
     for (uint32_t r = r0 + tid; r < r1; r += 256) {
         const uint32_t lo = coff[r];
-        const uint32_t hi = min(r + 1 < V ? coff[r + 1] : T, P);
+        const uint32_t hi = min(r + 1 < v_end ? coff[r + 1] : t_end, p_end);
         uint32_t w = lo < hi ? hi - lo : 0u;
         uint2 rect = make_uint2(0xFFFFFFFFu, 0u);
         if (w) {
@@ -823,11 +820,25 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const SplatRecord* __res
     __syncthreads();
     for (uint32_t i = tid; i < g.nregions; i += 256)
         if (s_cnt[i]) {
-            atomicAdd(&reg_cnt[i], s_cnt[i]);
-            atomicAdd(&reg_pts[i], s_pts[i]);
+            atomicAdd(&reg_cnt[rbase + i], s_cnt[i]);
+            atomicAdd(&reg_pts[rbase + i], s_pts[i]);
         }
 }
 
+__global__ __launch_bounds__(256) void bin_count_kernel(const SplatRecord* __restrict__ recs,
+                                                        const uint32_t* __restrict__ cidx,
+                                                        const uint32_t* __restrict__ coff,
+                                                        const PassCounters* __restrict__ ctr, BinGeom g,
+                                                        uint2* __restrict__ erect, uint32_t* __restrict__ ew,
+                                                        uint32_t* __restrict__ reg_cnt, uint32_t* __restrict__ reg_pts,
+                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ state) {
+    extern __shared__ uint32_t s_dyn[];  // 2 * nregions words
+    const uint32_t V = ctr->valid, P = ctr->points, T = ctr->total;
+    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = min(V, blockIdx.x * per), r1 = min(V, r0 + per);
+    bin_count_range(recs, cidx, coff, g, r0, r1, V, T, P, 0u, erect, ew, reg_cnt, reg_pts, big, state, s_dyn,
+                    s_dyn + g.nregions);
+}
 // One workgroup: region offsets, jobs of at most `job_points` points per
 // region, the job table, and the list length (overflow flag).  Clears the
 // region counters for the next pass.
@@ -844,17 +855,24 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinGeom g, uint32_t* __r
                                                         uint32_t job_points, uint32_t job_cap) {
     __shared__ uint32_t s_a[16], s_b[16], s_c[16];
     __shared__ uint32_t s_ca, s_cb, s_cc;
+    __shared__ unsigned long long s_wide_e, s_wide_p;  // 64-bit totals: list and point overflow
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) {
         s_ca = 0u;
         s_cb = 0u;
         s_cc = 0u;
+        s_wide_e = 0ull;
+        s_wide_p = 0ull;
     }
     __syncthreads();
     for (uint32_t base = 0; base < g.nregions; base += 1024) {
         const uint32_t i = base + tid;
         const uint32_t cnt = i < g.nregions ? reg_cnt[i] : 0u;
         const uint32_t pts = i < g.nregions ? reg_pts[i] : 0u;
+        if (cnt) {
+            atomicAdd(&s_wide_e, (unsigned long long)cnt);
+            atomicAdd(&s_wide_p, (unsigned long long)pts);
+        }
         const uint32_t nj = pts ? (pts + job_points - 1u) / job_points : 0u;
         const uint32_t ia = wave_scan_incl(cnt), ib = wave_scan_incl(pts), ic = wave_scan_incl(nj);
         if (lane == 63) {
@@ -896,31 +914,22 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinGeom g, uint32_t* __r
     if (tid == 0) {
         state[0] = s_ca;
         state[1] = min(s_cc, job_cap);
-        state[3] = (s_ca > list_cap || s_cc > job_cap) ? 1u : 0u;
+        state[3] = (s_wide_e > list_cap || s_cc > job_cap || s_wide_p > 0xFFFFFFFFull) ? 1u : 0u;
         state[4] = 0u;
         state[5] = 0u;
     }
 }
 
-// Region lists: the same entry ranges as bin_count_kernel; the workgroup
-// counts its entries per region in LDS, reserves one range per region with a
-// single global atomic, and fills it.
-__global__ __launch_bounds__(256) void bin_scatter_kernel(const PassCounters* __restrict__ ctr, BinGeom g,
-                                                          const uint2* __restrict__ erect,
-                                                          const uint32_t* __restrict__ ew,
-                                                          uint32_t* __restrict__ reg_cursor,
-                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ lw,
-                                                          const uint32_t* __restrict__ state) {
-    extern __shared__ uint32_t s_dyn[];  // 2 * nregions words
-    uint32_t* s_cnt = s_dyn;
-    uint32_t* s_base = s_dyn + g.nregions;
-    if (state[3]) return;
+// Region lists: the entry ranges of bin_count_range; the workgroup counts its
+// entries per region in LDS, reserves one range per region with a single
+// global atomic, and fills it.
+__device__ __forceinline__ void bin_scatter_range(const BinGeom& g, uint32_t r0, uint32_t r1, uint32_t rbase,
+                                                  const uint2* __restrict__ erect, const uint32_t* __restrict__ ew,
+                                                  uint32_t* __restrict__ reg_cursor, uint32_t* __restrict__ list,
+                                                  uint32_t* __restrict__ lw, uint32_t* s_cnt, uint32_t* s_base) {
     const int tid = threadIdx.x;
     for (uint32_t i = tid; i < g.nregions; i += 256) s_cnt[i] = 0u;
     __syncthreads();
-    const uint32_t V = ctr->valid;
-    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
-    const uint32_t r0 = blockIdx.x * per, r1 = min(V, r0 + per);
     for (uint32_t r = r0 + tid; r < r1; r += 256) {
         const uint2 rect = erect[r];
         if (rect.x == 0xFFFFFFFFu) continue;
@@ -930,7 +939,7 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const PassCounters* __
     }
     __syncthreads();
     for (uint32_t i = tid; i < g.nregions; i += 256) {
-        if (s_cnt[i]) s_base[i] = atomicAdd(&reg_cursor[i], s_cnt[i]);
+        if (s_cnt[i]) s_base[i] = atomicAdd(&reg_cursor[rbase + i], s_cnt[i]);
         s_cnt[i] = 0u;
     }
     __syncthreads();
@@ -948,6 +957,19 @@ __global__ __launch_bounds__(256) void bin_scatter_kernel(const PassCounters* __
     }
 }
 
+__global__ __launch_bounds__(256) void bin_scatter_kernel(const PassCounters* __restrict__ ctr, BinGeom g,
+                                                          const uint2* __restrict__ erect,
+                                                          const uint32_t* __restrict__ ew,
+                                                          uint32_t* __restrict__ reg_cursor,
+                                                          uint32_t* __restrict__ list, uint32_t* __restrict__ lw,
+                                                          const uint32_t* __restrict__ state) {
+    extern __shared__ uint32_t s_dyn[];  // 2 * nregions words
+    if (state[3]) return;
+    const uint32_t V = ctr->valid;
+    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = min(V, blockIdx.x * per), r1 = min(V, r0 + per);
+    bin_scatter_range(g, r0, r1, 0u, erect, ew, reg_cursor, list, lw, s_dyn, s_dyn + g.nregions);
+}
 // Exclusive scan of lw over the list (region-major) in tiles of 4096 slots:
 // per-tile sums, a one-workgroup scan of the tile sums, then the local scan.
 // The list length comes from the device (grid-stride over its tiles).
@@ -1089,6 +1111,109 @@ __device__ __forceinline__ void splat_point_region(const SplatRecord& rec, uint3
     }
 }
 
+// One region job: the region's words from the framebuffer (a region with one
+// job) or the background (several jobs), the job's points [p_lo, p_hi) of
+// the region's list (e0, ecnt: its slots; pbase: its first point in list
+// order), then the region back to the framebuffer: a plain store, or
+// atomicMin of the changed words when the region has several jobs.  `pbias`
+// turns the entries' point offsets into the frame's own point indices.
+// Returns the global atomics issued.  Called by every thread of the
+// workgroup.
+__device__ __forceinline__ uint32_t bin_splat_job(const SplatRecord* __restrict__ recs,
+                                                  const uint32_t* __restrict__ cidx, const uint32_t* __restrict__ coff,
+                                                  const uint32_t* __restrict__ list, const uint32_t* __restrict__ lpre,
+                                                  uint32_t e0, uint32_t ecnt, uint32_t pbase, uint32_t p_lo,
+                                                  uint32_t p_hi, bool single, uint32_t lreg, const BinSplat& b,
+                                                  unsigned long long* __restrict__ fb, uint32_t pbias,
+                                                  unsigned long long* s_fb, uint32_t* s_off, uint32_t* s_slot0) {
+    const int tid = threadIdx.x;
+    const uint32_t words = bin_region_words(b.g);
+    const uint32_t tile_words = GPS_TILE_WORDS * (uint32_t)b.g.spp;
+    const int tiles_rows = (b.g.up_h + GPS_TILE_SIZE - 1) >> GPS_TILE_BITS;
+    const int rxi = (int)(lreg % (uint32_t)b.g.nrx), ryi = (int)(lreg / (uint32_t)b.g.nrx);
+    const int rx0 = rxi << b.g.rw_shift, ry0 = ryi << b.g.rh_shift;
+    const int tx0 = rx0 >> GPS_TILE_BITS, ty0 = ry0 >> GPS_TILE_BITS;
+    uint32_t merged = 0;
+    // region words <- framebuffer (single job) or background (merged later)
+    for (uint32_t w = (uint32_t)tid * 2u; w < words; w += 2u * GPS_BIN_THREADS) {
+        const uint32_t t = w / tile_words, within = w - t * tile_words;
+        const int tx = tx0 + (int)(t % (uint32_t)b.g.rtx), ty = ty0 + (int)(t / (uint32_t)b.g.rtx);
+        uint4 v = make_uint4((uint32_t)b.background, (uint32_t)(b.background >> 32), (uint32_t)b.background,
+                             (uint32_t)(b.background >> 32));
+        if (single && tx < b.g.tiles_per_row && ty < tiles_rows)
+            v = *reinterpret_cast<const uint4*>(fb + (uint64_t)(ty * b.g.tiles_per_row + tx) * tile_words + within);
+        s_fb[w] = ((unsigned long long)v.y << 32) | v.x;
+        s_fb[w + 1] = ((unsigned long long)v.w << 32) | v.z;
+    }
+    for (uint32_t q0 = p_lo; q0 < p_hi; q0 += GPS_BIN_CHUNK) {
+        // last slot whose region-local prefix is <= q0
+        if (tid == 0) {
+            uint32_t lo = 0, hi = ecnt - 1u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (lpre[e0 + mid] - pbase <= q0) lo = mid;
+                else hi = mid - 1u;
+            }
+            *s_slot0 = lo;
+        }
+        __syncthreads();
+        const uint32_t j0 = *s_slot0;
+        for (int j = tid; j <= GPS_BIN_CHUNK; j += GPS_BIN_THREADS)
+            s_off[j] = j0 + (uint32_t)j < ecnt ? lpre[e0 + j0 + (uint32_t)j] - pbase : 0xFFFFFFFFu;
+        __syncthreads();
+        const uint32_t qa = q0 + (uint32_t)tid * GPS_BIN_PPT;
+        if (qa < p_hi) {
+            int lo = 0, hi = GPS_BIN_CHUNK;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= qa) lo = mid;
+                else hi = mid - 1;
+            }
+            int j = lo;
+            uint32_t cur = 0xFFFFFFFFu, pstart = 0;
+            SplatRecord rec;
+            for (int k = 0; k < GPS_BIN_PPT; ++k) {
+                const uint32_t q = qa + (uint32_t)k;
+                if (q >= p_hi) break;
+                while (j < GPS_BIN_CHUNK - 1 && s_off[j + 1] <= q) ++j;
+                const uint32_t slot = j0 + (uint32_t)j;
+                if (slot >= ecnt) break;
+                if (slot != cur) {
+                    cur = slot;
+                    const uint32_t r = list[e0 + slot];
+                    pstart = coff[r] - pbias;
+                    rec = load_record(recs, cidx[r]);
+                }
+                splat_point_region(rec, pstart + (q - s_off[j]), b, rx0, ry0, s_fb);
+            }
+        }
+        __syncthreads();
+    }
+    // region -> framebuffer
+    for (uint32_t w = (uint32_t)tid * 2u; w < words; w += 2u * GPS_BIN_THREADS) {
+        const uint32_t t = w / tile_words, within = w - t * tile_words;
+        const int tx = tx0 + (int)(t % (uint32_t)b.g.rtx), ty = ty0 + (int)(t / (uint32_t)b.g.rtx);
+        if (tx >= b.g.tiles_per_row || ty >= tiles_rows) continue;
+        unsigned long long* dst = fb + (uint64_t)(ty * b.g.tiles_per_row + tx) * tile_words + within;
+        const unsigned long long a0 = s_fb[w], a1 = s_fb[w + 1];
+        if (single) {
+            *reinterpret_cast<uint4*>(dst) = make_uint4((uint32_t)a0, (uint32_t)(a0 >> 32), (uint32_t)a1,
+                                                        (uint32_t)(a1 >> 32));
+        } else {
+            if (a0 != b.background) {
+                atomicMin(dst, a0);
+                ++merged;
+            }
+            if (a1 != b.background) {
+                atomicMin(dst + 1, a1);
+                ++merged;
+            }
+        }
+    }
+    __syncthreads();
+    return merged;
+}
+
 // Persistent, one 1024-thread workgroup per CU (the region buffer is most of
 // the LDS); jobs are taken from a device counter.
 __global__ __launch_bounds__(GPS_BIN_THREADS) void bin_splat_kernel(
@@ -1102,103 +1227,18 @@ __global__ __launch_bounds__(GPS_BIN_THREADS) void bin_splat_kernel(
     __shared__ uint32_t s_off[GPS_BIN_CHUNK + 1];
     __shared__ uint32_t s_job, s_slot0;
     if (state[3]) return;
-    const int tid = threadIdx.x;
     const uint32_t njobs = state[1];
-    const uint32_t words = bin_region_words(b.g);
-    const uint32_t tile_words = GPS_TILE_WORDS * (uint32_t)b.g.spp;
-    const int tiles_rows = (b.g.up_h + GPS_TILE_SIZE - 1) >> GPS_TILE_BITS;
     uint32_t merged = 0;
     for (;;) {
-        if (tid == 0) s_job = atomicAdd(&state[4], 1u);
+        if (threadIdx.x == 0) s_job = atomicAdd(&state[4], 1u);
         __syncthreads();
         const uint32_t job = s_job;
         if (job >= njobs) break;
         const uint32_t reg = job_reg[job];
-        const uint32_t ptot = reg_ptot[reg];
         const uint32_t p_lo = job_p0[job];
-        const uint32_t p_hi = min(ptot, p_lo + job_points);
-        const bool single = reg_jobs[reg] == 1u;
-        const int rxi = (int)(reg % (uint32_t)b.g.nrx), ryi = (int)(reg / (uint32_t)b.g.nrx);
-        const int rx0 = rxi << b.g.rw_shift, ry0 = ryi << b.g.rh_shift;
-        const int tx0 = rx0 >> GPS_TILE_BITS, ty0 = ry0 >> GPS_TILE_BITS;
-        // region words <- framebuffer (single job) or background (merged later)
-        for (uint32_t w = (uint32_t)tid * 2u; w < words; w += 2u * GPS_BIN_THREADS) {
-            const uint32_t t = w / tile_words, within = w - t * tile_words;
-            const int tx = tx0 + (int)(t % (uint32_t)b.g.rtx), ty = ty0 + (int)(t / (uint32_t)b.g.rtx);
-            uint4 v = make_uint4((uint32_t)b.background, (uint32_t)(b.background >> 32), (uint32_t)b.background,
-                                 (uint32_t)(b.background >> 32));
-            if (single && tx < b.g.tiles_per_row && ty < tiles_rows)
-                v = *reinterpret_cast<const uint4*>(fb + (uint64_t)(ty * b.g.tiles_per_row + tx) * tile_words + within);
-            s_fb[w] = ((unsigned long long)v.y << 32) | v.x;
-            s_fb[w + 1] = ((unsigned long long)v.w << 32) | v.z;
-        }
-        const uint32_t e0 = reg_eoff[reg], ecnt = reg_ecnt[reg], pbase = reg_poff[reg];
-        for (uint32_t q0 = p_lo; q0 < p_hi; q0 += GPS_BIN_CHUNK) {
-            // last slot whose region-local prefix is <= q0
-            if (tid == 0) {
-                uint32_t lo = 0, hi = ecnt - 1u;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi + 1u) >> 1;
-                    if (lpre[e0 + mid] - pbase <= q0) lo = mid;
-                    else hi = mid - 1u;
-                }
-                s_slot0 = lo;
-            }
-            __syncthreads();
-            const uint32_t j0 = s_slot0;
-            for (int j = tid; j <= GPS_BIN_CHUNK; j += GPS_BIN_THREADS)
-                s_off[j] = j0 + (uint32_t)j < ecnt ? lpre[e0 + j0 + (uint32_t)j] - pbase : 0xFFFFFFFFu;
-            __syncthreads();
-            const uint32_t qa = q0 + (uint32_t)tid * GPS_BIN_PPT;
-            if (qa < p_hi) {
-                int lo = 0, hi = GPS_BIN_CHUNK;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_off[mid] <= qa) lo = mid;
-                    else hi = mid - 1;
-                }
-                int j = lo;
-                uint32_t cur = 0xFFFFFFFFu, pstart = 0;
-                SplatRecord rec;
-                for (int k = 0; k < GPS_BIN_PPT; ++k) {
-                    const uint32_t q = qa + (uint32_t)k;
-                    if (q >= p_hi) break;
-                    while (j < GPS_BIN_CHUNK - 1 && s_off[j + 1] <= q) ++j;
-                    const uint32_t slot = j0 + (uint32_t)j;
-                    if (slot >= ecnt) break;
-                    if (slot != cur) {
-                        cur = slot;
-                        const uint32_t r = list[e0 + slot];
-                        pstart = coff[r];
-                        rec = load_record(recs, cidx[r]);
-                    }
-                    splat_point_region(rec, pstart + (q - s_off[j]), b, rx0, ry0, s_fb);
-                }
-            }
-            __syncthreads();
-        }
-        // region -> framebuffer
-        for (uint32_t w = (uint32_t)tid * 2u; w < words; w += 2u * GPS_BIN_THREADS) {
-            const uint32_t t = w / tile_words, within = w - t * tile_words;
-            const int tx = tx0 + (int)(t % (uint32_t)b.g.rtx), ty = ty0 + (int)(t / (uint32_t)b.g.rtx);
-            if (tx >= b.g.tiles_per_row || ty >= tiles_rows) continue;
-            unsigned long long* dst = fb + (uint64_t)(ty * b.g.tiles_per_row + tx) * tile_words + within;
-            const unsigned long long a0 = s_fb[w], a1 = s_fb[w + 1];
-            if (single) {
-                *reinterpret_cast<uint4*>(dst) = make_uint4((uint32_t)a0, (uint32_t)(a0 >> 32), (uint32_t)a1,
-                                                            (uint32_t)(a1 >> 32));
-            } else {
-                if (a0 != b.background) {
-                    atomicMin(dst, a0);
-                    ++merged;
-                }
-                if (a1 != b.background) {
-                    atomicMin(dst + 1, a1);
-                    ++merged;
-                }
-            }
-        }
-        __syncthreads();
+        const uint32_t p_hi = min(reg_ptot[reg], p_lo + job_points);
+        merged += bin_splat_job(recs, cidx, coff, list, lpre, reg_eoff[reg], reg_ecnt[reg], reg_poff[reg], p_lo, p_hi,
+                                reg_jobs[reg] == 1u, reg, b, fb, 0u, s_fb, s_off, &s_slot0);
     }
     if (atomics_count) add_count(atomics_count, merged);
 }
@@ -1231,6 +1271,133 @@ __global__ __launch_bounds__(256) void bin_big_kernel(const SplatRecord* __restr
     if (atomics_count) add_count(atomics_count, issued);
 }
 
+// ------------------------------------------------------------------ binned splat, batches
+// The regions of all binned views of a batch are numbered consecutively
+// (view v owns [rbase(v), rbase(v) + nregions(v))), so the region scan, the
+// list scans and the job queue are the single-frame kernels over the whole
+// batch; counting and scattering take a grid row per view (grid.y = view)
+// with the view's own histogram in LDS, and a job finds its view by its
+// region.  Views whose bg.nregions is 0 are left to the point-parallel batch
+// splat.
+
+__global__ __launch_bounds__(256) void batch_bin_count_kernel(const SplatRecord* __restrict__ recs,
+                                                              const uint32_t* __restrict__ cidx,
+                                                              const uint32_t* __restrict__ coff,
+                                                              const BatchView* __restrict__ views,
+                                                              uint2* __restrict__ erect, uint32_t* __restrict__ ew,
+                                                              uint32_t* __restrict__ reg_cnt,
+                                                              uint32_t* __restrict__ reg_pts,
+                                                              uint32_t* __restrict__ big, uint32_t* __restrict__ state) {
+    extern __shared__ uint32_t s_dyn[];  // 2 * the largest view's regions
+    const BatchView& bv = views[blockIdx.y];
+    const BinGeom g = bv.bg;
+    if (g.nregions == 0u) return;
+    const uint32_t V = bv.valid, eb = bv.ebase;
+    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = eb + min(V, blockIdx.x * per), r1 = eb + min(V, blockIdx.x * per + per);
+    bin_count_range(recs, cidx, coff, g, r0, r1, eb + V, bv.base + bv.total, bv.base + bv.points, bv.rbase, erect, ew,
+                    reg_cnt, reg_pts, big, state, s_dyn, s_dyn + g.nregions);
+}
+
+__global__ __launch_bounds__(256) void batch_bin_scatter_kernel(const BatchView* __restrict__ views,
+                                                                const uint2* __restrict__ erect,
+                                                                const uint32_t* __restrict__ ew,
+                                                                uint32_t* __restrict__ reg_cursor,
+                                                                uint32_t* __restrict__ list, uint32_t* __restrict__ lw,
+                                                                const uint32_t* __restrict__ state) {
+    extern __shared__ uint32_t s_dyn[];
+    if (state[3]) return;
+    const BatchView& bv = views[blockIdx.y];
+    const BinGeom g = bv.bg;
+    if (g.nregions == 0u) return;
+    const uint32_t V = bv.valid, eb = bv.ebase;
+    const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = eb + min(V, blockIdx.x * per), r1 = eb + min(V, blockIdx.x * per + per);
+    bin_scatter_range(g, r0, r1, bv.rbase, erect, ew, reg_cursor, list, lw, s_dyn, s_dyn + g.nregions);
+}
+
+// The view that owns global region `reg`: the last view whose rbase <= reg
+// (views without regions share the next binned view's rbase).
+__device__ __forceinline__ int batch_region_view(const BatchView* __restrict__ views, int nviews, uint32_t reg) {
+    int lo = 0, hi = nviews - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].rbase <= reg) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(GPS_BIN_THREADS) void batch_bin_splat_kernel(
+    const SplatRecord* __restrict__ recs, const uint32_t* __restrict__ cidx, const uint32_t* __restrict__ coff,
+    const uint32_t* __restrict__ list, const uint32_t* __restrict__ lpre, const uint32_t* __restrict__ reg_ecnt,
+    const uint32_t* __restrict__ reg_ptot, const uint32_t* __restrict__ reg_eoff, const uint32_t* __restrict__ reg_poff,
+    const uint32_t* __restrict__ reg_jobs, const uint32_t* __restrict__ job_reg, const uint32_t* __restrict__ job_p0,
+    uint32_t* __restrict__ state, const BatchView* __restrict__ views, int nviews,
+    unsigned long long* __restrict__ fb, uint32_t job_points, uint32_t* __restrict__ atomics_count) {
+    __shared__ unsigned long long s_fb[GPS_BIN_WORDS];
+    __shared__ uint32_t s_off[GPS_BIN_CHUNK + 1];
+    __shared__ uint32_t s_job, s_slot0;
+    __shared__ int s_view;
+    if (state[3]) return;
+    const uint32_t njobs = state[1];
+    uint32_t merged = 0;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            s_job = atomicAdd(&state[4], 1u);
+            s_view = s_job < njobs ? batch_region_view(views, nviews, job_reg[s_job]) : 0;
+        }
+        __syncthreads();
+        const uint32_t job = s_job;
+        if (job >= njobs) break;
+        const BatchView& bv = views[s_view];
+        BinSplat b;
+        b.g = bv.bg;
+        b.K = bv.f.Ki;
+        b.unbiased = bv.f.unbiased;
+        b.frame_seed = bv.f.frame_seed + 1u;
+        b.background = bv.background;
+        const uint32_t reg = job_reg[job];
+        const uint32_t p_lo = job_p0[job];
+        const uint32_t p_hi = min(reg_ptot[reg], p_lo + job_points);
+        merged += bin_splat_job(recs, cidx, coff, list, lpre, reg_eoff[reg], reg_ecnt[reg], reg_poff[reg], p_lo, p_hi,
+                                reg_jobs[reg] == 1u, reg - bv.rbase, b, fb + bv.fb_offset, bv.base, s_fb, s_off,
+                                &s_slot0);
+    }
+    if (atomics_count) add_count(atomics_count, merged);
+}
+
+__global__ __launch_bounds__(256) void batch_bin_big_kernel(const SplatRecord* __restrict__ recs,
+                                                            const uint32_t* __restrict__ cidx,
+                                                            const uint32_t* __restrict__ coff,
+                                                            const uint8_t* __restrict__ cview,
+                                                            const uint32_t* __restrict__ ew,
+                                                            const uint32_t* __restrict__ big,
+                                                            uint32_t* __restrict__ state,
+                                                            const BatchView* __restrict__ views,
+                                                            unsigned long long* __restrict__ fb,
+                                                            uint32_t* __restrict__ atomics_count) {
+    __shared__ uint32_t s_e;
+    if (state[3]) return;
+    const int tid = threadIdx.x;
+    const uint32_t nbig = state[2];
+    uint32_t issued = 0;
+    for (;;) {
+        if (tid == 0) s_e = atomicAdd(&state[5], 1u);
+        __syncthreads();
+        const uint32_t e = s_e;
+        __syncthreads();
+        if (e >= nbig) break;
+        const uint32_t r = big[e];
+        const BatchView& bv = views[cview[r]];
+        const SplatRecord rec = load_record(recs, cidx[r]);
+        const uint32_t p0 = coff[r] - bv.base, w = ew[r];
+        const uint32_t seed = bv.f.frame_seed + 1u;
+        for (uint32_t k = (uint32_t)tid; k < w; k += 256u)
+            issued += splat_point(rec, p0 + k, seed, bv.f.Ki, bv.f.spp, bv.f.up_w, bv.f.unbiased, fb + bv.fb_offset);
+    }
+    if (atomics_count) add_count(atomics_count, issued);
+}
 // ------------------------------------------------------------------ combine
 // One workgroup = 16x16 output pixels; lanes follow Morton order inside the
 // block so a wave's framebuffer words are one contiguous range when ss is a
@@ -1667,11 +1834,13 @@ __global__ __launch_bounds__(256) void batch_views_kernel(BatchView* __restrict_
     views[v].total = end - base;
     views[v].points = min(end - base, views[v].budget);
     views[v].valid = vend - vbase;
+    views[v].ebase = vbase;
 }
 
 // Persistent splat over the points of all views (same schedule as
 // splat_kernel, THREADS x PPT points per chunk).  Point p of view v is point
-// p - base(v) of that view's own frame.
+// p - base(v) of that view's own frame.  Views with binned regions are
+// skipped unless the binned kernels stood down (bin_state[3], list overflow).
 template <int THREADS, int PPT>
 __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord* __restrict__ recs,
                                                               const uint32_t* __restrict__ cidx,
@@ -1681,8 +1850,10 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
                                                               const PassCounters* __restrict__ ctr,
                                                               const BatchView* __restrict__ views,
                                                               unsigned long long* __restrict__ fb, int32_t xcds,
-                                                              uint32_t* __restrict__ atomics_count) {
+                                                              uint32_t* __restrict__ atomics_count,
+                                                              const uint32_t* __restrict__ bin_state) {
     constexpr int CHUNK = THREADS * PPT;
+    const bool bin_ran = bin_state && bin_state[3] == 0u;
     __shared__ uint32_t s_off[CHUNK + 1];
     const uint32_t P = ctr->points;
     const uint32_t V = ctr->valid;
@@ -1715,6 +1886,7 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
             SplatRecord rec;
             uint32_t vbase = 0, vpoints = 0, vseed = 0;
             int vK = 1, vspp = 1, vup_w = 1, vunb = 1;
+            bool vskip = false;
             uint64_t voff = 0;
             for (int k = 0; k < PPT; ++k) {
                 const uint32_t p = p0 + (uint32_t)k;
@@ -1725,7 +1897,6 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
                 const uint32_t g = cidx[r];
                 if (g != cur_g) {
                     cur_g = g;
-                    rec = load_record(recs, g);
                     const int v = cview[r];
                     if (v != cur_v) {
                         cur_v = v;
@@ -1738,8 +1909,11 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
                         vup_w = bv.f.up_w;
                         vunb = bv.f.unbiased;
                         voff = bv.fb_offset;
+                        vskip = bin_ran && bv.bg.nregions != 0u;
                     }
+                    if (!vskip) rec = load_record(recs, g);
                 }
+                if (vskip) continue;  // rendered by the binned kernels
                 const uint32_t pl = p - vbase;
                 if (pl >= vpoints) continue;  // beyond the view's point budget
                 issued += splat_point(rec, pl, vseed, vK, vspp, vup_w, vunb, fb + voff);
@@ -1940,7 +2114,8 @@ template <int THREADS, int PPT> static void launch_batch_splat_t(const BatchArgs
     hipLaunchKernelGGL((batch_splat_kernel<THREADS, PPT>), dim3(a.grid), dim3(THREADS), 0, st,
                        (const SplatRecord*)a.recs, (const uint32_t*)a.cidx, (const uint32_t*)a.coff,
                        (const uint8_t*)a.cview, (const uint32_t*)a.chunk_start, (const PassCounters*)a.ctr,
-                       (const BatchView*)a.views, reinterpret_cast<unsigned long long*>(a.fb), a.xcds, a.atomics);
+                       (const BatchView*)a.views, reinterpret_cast<unsigned long long*>(a.fb), a.xcds, a.atomics,
+                       a.bin.any ? (const uint32_t*)a.bin.state : nullptr);
 }
 
 // Splat variants the batch autotuner chooses from: (threads, points per thread).
@@ -1957,6 +2132,48 @@ bool launch_batch_splat(const BatchArgs& a, hipStream_t st) {
     else if (t == 512 && q == 4) launch_batch_splat_t<512, 4>(a, st);
     else return false;
     return true;
+}
+
+void launch_batch_binned(const BatchArgs& a, hipStream_t st) {
+    const BatchBinArgs& b = a.bin;
+    if (!b.any) return;
+    launch_zero_u32(b.state, 8, st);
+    uint32_t gx = (b.cus * 4u + (uint32_t)a.nviews - 1u) / (uint32_t)a.nviews;  // workgroups per view
+    if (gx < 2u) gx = 2u;
+    const size_t hist = 2u * b.max_view_regions * sizeof(uint32_t);
+    hipLaunchKernelGGL(batch_bin_count_kernel, dim3(gx, a.nviews), dim3(256), hist, st, (const SplatRecord*)a.recs,
+                       (const uint32_t*)a.cidx, (const uint32_t*)a.coff, (const BatchView*)a.views, b.erect, b.ew,
+                       b.reg_cnt, b.reg_pts, b.big, b.state);
+    BinGeom all{};
+    all.nregions = b.nregions;
+    hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, all, b.reg_cnt, b.reg_pts, b.reg_ecnt, b.reg_ptot,
+                       b.reg_eoff, b.reg_poff, b.reg_cursor, b.reg_jobs, b.job_reg, b.job_p0, b.state, b.list_cap,
+                       b.job_points, b.job_cap);
+    hipLaunchKernelGGL(batch_bin_scatter_kernel, dim3(gx, a.nviews), dim3(256), hist, st, (const BatchView*)a.views,
+                       (const uint2*)b.erect, (const uint32_t*)b.ew, b.reg_cursor, b.list, b.lw,
+                       (const uint32_t*)b.state);
+    const uint32_t grid = b.cus * 4u;
+    hipLaunchKernelGGL(bin_tile_sum_kernel, dim3(grid), dim3(256), 0, st, (const uint32_t*)b.lw,
+                       (const uint32_t*)b.state, b.tsum);
+    hipLaunchKernelGGL(bin_tiles_scan_kernel, dim3(1), dim3(1024), 0, st, (const uint32_t*)b.tsum,
+                       (const uint32_t*)b.state, b.toff);
+    hipLaunchKernelGGL(bin_local_scan_kernel, dim3(grid), dim3(256), 0, st, (const uint32_t*)b.lw,
+                       (const uint32_t*)b.state, (const uint32_t*)b.toff, b.lpre);
+    hipLaunchKernelGGL(batch_bin_splat_kernel, dim3(b.cus), dim3(GPS_BIN_THREADS), 0, st, (const SplatRecord*)a.recs,
+                       (const uint32_t*)a.cidx, (const uint32_t*)a.coff, (const uint32_t*)b.list,
+                       (const uint32_t*)b.lpre, (const uint32_t*)b.reg_ecnt, (const uint32_t*)b.reg_ptot,
+                       (const uint32_t*)b.reg_eoff, (const uint32_t*)b.reg_poff, (const uint32_t*)b.reg_jobs,
+                       (const uint32_t*)b.job_reg, (const uint32_t*)b.job_p0, b.state, (const BatchView*)a.views,
+                       a.nviews, reinterpret_cast<unsigned long long*>(a.fb), b.job_points, a.atomics);
+}
+
+void launch_batch_bin_big(const BatchArgs& a, hipStream_t st) {
+    const BatchBinArgs& b = a.bin;
+    if (!b.any) return;
+    hipLaunchKernelGGL(batch_bin_big_kernel, dim3(b.cus * 4u), dim3(256), 0, st, (const SplatRecord*)a.recs,
+                       (const uint32_t*)a.cidx, (const uint32_t*)a.coff, (const uint8_t*)a.cview,
+                       (const uint32_t*)b.ew, (const uint32_t*)b.big, b.state, (const BatchView*)a.views,
+                       reinterpret_cast<unsigned long long*>(a.fb), a.atomics);
 }
 
 // ------------------------------------------------------------------ binned splat launcher

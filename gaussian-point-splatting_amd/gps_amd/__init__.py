@@ -750,6 +750,7 @@ _sig("gps_batch_render", C.c_int32, _vp, C.c_int32, C.POINTER(FrameParams), C.PO
 _sig("gps_batch_autotune", C.c_int32, _vp, C.c_int32, C.POINTER(FrameParams), C.POINTER(_vp), C.POINTER(_vp), _vp,
      C.c_int32, C.POINTER(BatchTuning))
 _sig("gps_batch_set_tuning", C.c_int32, _vp, C.POINTER(BatchTuning))
+_sig("gps_batch_set_splat_mode", C.c_int32, _vp, C.c_int32)
 _sig("gps_batch_view_stats", C.c_int32, _vp, C.c_int32, C.POINTER(Stats), _vp)
 _sig("gps_batch_device_bytes", C.c_uint64, _vp)
 _sig("gps_batch_destroy", None, _vp)
@@ -793,6 +794,10 @@ class Batch:
         t = BatchTuning()
         t.splat_threads, t.splat_points_per_thread, t.use_graph = threads, points_per_thread, int(graph)
         check(_lib.gps_batch_set_tuning(self.handle, C.byref(t)), "batch tuning")
+
+    def set_splat_mode(self, mode: int) -> None:
+        """Pass.SPLAT_AUTO / SPLAT_POINT_PARALLEL / SPLAT_BINNED, per view."""
+        check(_lib.gps_batch_set_splat_mode(self.handle, mode), "batch splat mode")
 
     def view_stats(self, view: int, stream: Optional[Stream] = None) -> dict:
         s = Stats()

@@ -380,7 +380,16 @@ gps_status gps_batch_autotune(gps_batch* batch, int32_t num_views, const gps_fra
                               float* const* d_colors, float* const* d_depths, void* stream, int32_t repeats,
                               gps_batch_tuning* out);
 gps_status gps_batch_set_tuning(gps_batch* batch, const gps_batch_tuning* tuning);
-/* Counters of view `view` of the last batch (atomics: whole batch). */
+/* Splat form per view, with the rule and modes of gps_pass_set_splat_mode:
+ * the binned views of a batch share one set of binning launches (regions of
+ * all of them in one job queue), the rest use the point-parallel batch
+ * splat.  The binned buffers (16 B per compacted entry, 12 B per list slot,
+ * about 1.5 slots per entry) are allocated on the first batch that bins a
+ * view; if that allocation fails every view stays point-parallel.  No
+ * reference counterpart. */
+gps_status gps_batch_set_splat_mode(gps_batch* batch, int32_t mode);
+/* Counters of view `view` of the last batch (atomics, bin_entries, bin_jobs,
+ * bin_big: whole batch). */
 gps_status gps_batch_view_stats(gps_batch* batch, int32_t view, gps_stats* out, void* stream);
 uint64_t gps_batch_device_bytes(const gps_batch* batch);
 void gps_batch_destroy(gps_batch* batch);

@@ -446,6 +446,60 @@ def test_batch_views_identical_to_single_renders(gpu):
         check_batch(fr, cols, deps, r, f"graph {k}")
 
 
+@pytest.mark.parametrize("case", ["mixed", "closeup_big", "budget"])
+def test_batch_binned_views_identical(gpu, case):
+    """Binned batch views (the regions of all binned views in one job queue,
+    counting and scattering per view, jobs that find their view by region)
+    give every view's colour and depth bit for bit as a pass rendering the
+    frame alone, with the binned form forced, automatic (K * spp >= 8) and
+    off; with Gaussians too large to bin and with a point budget that cuts
+    views short."""
+    cfg = dict(H.C1, half_extent=6.0, log_scale_mu=-2.0) if case != "closeup_big" else dict(H.C1, log_scale_mu=-0.7)
+    arrays = H.scene(0x5EED0701 + len(case), 30_000, cfg).morton_sorted()
+    scene = gpu.Scene(arrays, gpu.FORMAT_FP32)
+    budget = 6_000 if case == "budget" else 250_000_000
+    stream = gpu.Stream()
+    batch = gpu.Batch(scene, 5, 200, 120, max_spp=4, max_supersampling=2, gaussians_per_block=64,
+                      max_number_of_points=budget)
+    single = gpu.Pass(gpu.RenderSettings(200, 120, 4, 2, flexible_frame_size=True, max_number_of_points=budget,
+                                         gaussians_per_block=64), scene)
+    # (width, height, spp, ss): K * spp = 4, 4, 16, 3, 8
+    shapes = [(200, 120, 1, 2), (160, 90, 4, 1), (200, 120, 4, 2), (96, 64, 3, 1), (150, 100, 2, 2)]
+    eye = (0.0, 2.0, -9.0) if case == "closeup_big" else (0.0, 3.0, -25.0)
+    frames = [gpu.frame_params((eye[0] + 0.7 * v, eye[1] - 0.3 * v, eye[2]), (0, 0, 0), (0, 1, 0), 45.0, w, h,
+                               ss=ss, frame=v + 3, hierarchical=(v != 1), spp=spp)
+              for v, (w, h, spp, ss) in enumerate(shapes)]
+    ref = []
+    for f in frames:
+        c = gpu.DeviceArray((f.height, f.width, 3), np.float32)
+        d = gpu.DeviceArray((f.height, f.width), np.float32)
+        single.render(f, c, d, stream)
+        stream.synchronize()
+        ref.append((c.download(), d.download(), single.stats(stream)))
+    cols = [gpu.DeviceArray((f.height, f.width, 3), np.float32, zero=True) for f in frames]
+    deps = [gpu.DeviceArray((f.height, f.width), np.float32, zero=True) for f in frames]
+    for mode, name in ((gpu.Pass.SPLAT_BINNED, "binned"), (gpu.Pass.SPLAT_AUTO, "auto"),
+                       (gpu.Pass.SPLAT_POINT_PARALLEL, "point-parallel")):
+        flush(f"batch binned {case}: {name}")
+        batch.set_splat_mode(mode)
+        batch.render(frames, cols, deps, stream)
+        stream.synchronize()
+        for v, (f, (w, h, spp, ss)) in enumerate(zip(frames, shapes)):
+            assert_same(f"{case} {name} view {v} colour", cols[v].download(), ref[v][0])
+            assert_same(f"{case} {name} view {v} depth", deps[v].download(), ref[v][1])
+            st = batch.view_stats(v, stream)
+            assert st["num_points"][0] == ref[v][2]["num_points"][0], (case, name, v)
+            binned = {"binned": True, "auto": ss * ss * spp >= 8, "point-parallel": False}[name]
+            assert st["splat_binned"] == int(binned) and st["splat_fallback"] == 0, (case, name, v, st)
+        if name == "binned":
+            st = batch.view_stats(0, stream)
+            flush(f"{case}: entries {st['bin_entries']} jobs {st['bin_jobs']} big {st['bin_big']}")
+            if case == "closeup_big":
+                assert st["bin_big"] > 0
+    if case == "budget":
+        assert any(r[2]["budget_truncated"] for r in ref)
+
+
 # ------------------------------------------------------------------ binned splat
 @pytest.mark.parametrize("case", ["spp1_ss1", "spp2_ss2", "spp3_ss1", "spp4_ss2", "closeup_big", "budget",
                                   "occlusion_two_pass"])
