@@ -57,6 +57,8 @@ uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 
 uint16_t gps_float_to_half_rne(float f);
 void gps_encode_sh_record(const float* sh, int ncoef, uint8_t out[64]);
+void gps_encode_quantized_one(const gps_scene_desc* d, const gps_quantized_bounds* b, uint64_t g, uint32_t geo[4],
+                              uint8_t* opacity, uint8_t rec[64]);
 
 namespace {
 
@@ -238,7 +240,8 @@ gps_status gps_scene_create_empty(uint64_t N, int32_t sh_degree, int32_t format,
     if (!out) return GPS_ERR_INVALID_ARG;
     *out = nullptr;
     if (sh_degree < 0 || sh_degree > 3) return GPS_ERR_INVALID_ARG;
-    if (format != GPS_FORMAT_FP32 && format != GPS_FORMAT_HALF_FP8) return GPS_ERR_INVALID_ARG;
+    if (format != GPS_FORMAT_FP32 && format != GPS_FORMAT_HALF_FP8 && format != GPS_FORMAT_QUANTIZED)
+        return GPS_ERR_INVALID_ARG;
     if (N >= (1ull << 32) - GPS_TILE) return GPS_ERR_UNSUPPORTED;
     const int ncoef = (sh_degree + 1) * (sh_degree + 1);
     const uint64_t stride = round_up(std::max<uint64_t>(N, 1), 64);
@@ -255,6 +258,24 @@ gps_status gps_scene_create_empty(uint64_t N, int32_t sh_degree, int32_t format,
         s->planes = planes;
         s->view.planes = planes;
         s->bytes = stride * nplanes * sizeof(float);
+    } else if (format == GPS_FORMAT_QUANTIZED) {
+        uint4* geo = nullptr;
+        uint4* shrec = nullptr;
+        uint8_t* qop = nullptr;
+        float* qb = nullptr;
+        st = dalloc(&geo, stride, true);
+        if (st == GPS_OK) st = dalloc(&shrec, 4 * stride, true);
+        if (st == GPS_OK) st = dalloc(&qop, stride, true);
+        if (st == GPS_OK) st = dalloc(&qb, sizeof(gps_quantized_bounds) / sizeof(float), true);
+        s->geo = geo;
+        s->shrec = shrec;
+        s->qop = qop;
+        s->qbounds = qb;
+        s->view.geo = geo;
+        s->view.shrec = shrec;
+        s->view.qop = qop;
+        s->view.qbounds = qb;
+        s->bytes = stride * (16 + 64 + 1) + sizeof(gps_quantized_bounds);
     } else {
         float* means = nullptr;
         uint4* geo = nullptr;
@@ -310,6 +331,27 @@ gps_status gps_scene_upload_range(gps_scene* s, uint64_t first, const gps_scene_
                 if (e != hipSuccess) return hip_fail(e, "scene upload");
             }
         }
+    } else if (d->format == GPS_FORMAT_QUANTIZED) {
+        // The reference's compressed scene, encoded on the host against the
+        // scene box set beforehand.
+        if (!s->have_bounds) return GPS_ERR_INVALID_ARG;
+        uint4* geo = (uint4*)s->geo;
+        uint4* shrec = (uint4*)s->shrec;
+        uint8_t* qop = (uint8_t*)s->qop;
+        std::vector<uint32_t> gstage(4 * chunk), sstage(16 * chunk);
+        std::vector<uint8_t> ostage(chunk);
+        for (uint64_t b = 0; b < n_all; b += chunk) {
+            const uint64_t n = std::min(chunk, n_all - b);
+            parallel_range(n, [&](uint64_t lo, uint64_t hi) {
+                for (uint64_t i = lo; i < hi; ++i)
+                    gps_encode_quantized_one(d, &s->bounds, b + i, &gstage[4 * i], &ostage[i],
+                                             reinterpret_cast<uint8_t*>(&sstage[16 * i]));
+            });
+            hipError_t e = hipMemcpy(geo + first + b, gstage.data(), n * 16, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(shrec + 4 * (first + b), sstage.data(), n * 64, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(qop + first + b, ostage.data(), n, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "scene upload");
+        }
     } else {
         // GPS_FORMAT_HALF_FP8: encoded on the host (fp16 RNE, fp8 E4M3 RNE with a
         // per-Gaussian scale = max|rest coefficient| / 448).
@@ -355,7 +397,12 @@ gps_status gps_scene_create(const gps_scene_desc* d, gps_scene** out) {
     gps_scene* s = nullptr;
     gps_status st = gps_scene_create_empty(d->num_gaussians, d->sh_degree, d->format, &s);
     if (st != GPS_OK) return st;
-    st = gps_scene_upload_range(s, 0, d);
+    if (d->format == GPS_FORMAT_QUANTIZED) {
+        gps_quantized_bounds b;
+        st = gps_quantized_bounds_compute(d, &b);
+        if (st == GPS_OK) st = gps_scene_set_quantized_bounds(s, &b);
+    }
+    if (st == GPS_OK) st = gps_scene_upload_range(s, 0, d);
     if (st != GPS_OK) {
         gps_scene_destroy(s);
         return st;
@@ -369,7 +416,18 @@ void gps_scene_destroy(gps_scene* s) {
     if (s->planes) (void)hipFree(s->planes);
     if (s->geo) (void)hipFree(s->geo);
     if (s->shrec) (void)hipFree(s->shrec);
+    if (s->qop) (void)hipFree(s->qop);
+    if (s->qbounds) (void)hipFree(s->qbounds);
     delete s;
+}
+
+gps_status gps_scene_set_quantized_bounds(gps_scene* s, const gps_quantized_bounds* b) {
+    if (!s || !b) return GPS_ERR_INVALID_ARG;
+    if (s->view.format != GPS_FORMAT_QUANTIZED) return GPS_ERR_INVALID_ARG;
+    s->bounds = *b;
+    s->have_bounds = true;
+    const hipError_t e = hipMemcpy(s->qbounds, b, sizeof(*b), hipMemcpyHostToDevice);
+    return e == hipSuccess ? GPS_OK : hip_fail(e, "scene bounds");
 }
 
 uint64_t gps_scene_device_bytes(const gps_scene* s) { return s ? s->bytes : 0; }

@@ -22,11 +22,18 @@ namespace gps {
 //   shrec: 64 bytes per Gaussian: 3 halves DC (rgb), 1 pad half, float scale,
 //          45 fp8-E4M3 rest coefficients (coefficient-major, rgb interleaved),
 //          3 pad bytes
+// GPS_FORMAT_QUANTIZED (the reference's compressed scene):
+//   geo:     uint4 per Gaussian = the 4 quantized geometry words
+//   qop:     one opacity byte per Gaussian
+//   shrec:   64 bytes per Gaussian: DC rgb fp32, 45 coefficient bytes, 7 pad
+//   qbounds: 102 floats = gps_quantized_bounds (scene box)
 struct DevScene {
     const float* planes;        // FP32 format
     const float* means;         // HALF_FP8 format
-    const uint4* geo;           // HALF_FP8 format
-    const uint4* shrec;         // HALF_FP8 format (4 x uint4 per Gaussian)
+    const uint4* geo;           // HALF_FP8 and QUANTIZED formats
+    const uint4* shrec;         // HALF_FP8 and QUANTIZED formats (4 x uint4 per Gaussian)
+    const uint8_t* qop;         // QUANTIZED format
+    const float* qbounds;       // QUANTIZED format
     uint64_t stride;            // plane stride in elements
     uint32_t N;
     int32_t sh_degree;
@@ -55,6 +62,7 @@ struct SplatRecord {
     uint32_t pad;
 };
 static_assert(sizeof(SplatRecord) == 64, "record must stay 64 bytes");
+static_assert(sizeof(gps_quantized_bounds) == 102 * sizeof(float), "scene box is read as 102 floats");
 
 // Hi-Z pyramid (coarsest level 0 first), src/core/rendering/occlusion/depth_mip_chain.cuh:10-71.
 #define GPS_HIZ_MAX_LEVELS 16
@@ -178,5 +186,9 @@ struct gps_scene {
     void* planes = nullptr;  // FP32 planes or HALF_FP8 means
     void* geo = nullptr;
     void* shrec = nullptr;
+    void* qop = nullptr;      // QUANTIZED opacity bytes
+    void* qbounds = nullptr;  // QUANTIZED scene box (device copy)
+    gps_quantized_bounds bounds{};
+    bool have_bounds = false;
     uint64_t bytes = 0;
 };

@@ -135,6 +135,81 @@ template <> __device__ __forceinline__ Geo load_geo<GPS_FORMAT_HALF_FP8>(const D
     return o;
 }
 
+// The reference's compressed scene (gaussian_types.cuh:323-374, 507-509):
+// position = mix(box min, box max, code / (2^bits - 1)); rotation =
+// quat_iem(2 * code / 1023 - 1) with the host path's arithmetic (:224-230:
+// 1 / sqrt, sin, cos), stored (.x .. .w) like every other rotation; scale =
+// exp(mix(log-scale box, code / 1023)); opacity = byte / 255.  glm::mix(x,
+// y, a) = x * (1 - a) + y * a.
+__device__ __forceinline__ float glm_mix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+template <> __device__ __forceinline__ Geo load_geo<GPS_FORMAT_QUANTIZED>(const DevScene& s, uint32_t g) {
+    const uint4 w = s.geo[g];
+    const float* b = s.qbounds;  // min_pos[3] max_pos[3] min_log_scale[3] max_log_scale[3] ...
+    const uint32_t px = w.x & 0x7FFFFFu;
+    const uint32_t py = ((w.x >> 23) & 0x1FFu) | ((w.y & 0x1FFFu) << 9);
+    const uint32_t pz = ((w.y >> 13) & 0x7FFFFu) | ((w.z & 0xFu) << 19);
+    Geo o;
+    o.mx = glm_mix(b[0], b[3], (float)px / 8388607.0f);
+    o.my = glm_mix(b[1], b[4], (float)py / 4194303.0f);
+    o.mz = glm_mix(b[2], b[5], (float)pz / 8388607.0f);
+    const uint32_t rx = (w.z >> 4) & 0x3FFu, ry = (w.z >> 14) & 0x3FFu;
+    const uint32_t rz = ((w.z >> 24) & 0xFFu) | ((w.w & 0x3u) << 8);
+    const float vx = ((float)rx / 1023.0f) * 2.0f - 1.0f;
+    const float vy = ((float)ry / 1023.0f) * 2.0f - 1.0f;
+    const float vz = ((float)rz / 1023.0f) * 2.0f - 1.0f;
+    const float d = (vx * vx + vy * vy) + vz * vz;
+    const float t = 1.0f / det_sqrt(d + 1e-8f);
+    const float a = (1.57079632679f * t) * d;
+    float sn, cs;
+    det_sincosf_inrange(a, &sn, &cs);  // a in [0, 2.73]
+    const float k = sn * t;
+    o.qw = k * vx;
+    o.qx = k * vy;
+    o.qy = k * vz;
+    o.qz = cs;
+    o.sx = det_expf(glm_mix(b[6], b[9], (float)((w.w >> 2) & 0x3FFu) / 1023.0f));
+    o.sy = det_expf(glm_mix(b[7], b[10], (float)((w.w >> 12) & 0x3FFu) / 1023.0f));
+    o.sz = det_expf(glm_mix(b[8], b[11], (float)((w.w >> 22) & 0x3FFu) / 1023.0f));
+    o.op = (float)s.qop[g] / 255.0f;
+    return o;
+}
+
+// Sloan's real SH basis to degree 3 at a unit direction
+// (eval_sh_sloan_l3, src/core/math/sh.cuh:55-96), same operation order.
+__device__ __forceinline__ void sh_sloan_l3(float x, float y, float z, float o[16]) {
+    const float z2 = z * z;
+    o[0] = 0.282094792f;
+    o[2] = 0.488602512f * z;
+    float a = z2 - 0.333333333f;
+    o[6] = 0.946174696f * a;
+    const float b = z * (a - 0.266666667f);
+    o[12] = 1.865881663f * b;
+    float c1 = x, s1 = y;
+    float d = -0.488602512f;
+    o[1] = s1 * d;
+    o[3] = c1 * d;
+    d = -1.092548431f * z;
+    o[5] = s1 * d;
+    o[7] = c1 * d;
+    a = z2 - 0.2f;
+    d = -2.285228997f * a;
+    o[11] = s1 * d;
+    o[13] = c1 * d;
+    const float c0 = x * c1 - y * s1;
+    const float s0 = y * c1 + x * s1;
+    d = 0.546274215f;
+    o[4] = s0 * d;
+    o[8] = c0 * d;
+    d = 1.445305721f * z;
+    o[10] = s0 * d;
+    o[14] = c0 * d;
+    c1 = x * c0 - y * s0;
+    s1 = y * c0 + x * s0;
+    d = -0.59004359f;
+    o[9] = s1 * d;
+    o[15] = c1 * d;
+}
+
 // SH colour of Gaussian g.
 template <int FORMAT, int DEG> struct ShEval;
 template <int DEG> struct ShEval<GPS_FORMAT_FP32, DEG> {
@@ -173,6 +248,42 @@ template <int DEG> struct ShEval<GPS_FORMAT_HALF_FP8, DEG> {
                 const uint32_t w = words[3 + (b >> 2)];    // rest bytes start at byte 12
                 return fp8_e4m3_to_float_hw(w, b & 3) * scale;
             });
+        }
+    }
+};
+
+// computeColorFromCompactSH (sh.cuh:257-273): all 16 coefficients whatever
+// the scene's degree, summed in order from 0; coefficient i > 0 of channel c
+// is mix(min_sh, max_sh, byte * (1/255)) (CompactSH::get, sh.cuh:41-47).
+template <int DEG> struct ShEval<GPS_FORMAT_QUANTIZED, DEG> {
+    __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
+        const uint4* r = s.shrec + 4 * (uint64_t)g;
+        uint32_t words[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = r[q];
+            words[4 * q] = v.x;
+            words[4 * q + 1] = v.y;
+            words[4 * q + 2] = v.z;
+            words[4 * q + 3] = v.w;
+        }
+        float e[16];
+        sh_sloan_l3(x, y, z, e);
+        const float* mn = s.qbounds + 12;
+        const float* mx = s.qbounds + 57;
+        const float inv255 = 1.0f / 255.0f;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            float acc = 0.0f;
+            acc = acc + e[0] * __builtin_bit_cast(float, words[ch]);
+#pragma unroll
+            for (int i = 1; i < 16; ++i) {
+                const int bi = 3 * (i - 1) + ch;  // code byte, after the 12 DC bytes
+                const uint32_t code = (words[3 + (bi >> 2)] >> (8 * (bi & 3))) & 0xFFu;
+                acc = acc + e[i] * glm_mix(mn[bi], mx[bi], (float)code * inv255);
+            }
+            acc = acc + 0.5f;
+            rgb[ch] = (acc < 0.0f) ? 0.0f : acc;
         }
     }
 };
@@ -1940,6 +2051,8 @@ void launch_preprocess(const PreprocessArgs& a, hipStream_t st) {
         else if (d == 1) launch_pre<GPS_FORMAT_FP32, 1>(a, st);
         else if (d == 2) launch_pre<GPS_FORMAT_FP32, 2>(a, st);
         else launch_pre<GPS_FORMAT_FP32, 3>(a, st);
+    } else if (a.scene.format == GPS_FORMAT_QUANTIZED) {
+        launch_pre<GPS_FORMAT_QUANTIZED, 3>(a, st);  // the Sloan basis always has 16 coefficients
     } else {
         if (d == 0) launch_pre<GPS_FORMAT_HALF_FP8, 0>(a, st);
         else if (d == 1) launch_pre<GPS_FORMAT_HALF_FP8, 1>(a, st);
@@ -2036,6 +2149,9 @@ void launch_build_nodes(const DevScene& s, uint32_t gpb, uint32_t nb, CullNode* 
     const uint32_t blocks = (nb + 255) / 256;
     if (s.format == GPS_FORMAT_FP32)
         hipLaunchKernelGGL(build_nodes_kernel<GPS_FORMAT_FP32>, dim3(blocks), dim3(256), 0, st, s, gpb, nb, nodes);
+    else if (s.format == GPS_FORMAT_QUANTIZED)
+        hipLaunchKernelGGL(build_nodes_kernel<GPS_FORMAT_QUANTIZED>, dim3(blocks), dim3(256), 0, st, s, gpb, nb,
+                           nodes);
     else
         hipLaunchKernelGGL(build_nodes_kernel<GPS_FORMAT_HALF_FP8>, dim3(blocks), dim3(256), 0, st, s, gpb, nb, nodes);
 }
@@ -2086,6 +2202,8 @@ void launch_batch_preprocess(const BatchArgs& a, hipStream_t st) {
         else if (d == 1) launch_batch_pre_t<GPS_FORMAT_FP32, 1>(a, st);
         else if (d == 2) launch_batch_pre_t<GPS_FORMAT_FP32, 2>(a, st);
         else launch_batch_pre_t<GPS_FORMAT_FP32, 3>(a, st);
+    } else if (a.scene.format == GPS_FORMAT_QUANTIZED) {
+        launch_batch_pre_t<GPS_FORMAT_QUANTIZED, 3>(a, st);
     } else {
         if (d == 0) launch_batch_pre_t<GPS_FORMAT_HALF_FP8, 0>(a, st);
         else if (d == 1) launch_batch_pre_t<GPS_FORMAT_HALF_FP8, 1>(a, st);

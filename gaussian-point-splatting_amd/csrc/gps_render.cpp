@@ -100,7 +100,10 @@ Options parse(int argc, char** argv) {
         else if (a == "--seed") o.seed = std::stoull(next());
         else if (a == "--frames") o.frames = std::stoi(next());
         else if (a == "--out") o.out = next();
-        else if (a == "--format") o.format = next() == "half_fp8" ? GPS_FORMAT_HALF_FP8 : GPS_FORMAT_FP32;
+        else if (a == "--format") {
+            const std::string f = next();
+            o.format = f == "half_fp8" ? GPS_FORMAT_HALF_FP8 : f == "quantized" ? GPS_FORMAT_QUANTIZED : GPS_FORMAT_FP32;
+        }
         else if (a == "--orbit") {
             float r[2];
             parse_vec(i, argc, argv, r, 2);

@@ -41,6 +41,7 @@ _lib = C.CDLL(LIB_PATH)
 GPS_OK = 0
 FORMAT_FP32 = 0
 FORMAT_HALF_FP8 = 1
+FORMAT_QUANTIZED = 2
 
 DBG_WEIGHTS = 0
 DBG_MASK_CURRENT = 1
@@ -64,6 +65,15 @@ class SceneDesc(C.Structure):
     _fields_ = [("num_gaussians", C.c_uint64), ("sh_degree", C.c_int32), ("format", C.c_int32),
                 ("means", C.c_void_p), ("rotations", C.c_void_p), ("scales", C.c_void_p),
                 ("opacities", C.c_void_p), ("shs", C.c_void_p)]
+
+
+class QuantizedBounds(C.Structure):
+    """gps_quantized_bounds: the scene box of FORMAT_QUANTIZED."""
+    _fields_ = [("min_pos", C.c_float * 3), ("max_pos", C.c_float * 3), ("min_log_scale", C.c_float * 3),
+                ("max_log_scale", C.c_float * 3), ("min_sh", C.c_float * 45), ("max_sh", C.c_float * 45)]
+
+    def as_array(self) -> np.ndarray:
+        return np.frombuffer(bytes(self), np.float32).copy()
 
 
 class RenderSettingsC(C.Structure):
@@ -156,6 +166,9 @@ _sig("gps_synth_generate", C.c_int32, C.POINTER(SynthParams), C.c_uint64, C.c_ui
 _sig("gps_synth_generate_indices", C.c_int32, C.POINTER(SynthParams), _vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp)
 _sig("gps_morton_order", C.c_int32, _vp, C.c_uint64, _vp)
 _sig("gps_encode_half_fp8", C.c_int32, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("gps_quantized_bounds_compute", C.c_int32, C.POINTER(SceneDesc), C.POINTER(QuantizedBounds))
+_sig("gps_encode_quantized", C.c_int32, C.POINTER(SceneDesc), C.POINTER(QuantizedBounds), _vp, _vp, _vp, _vp)
+_sig("gps_scene_set_quantized_bounds", C.c_int32, _vp, C.POINTER(QuantizedBounds))
 _sig("gps_float_to_half", C.c_uint16, C.c_float)
 _sig("gps_float_to_fp8e4m3", C.c_uint8, C.c_float)
 _sig("gps_device_count", C.c_int32, C.POINTER(C.c_int32))
@@ -388,6 +401,27 @@ def synth_scene(p: SynthParams, first: int = 0, count: Optional[int] = None) -> 
     return SceneArrays(means, rots, scales, ops, shs, int(p.sh_degree))
 
 
+def quantized_bounds(s: SceneArrays) -> QuantizedBounds:
+    """Scene box of the reference's compressed format (GaussianSceneBounds)."""
+    b = QuantizedBounds()
+    d = s.desc(FORMAT_QUANTIZED)
+    check(_lib.gps_quantized_bounds_compute(C.byref(d), C.byref(b)), "quantized bounds")
+    return b
+
+
+def encode_quantized(s: SceneArrays, bounds: Optional[QuantizedBounds] = None) -> dict:
+    """Host encoding of FORMAT_QUANTIZED (the bytes the GPU scene holds)."""
+    n = s.n
+    b = bounds if bounds is not None else quantized_bounds(s)
+    out = {"geometry": np.empty((n, 4), np.uint32), "opacity": np.empty((n,), np.uint8),
+           "sh_dc": np.empty((n, 3), np.float32), "sh_codes": np.empty((n, 45), np.uint8), "bounds": b}
+    d = s.desc(FORMAT_QUANTIZED)
+    check(_lib.gps_encode_quantized(C.byref(d), C.byref(b), *[_ptr(out[k]) for k in
+                                                             ("geometry", "opacity", "sh_dc", "sh_codes")]),
+          "encode quantized")
+    return out
+
+
 def encode_half_fp8(s: SceneArrays) -> dict:
     """Host encoding of the compact format (the bytes the GPU scene holds)."""
     n = s.n
@@ -427,6 +461,10 @@ class Scene:
         s.format = fmt
         s.sh_degree = sh_degree
         return s
+
+    def set_quantized_bounds(self, bounds: QuantizedBounds) -> None:
+        """FORMAT_QUANTIZED: the scene box, needed before upload_range."""
+        check(_lib.gps_scene_set_quantized_bounds(self.handle, C.byref(bounds)), "quantized bounds")
 
     def upload_range(self, first: int, arrays: SceneArrays) -> None:
         d = arrays.desc(self.format)

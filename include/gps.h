@@ -43,7 +43,19 @@ typedef enum gps_scene_format {
     GPS_FORMAT_FP32 = 0,
     /* fp32 positions, fp16 scale / rotation / opacity, fp8-E4M3 SH with one
        fp32 scale per Gaussian.  Project-defined compact format. */
-    GPS_FORMAT_HALF_FP8 = 1
+    GPS_FORMAT_HALF_FP8 = 1,
+    /* The reference's compressed scene, its default build
+       (USE_COMPRESSED_GAUSSIANS, src/config.h:15; GPUGaussianSceneCompressed,
+       src/core/gaussians/gaussian_types.cuh:421-541): 16-byte geometry
+       (position 23/22/23 bits in the scene box, rotation in three 10-bit
+       "fem" components, log-scale 10 bits each in the scene's log-scale box;
+       encode / decode :237-374, quat_iem :212-235), one opacity byte
+       (/255, :199-202), SH DC in fp32 and the 45 higher coefficients as
+       bytes in per-coefficient scene ranges (CompactSH,
+       src/core/math/sh.cuh:28-51).  Colour is evaluated with the Sloan
+       degree-3 basis whatever sh_degree is (computeColorFromCompactSH,
+       sh.cuh:257-273 with eval_sh_sloan_l3 :55-96). */
+    GPS_FORMAT_QUANTIZED = 2
 } gps_scene_format;
 
 /* Host-side scene description, one record per Gaussian in structure-of-arrays
@@ -153,6 +165,29 @@ gps_status gps_scene_upload_range(gps_scene* scene, uint64_t first, const gps_sc
 void gps_scene_destroy(gps_scene* scene);
 /* Device bytes held by the scene. */
 uint64_t gps_scene_device_bytes(const gps_scene* scene);
+
+/* Scene box of GPS_FORMAT_QUANTIZED (GaussianSceneBounds,
+ * src/core/gaussians/gaussian_types.cuh:123-170): the position box widened
+ * by 0.001 on each side, the box of log(scale), and for higher SH
+ * coefficient j (0..14) and channel c the range [min_sh[3j+c],
+ * max_sh[3j+c]] (coefficients above sh_degree count as 0). */
+typedef struct gps_quantized_bounds {
+    float min_pos[3], max_pos[3];
+    float min_log_scale[3], max_log_scale[3];
+    float min_sh[45], max_sh[45];
+} gps_quantized_bounds;
+gps_status gps_quantized_bounds_compute(const gps_scene_desc* desc, gps_quantized_bounds* out);
+/* GPS_FORMAT_QUANTIZED needs its box before gps_scene_upload_range
+ * (gps_scene_create computes it from the whole description).  A box wider
+ * than the data is allowed; positions outside it are clamped to it. */
+gps_status gps_scene_set_quantized_bounds(gps_scene* scene, const gps_quantized_bounds* bounds);
+/* Host encoder of GPS_FORMAT_QUANTIZED (the code gps_scene_upload_range
+ * uses; encode_gaussian_geometry :237-320, normalized_float_to_byte :204-207,
+ * CompactSH's constructor sh.cuh:29-39): geometry N*4 words, opacity N
+ * bytes, sh_dc N*3 floats, sh_codes N*45 bytes (coefficient-major, rgb
+ * interleaved).  A component whose box has zero width (0/0) encodes as 0. */
+gps_status gps_encode_quantized(const gps_scene_desc* desc, const gps_quantized_bounds* bounds, uint32_t* geometry,
+                                uint8_t* opacity, float* sh_dc, uint8_t* sh_codes);
 
 /* ------------------------------------------------------------------ passes */
 /* Replaces GaussianPointSplatting::GaussianPointSplatting(const RenderSettings&)

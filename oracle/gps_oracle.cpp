@@ -183,6 +183,12 @@ struct orc_pass {
     // scene (decoded to fp32)
     uint64_t N = 0;
     std::vector<float> means, rots, scales, opac, sh48;
+    // colour from the compressed scene's Sloan-basis evaluation
+    // (computeColorFromCompactSH) instead of computeColorFromSHFloat
+    bool compact_sh = false;
+    template <typename T> vec3<T> sh_color(vec3<T> pos, vec3<T> campos, const T* shs) const {
+        return compact_sh ? computeColorFromCompactSH(pos, campos, shs) : computeColorFromSHFloat(pos, campos, shs);
+    }
     // settings
     int W = 0, H = 0, spp = 1, ss_created = 1;
     uint32_t maxPoints = 250000000u;
@@ -395,7 +401,7 @@ struct orc_pass {
                         if (K > 1) n = (uint32_t)stochastic_round(static_cast<float>(n) / K, pcg2d(seed).x);
                         if (n <= 0) break;
                         n = std::min(n, gps::det_f2u_rz((float)(f.up_w * f.up_h) / (2 * K)));
-                        colors[idx] = computeColorFromSHFloat(mean(idx), f.campos, &sh48[48 * idx]);
+                        colors[idx] = sh_color(mean(idx), f.campos, &sh48[48 * idx]);
                         weights[idx] = n;
                         splats = true;
                     } while (false);
@@ -683,6 +689,54 @@ orc_pass* orc_pass_create_half_fp8(const gps_render_settings* s, const orc_half_
     return p;
 }
 
+// GPUGaussianSceneCompressed (gaussian_types.cuh:421-541): every Gaussian
+// decoded once (decode_gaussian_geometry, get_opacity, CompactSH::get); the
+// colour is then the Sloan-basis evaluation.
+orc_pass* orc_pass_create_quantized(const gps_render_settings* s, const orc_quantized_scene* d, int32_t nthreads) {
+    if (!s || !d) return nullptr;
+    orc_pass* p = new orc_pass();
+    const uint64_t N = d->num_gaussians;
+    p->N = N;
+    p->means.resize(3 * N);
+    p->rots.resize(4 * N);
+    p->scales.resize(3 * N);
+    p->opac.resize(N);
+    p->sh48.assign(48 * N, 0.0f);
+    for (uint64_t g = 0; g < N; ++g) {
+        float out[59];
+        orc_quantized_decode(d, g, out);
+        std::copy(out, out + 3, &p->means[3 * g]);
+        std::copy(out + 3, out + 7, &p->rots[4 * g]);
+        std::copy(out + 7, out + 10, &p->scales[3 * g]);
+        p->opac[g] = out[10];
+        std::copy(out + 11, out + 59, &p->sh48[48 * g]);
+    }
+    p->compact_sh = true;
+    p->init_common(s, nthreads);
+    return p;
+}
+
+void orc_quantized_decode(const orc_quantized_scene* d, uint64_t g, float* out) {
+    const GaussianGeometry geo = decode_gaussian_geometry(d->geometry + 4 * g, d->bounds);
+    out[0] = geo.mean.x;
+    out[1] = geo.mean.y;
+    out[2] = geo.mean.z;
+    out[3] = geo.quat.x;
+    out[4] = geo.quat.y;
+    out[5] = geo.quat.z;
+    out[6] = geo.quat.w;
+    out[7] = geo.scale.x;
+    out[8] = geo.scale.y;
+    out[9] = geo.scale.z;
+    out[10] = normalized_byte_to_float(d->opacity[g]);
+    for (int i = 0; i < 16; ++i) {
+        const vec3<float> c = compact_sh_get(i, d->sh_dc + 3 * g, d->sh_codes + 45 * g, d->bounds);
+        out[11 + 3 * i] = c.x;
+        out[12 + 3 * i] = c.y;
+        out[13 + 3 * i] = c.z;
+    }
+}
+
 void orc_pass_destroy(orc_pass* p) { delete p; }
 void orc_pass_set_threads(orc_pass* p, int32_t n) { p->nthreads = std::max(1, (int)n); }
 
@@ -784,7 +838,7 @@ int32_t orc_project(orc_pass* p, const gps_frame_params* fp, int32_t use_fp64, d
                 ivec2_ a, bb;
                 float det, vz;
                 if (!p->loadAndTransform((uint32_t)g, f, pmn, pm, cov, a, bb, det, vz)) continue;
-                const vec3<float> c = computeColorFromSHFloat(p->mean(g), f.campos, &p->sh48[48 * g]);
+                const vec3<float> c = p->sh_color(p->mean(g), f.campos, &p->sh48[48 * g]);
                 const float op = p->opacity(g);
                 const float imp = 2.0f * 3.14159265358979323846f * std::sqrt(det) * dilog(op);
                 const double vals[16] = {1, vz, pm.x, pm.y, cov.x, cov.y, cov.z, c.x, c.y, c.z, imp, pmn.z,
@@ -819,7 +873,7 @@ int32_t orc_project(orc_pass* p, const gps_frame_params* fp, int32_t use_fp64, d
                 if (!compute2DGaussianBounds3DGS(bmin, bmax, f.up_w, f.up_h, cov, pm)) continue;
                 std::vector<double> sh(48);
                 for (int k = 0; k < 48; ++k) sh[k] = p->sh48[48 * g + k];
-                const vec3<double> c = computeColorFromSHFloat(m, campos, sh.data());
+                const vec3<double> c = p->sh_color(m, campos, sh.data());
                 const double op = std::min((double)p->opac[g], 1.0);
                 // the reference's dilog fit (random.cuh:88-104) evaluated in FP64
                 const double cf[8] = {-6.09201442e-01, 1.79126616e+00, -2.14953223e+00, 1.26304372e+00,
@@ -971,6 +1025,21 @@ void orc_sh_eval(const float dir[3], const float* coeffs48, int32_t fp64, double
         out[0] = r.x; out[1] = r.y; out[2] = r.z;
     } else {
         const vec3<float> r = computeColorFromSHFloat(vec3<float>{dir[0], dir[1], dir[2]}, vec3<float>{0, 0, 0}, coeffs48);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    }
+}
+
+// computeColorFromCompactSH at view direction `dir`, decoded coefficients.
+void orc_sh_eval_compact(const float dir[3], const float* coeffs48, int32_t fp64, double out[3]) {
+    if (fp64) {
+        double c[48];
+        for (int k = 0; k < 48; ++k) c[k] = coeffs48[k];
+        const vec3<double> r =
+            computeColorFromCompactSH(vec3<double>{dir[0], dir[1], dir[2]}, vec3<double>{0, 0, 0}, c);
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    } else {
+        const vec3<float> r =
+            computeColorFromCompactSH(vec3<float>{dir[0], dir[1], dir[2]}, vec3<float>{0, 0, 0}, coeffs48);
         out[0] = r.x; out[1] = r.y; out[2] = r.z;
     }
 }

@@ -36,7 +36,22 @@ typedef struct orc_half_fp8_scene {
     const float* sh_scale;     /* N */
 } orc_half_fp8_scene;
 
+/* Encoded GPS_FORMAT_QUANTIZED scene as produced by gps_encode_quantized(). */
+typedef struct orc_quantized_scene {
+    uint64_t num_gaussians;
+    gps_quantized_bounds bounds;
+    const uint32_t* geometry; /* N*4 */
+    const uint8_t* opacity;   /* N */
+    const float* sh_dc;       /* N*3 */
+    const uint8_t* sh_codes;  /* N*45 */
+} orc_quantized_scene;
+
 orc_pass* orc_pass_create(const gps_render_settings* settings, const gps_scene_desc* scene, int32_t nthreads);
+orc_pass* orc_pass_create_quantized(const gps_render_settings* settings, const orc_quantized_scene* scene,
+                                    int32_t nthreads);
+/* Decoded Gaussian g of a quantized scene: mean[3], rotation[4] (stored
+ * order), scale[3], opacity, 16 SH coefficients x rgb (59 floats). */
+void orc_quantized_decode(const orc_quantized_scene* scene, uint64_t g, float* out);
 orc_pass* orc_pass_create_half_fp8(const gps_render_settings* settings, const orc_half_fp8_scene* scene,
                                    int32_t nthreads);
 void orc_pass_destroy(orc_pass* pass);
@@ -80,6 +95,7 @@ uint64_t orc_depth_bits(float view_z, float near_plane, float far_plane);
 void orc_detmath(int32_t kind, const float* in, float* out, uint64_t n);
 /* computeColorFromSHFloat (full degree 3) at direction dir, FP32 or FP64. */
 void orc_sh_eval(const float dir[3], const float* coeffs48, int32_t fp64, double out[3]);
+void orc_sh_eval_compact(const float dir[3], const float* coeffs48, int32_t fp64, double out[3]);
 
 #ifdef __cplusplus
 }

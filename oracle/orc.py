@@ -42,6 +42,11 @@ class HalfFp8Scene(C.Structure):
                 ("sh_scale", C.c_void_p)]
 
 
+class QuantizedScene(C.Structure):
+    _fields_ = [("num_gaussians", C.c_uint64), ("bounds", gps_amd.QuantizedBounds), ("geometry", C.c_void_p),
+                ("opacity", C.c_void_p), ("sh_dc", C.c_void_p), ("sh_codes", C.c_void_p)]
+
+
 _vp = C.c_void_p
 
 
@@ -54,6 +59,8 @@ def _sig(name, res, *args):
 
 _sig("orc_pass_create", _vp, C.POINTER(RenderSettingsC), C.POINTER(SceneDesc), C.c_int32)
 _sig("orc_pass_create_half_fp8", _vp, C.POINTER(RenderSettingsC), C.POINTER(HalfFp8Scene), C.c_int32)
+_sig("orc_pass_create_quantized", _vp, C.POINTER(RenderSettingsC), C.POINTER(QuantizedScene), C.c_int32)
+_sig("orc_quantized_decode", None, C.POINTER(QuantizedScene), C.c_uint64, _vp)
 _sig("orc_pass_destroy", None, _vp)
 _sig("orc_pass_set_threads", None, _vp, C.c_int32)
 _sig("orc_pass_render", C.c_int32, _vp, C.POINTER(FrameParams), _vp, _vp)
@@ -77,6 +84,7 @@ _sig("orc_depth_bits", C.c_uint64, C.c_float, C.c_float, C.c_float)
 
 _sig("orc_detmath", None, C.c_int32, _vp, _vp, C.c_uint64)
 _sig("orc_sh_eval", None, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_double))
+_sig("orc_sh_eval_compact", None, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_double))
 
 lib = _lib
 
@@ -88,12 +96,34 @@ def detmath(kind: int, x: np.ndarray) -> np.ndarray:
     return out
 
 
-def sh_eval(direction, coeffs48, fp64: bool) -> np.ndarray:
+def sh_eval(direction, coeffs48, fp64: bool, compact: bool = False) -> np.ndarray:
+    """computeColorFromSHFloat (or, compact, computeColorFromCompactSH) at a view direction."""
     d = np.ascontiguousarray(direction, np.float32)
     c = np.ascontiguousarray(coeffs48, np.float32)
     out = (C.c_double * 3)()
-    _lib.orc_sh_eval(d.ctypes.data_as(C.POINTER(C.c_float)), c.ctypes.data_as(C.POINTER(C.c_float)), int(fp64), out)
+    fn = _lib.orc_sh_eval_compact if compact else _lib.orc_sh_eval
+    fn(d.ctypes.data_as(C.POINTER(C.c_float)), c.ctypes.data_as(C.POINTER(C.c_float)), int(fp64), out)
     return np.array(list(out))
+
+
+def quantized_scene(enc: dict) -> QuantizedScene:
+    """The oracle's view of gps_amd.encode_quantized output (arrays stay owned by `enc`)."""
+    q = QuantizedScene()
+    q.num_gaussians = enc["geometry"].shape[0]
+    q.bounds = enc["bounds"]
+    for k in ("geometry", "opacity", "sh_dc", "sh_codes"):
+        setattr(q, k, _ptr(enc[k]))
+    return q
+
+
+def quantized_decode(enc: dict) -> np.ndarray:
+    """(N, 59): decoded mean, rotation (stored order), scale, opacity, 16 x rgb SH coefficients."""
+    q = quantized_scene(enc)
+    n = enc["geometry"].shape[0]
+    out = np.empty((n, 59), np.float32)
+    for g in range(n):
+        _lib.orc_quantized_decode(C.byref(q), g, _ptr(out[g]))
+    return out
 
 _DTYPE = {gps_amd.DBG_FRAMEBUFFER: np.uint64, gps_amd.DBG_HIZ: np.float32}
 
@@ -112,6 +142,10 @@ class OraclePass:
         if fmt == gps_amd.FORMAT_FP32:
             d = scene.desc(fmt)
             self.handle = _lib.orc_pass_create(C.byref(self._s), C.byref(d), threads)
+        elif fmt == gps_amd.FORMAT_QUANTIZED:
+            self._enc = gps_amd.encode_quantized(scene)
+            q = quantized_scene(self._enc)
+            self.handle = _lib.orc_pass_create_quantized(C.byref(self._s), C.byref(q), threads)
         else:
             enc = gps_amd.encode_half_fp8(scene)
             self._enc = enc

@@ -70,6 +70,7 @@ template <typename T> inline vec3<T> operator*(vec3<T> a, vec3<T> b) { return {a
 template <typename T> inline vec3<T> operator*(vec3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
 template <typename T> inline vec3<T> operator*(T s, vec3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
 template <typename T> inline vec3<T> operator+(vec3<T> a, T s) { return {a.x + s, a.y + s, a.z + s}; }
+template <typename T> inline vec3<T> operator-(vec3<T> a, T s) { return {a.x - s, a.y - s, a.z - s}; }
 template <typename T> inline vec3<T> operator/(vec3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
 template <typename T> inline vec3<T> operator-(vec3<T> a) { return {-a.x, -a.y, -a.z}; }
 

@@ -6,6 +6,7 @@
 // TEST INFRASTRUCTURE: part of the CPU oracle.
 #pragma once
 
+#include "../include/gps.h"
 #include "orc_glm.h"
 
 namespace orc {
@@ -269,6 +270,134 @@ template <typename T> inline vec3<T> computeColorFromSHFloat(vec3<T> pos, vec3<T
                        T(SH_C3[6]) * x * (xx - T(3.0f) * yy) * SH(15));
     result = result + T(0.5f);
     return vmax(result, vec3<T>{T(0), T(0), T(0)});
+}
+
+// sh.cuh:55-96 (Sloan's real SH basis to degree 3)
+template <typename T> inline void eval_sh_sloan_l3(T out_shs[16], const vec3<T> dir) {
+    T x, y, z, z2, c0, s0, c1, s1, d, a, b;
+    x = dir[0];
+    y = dir[1];
+    z = dir[2];
+    z2 = z * z;
+    d = T(0.282094792f);
+    out_shs[0] = d;
+    d = T(0.488602512f) * z;
+    out_shs[2] = d;
+    a = z2 - T(0.333333333f);
+    d = T(0.946174696f) * a;
+    out_shs[6] = d;
+    b = z * (a - T(0.266666667f));
+    d = T(1.865881663f) * b;
+    out_shs[12] = d;
+    c1 = x;
+    s1 = y;
+    d = T(-0.488602512f);
+    out_shs[1] = s1 * d;
+    out_shs[3] = c1 * d;
+    d = T(-1.092548431f) * z;
+    out_shs[5] = s1 * d;
+    out_shs[7] = c1 * d;
+    a = z2 - T(0.2f);
+    d = T(-2.285228997f) * a;
+    out_shs[11] = s1 * d;
+    out_shs[13] = c1 * d;
+    c0 = x * c1 - y * s1;
+    s0 = y * c1 + x * s1;
+    d = T(0.546274215f);
+    out_shs[4] = s0 * d;
+    out_shs[8] = c0 * d;
+    d = T(1.445305721f) * z;
+    out_shs[10] = s0 * d;
+    out_shs[14] = c0 * d;
+    c1 = x * c0 - y * s0;
+    s1 = y * c0 + x * s0;
+    d = T(-0.59004359f);
+    out_shs[9] = s1 * d;
+    out_shs[15] = c1 * d;
+}
+// sh.cuh:257-273, with the 16 coefficients already decoded by CompactSH::get
+// (shs[3 * i + c] = get(i)[c]).
+template <typename T> inline vec3<T> computeColorFromCompactSH(vec3<T> pos, vec3<T> campos, const T* shs) {
+    vec3<T> dir = pos - campos;
+    dir = normalize(dir);
+    T eval_sh[16];
+    eval_sh_sloan_l3(eval_sh, dir);
+    vec3<T> result = {T(0), T(0), T(0)};
+    for (int i = 0; i < 16; ++i) {
+        const vec3<T> c = {shs[3 * i], shs[3 * i + 1], shs[3 * i + 2]};
+        result = result + eval_sh[i] * c;
+    }
+    result = result + T(0.5f);
+    return vmax(result, vec3<T>{T(0), T(0), T(0)});
+}
+
+// ------------------------------------------ gaussian_types.cuh, compressed scene
+// glm::mix(x, y, a) = x * (1 - a) + y * a (func_common.inl:111-162)
+inline float mix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+inline vec3<float> mix(vec3<float> x, vec3<float> y, vec3<float> a) {
+    return {mix(x.x, y.x, a.x), mix(x.y, y.y, a.y), mix(x.z, y.z, a.z)};
+}
+// :212-235, host path (1 / sqrt, sin, cos)
+inline vec4<float> quat_iem(const vec3<float> v) {
+    const float d = v.x * v.x + v.y * v.y + v.z * v.z;
+    const float t = 1.0f / gps::det_sqrt(d + 1e-8f);
+    const float a = 1.57079632679f * t * d;
+    float s, c;
+    gps::det_sincosf(a, &s, &c);
+    const float k = s * t;
+    return {k * v.x, k * v.y, k * v.z, c};
+}
+struct GaussianGeometry {
+    vec3<float> mean;
+    vec4<float> quat;
+    vec3<float> scale;
+};
+// :323-374 (__expf replaced by the deterministic exp)
+inline GaussianGeometry decode_gaussian_geometry(const uint32_t qg[4], const gps_quantized_bounds& scene) {
+    GaussianGeometry result{};
+    const uint32_t px = qg[0] & 0x7FFFFFu;
+    const uint32_t py_low9 = (qg[0] >> 23) & 0x1FFu;
+    const uint32_t py_high13 = qg[1] & 0x1FFFu;
+    const uint32_t pz_low19 = (qg[1] >> 13) & 0x7FFFFu;
+    const uint32_t pz_high4 = qg[2] & 0xFu;
+    const uint32_t py = py_low9 | (py_high13 << 9);
+    const uint32_t pz = pz_low19 | (pz_high4 << 19);
+    const float fx = float(px) / float((1u << 23) - 1u);
+    const float fy = float(py) / float((1u << 22) - 1u);
+    const float fz = float(pz) / float((1u << 23) - 1u);
+    const vec3<float> minPos = {scene.min_pos[0], scene.min_pos[1], scene.min_pos[2]};
+    const vec3<float> maxPos = {scene.max_pos[0], scene.max_pos[1], scene.max_pos[2]};
+    result.mean = mix(minPos, maxPos, vec3<float>{fx, fy, fz});
+    const uint32_t rx = (qg[2] >> 4) & 0x3FFu;
+    const uint32_t ry = (qg[2] >> 14) & 0x3FFu;
+    const uint32_t rz_low8 = (qg[2] >> 24) & 0xFFu;
+    const uint32_t rz_high2 = qg[3] & 0x3u;
+    const uint32_t rz = rz_low8 | (rz_high2 << 8);
+    const vec3<float> vr = vec3<float>{float(rx), float(ry), float(rz)} / 1023.0f;
+    const vec3<float> v = vr * 2.0f - 1.0f;
+    result.quat = quat_iem(v);
+    const uint32_t sx = (qg[3] >> 2) & 0x3FFu;
+    const uint32_t sy = (qg[3] >> 12) & 0x3FFu;
+    const uint32_t sz = (qg[3] >> 22) & 0x3FFu;
+    const vec3<float> st = vec3<float>{float(sx), float(sy), float(sz)} / 1023.0f;
+    const vec3<float> minScale = {scene.min_log_scale[0], scene.min_log_scale[1], scene.min_log_scale[2]};
+    const vec3<float> maxScale = {scene.max_log_scale[0], scene.max_log_scale[1], scene.max_log_scale[2]};
+    result.scale = mix(minScale, maxScale, st);
+    result.scale = {gps::det_expf(result.scale.x), gps::det_expf(result.scale.y), gps::det_expf(result.scale.z)};
+    return result;
+}
+// :188-191 (normalized_byte_to_float, the decode_opacity of a byte)
+inline float normalized_byte_to_float(const uint8_t value) { return static_cast<float>(value) / 255.0f; }
+// CompactSH::get, sh.cuh:41-47
+inline vec3<float> compact_sh_get(int idx, const float base[3], const uint8_t sh[45],
+                                  const gps_quantized_bounds& scene) {
+    if (idx <= 0) return {base[0], base[1], base[2]};
+    --idx;
+    constexpr float r = 1.0f / 255.0f;
+    const vec3<float> vec = {sh[3 * idx] * r, sh[3 * idx + 1] * r, sh[3 * idx + 2] * r};
+    const vec3<float> mn = {scene.min_sh[3 * idx], scene.min_sh[3 * idx + 1], scene.min_sh[3 * idx + 2]};
+    const vec3<float> mx = {scene.max_sh[3 * idx], scene.max_sh[3 * idx + 1], scene.max_sh[3 * idx + 2]};
+    return mix(mn, mx, vec);
 }
 
 // ------------------------------------------------- post_processing_kernels.cuh

@@ -214,6 +214,26 @@ def test_frames_half_fp8(gpu):
              which=H.INTERMEDIATES + [H.G.DBG_HIZ, H.G.DBG_CULL_MASK])
 
 
+@pytest.mark.parametrize("case", ["sh3_two_pass_culling", "sh0_plain"])
+def test_frames_quantized(gpu, case):
+    """The reference's compressed scene (its default build): on-the-fly decode
+    of position / rotation / scale / opacity bytes, the Sloan-basis colour of
+    the SH bytes, and the whole frame bit for bit against the oracle, which
+    decodes the same bytes with its own restatement."""
+    if case == "sh0_plain":
+        arrays = H.scene(0x5EED0312, 30_000, H.C1)
+        st = gpu.RenderSettings(320, 200, 1, 1, fully_disable_hierarchical_culling=True)
+        frames = [c1_frame(320, 200)]
+        which = H.INTERMEDIATES
+    else:
+        arrays = H.scene(0x5EED0311, 30_000, H.C2, clusters=100)
+        st = gpu.RenderSettings(320, 200, 2, 2, gaussians_per_block=128)
+        frames = moving_frames(3, 320, 200, 2, radius=300.0, occlusion=True, hierarchical=True)
+        which = H.INTERMEDIATES + [H.G.DBG_HIZ, H.G.DBG_CULL_MASK]
+    res = run_pair(arrays, st, frames, fmt=H.G.FORMAT_QUANTIZED, which=which)
+    assert res[0][0].stats["num_points"][0] > 1000
+
+
 def test_c3_construction_streamed_culling_taa(gpu):
     """The 100M-Gaussian workload's construction at test size: 4 translated
     copies of a clustered SH3 scene, Morton load order over the whole scene,
@@ -446,7 +466,7 @@ def test_batch_views_identical_to_single_renders(gpu):
         check_batch(fr, cols, deps, r, f"graph {k}")
 
 
-@pytest.mark.parametrize("case", ["mixed", "closeup_big", "budget"])
+@pytest.mark.parametrize("case", ["mixed", "closeup_big", "budget", "quantized"])
 def test_batch_binned_views_identical(gpu, case):
     """Binned batch views (the regions of all binned views in one job queue,
     counting and scattering per view, jobs that find their view by region)
@@ -456,7 +476,7 @@ def test_batch_binned_views_identical(gpu, case):
     views short."""
     cfg = dict(H.C1, half_extent=6.0, log_scale_mu=-2.0) if case != "closeup_big" else dict(H.C1, log_scale_mu=-0.7)
     arrays = H.scene(0x5EED0701 + len(case), 30_000, cfg).morton_sorted()
-    scene = gpu.Scene(arrays, gpu.FORMAT_FP32)
+    scene = gpu.Scene(arrays, gpu.FORMAT_QUANTIZED if case == "quantized" else gpu.FORMAT_FP32)
     budget = 6_000 if case == "budget" else 250_000_000
     stream = gpu.Stream()
     batch = gpu.Batch(scene, 5, 200, 120, max_spp=4, max_supersampling=2, gaussians_per_block=64,
