@@ -55,7 +55,7 @@ def stage_bytes(w, st):
     N = w.n
     nc = (w.sh_degree + 1) ** 2
     sh_bytes = 3 * nc * 4 if w.fmt == 0 else 64
-    pos_geo = 44 if w.fmt == 0 else 12 + 16
+    pos_geo = {0: 44, 1: 12 + 16, 2: 16 + 1}[w.fmt]  # fp32 planes / half-fp8 / quantized words + opacity byte
     W, H, ss, spp = w.width, w.height, w.ss, w.spp
     F = math.ceil(W * ss / 64) * math.ceil(H * ss / 64) * 4096 * spp
     passes = st["passes"]
@@ -229,6 +229,8 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-closed", type=int, default=300, help="requests of the closed-loop comparison")
     ap.add_argument("--batch-views", type=int, default=64, help="views per batch (0 = skip the batched form)")
+    ap.add_argument("--format", choices=("workload", "fp32", "half_fp8", "quantized"), default="workload",
+                    help="scene storage format (default: the workload's own)")
     ap.add_argument("--no-splat-forms", action="store_true",
                     help="skip re-timing the frames with each splat form forced")
     ap.add_argument("--batch-count", type=int, default=5, help="timed batches")
@@ -243,6 +245,9 @@ def main() -> int:
         print("no GPU visible", file=sys.stderr)
         return 1
     w = W.WORKLOADS[args.config]
+    if args.format != "workload":
+        w = W.replace(w, fmt={"fp32": G.FORMAT_FP32, "half_fp8": G.FORMAT_HALF_FP8,
+                              "quantized": G.FORMAT_QUANTIZED}[args.format])
     if args.no_sort:
         w = W.replace(w, morton=False)
     K = args.steps or max(60, w.path_frames)
@@ -348,7 +353,7 @@ def main() -> int:
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.config}: {w.desc}", "gaussians": w.n, "width": w.width, "height": w.height,
-                   "ss": w.ss, "spp": w.spp, "format": "fp32" if w.fmt == 0 else "half_fp8",
+                   "ss": w.ss, "spp": w.spp, "format": ("fp32", "half_fp8", "quantized")[w.fmt],
                    "order": "generation" if args.no_sort else "morton (load-time, reference --sort-morton-order)",
                    "culling": "occlusion+blocks" if w.hierarchical else ("occlusion" if w.occlusion else "none"),
                    "resolve": "taa" if w.taa else "progressive"},

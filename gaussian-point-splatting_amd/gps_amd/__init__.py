@@ -409,6 +409,18 @@ def quantized_bounds(s: SceneArrays) -> QuantizedBounds:
     return b
 
 
+def merge_quantized_bounds(a: QuantizedBounds, b: QuantizedBounds) -> QuantizedBounds:
+    """The box of two parts of one scene (componentwise min of the minima, max of the maxima)."""
+    out = QuantizedBounds()
+    for name in ("pos", "log_scale", "sh"):
+        lo, hi = f"min_{name}", f"max_{name}"
+        n = len(getattr(a, lo))
+        for i in range(n):
+            getattr(out, lo)[i] = min(getattr(a, lo)[i], getattr(b, lo)[i])
+            getattr(out, hi)[i] = max(getattr(a, hi)[i], getattr(b, hi)[i])
+    return out
+
+
 def encode_quantized(s: SceneArrays, bounds: Optional[QuantizedBounds] = None) -> dict:
     """Host encoding of FORMAT_QUANTIZED (the bytes the GPU scene holds)."""
     n = s.n

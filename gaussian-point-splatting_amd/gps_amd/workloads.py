@@ -26,7 +26,8 @@ from typing import Callable, Optional
 
 import numpy as np
 
-from . import (FORMAT_FP32, FORMAT_HALF_FP8, DeviceArray, Pass, RenderSettings, Scene, SceneArrays, Stream,
+from . import (FORMAT_FP32, FORMAT_HALF_FP8, FORMAT_QUANTIZED, DeviceArray, Pass, RenderSettings, Scene, SceneArrays,
+               Stream, merge_quantized_bounds, quantized_bounds,
                _lib, _ptr, check, frame_params, mat4_mul, resolve_progressive, resolve_taa, synth_params,
                taa_reprojection)
 import ctypes as C
@@ -190,6 +191,16 @@ def device_scene(w: Workload, perm: Optional[np.ndarray] = None, chunk: int = 1 
                 f"{time.time() - t0:.1f}s")
     t0 = time.time()
     scene = Scene.empty(w.n, w.sh_degree, w.fmt)
+    if w.fmt == FORMAT_QUANTIZED:
+        # The scene box over the whole scene comes first; per-chunk boxes merge
+        # exactly (min / max, and the 0.001 widening commutes with them).
+        box = None
+        for b in range(0, w.n, chunk):
+            cb = quantized_bounds(_generate(w, perm[b:b + chunk]))
+            box = cb if box is None else merge_quantized_bounds(box, cb)
+        scene.set_quantized_bounds(box)
+        if log:
+            log(f"scene box of {w.n} Gaussians in {time.time() - t0:.1f}s")
     for b in range(0, w.n, chunk):
         arr = _generate(w, perm[b:b + chunk])
         scene.upload_range(b, arr)

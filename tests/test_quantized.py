@@ -117,3 +117,15 @@ def test_encoder_rejects_bad_input():
     s = _scene(n=10)
     with pytest.raises(G.GpsError):
         G.Scene.empty(10, 3, G.FORMAT_QUANTIZED).upload_range(0, s)  # no scene box yet
+
+
+def test_box_of_parts_is_box_of_whole():
+    """Streamed uploads take the scene box from per-chunk boxes: merging them
+    gives exactly the box of the whole scene."""
+    s = _scene(n=5000)
+    parts = [slice(0, 1200), slice(1200, 3900), slice(3900, 5000)]
+    boxes = [G.quantized_bounds(G.SceneArrays(*(np.ascontiguousarray(getattr(s, k)[p]) for k in
+                                                ("means", "rotations", "scales", "opacities", "shs")), s.sh_degree))
+             for p in parts]
+    m = G.merge_quantized_bounds(G.merge_quantized_bounds(boxes[0], boxes[1]), boxes[2])
+    assert np.array_equal(m.as_array(), G.quantized_bounds(s).as_array())
