@@ -6,9 +6,20 @@
 // as save_image_to_disk does) and PFM (linear colour).  Flags of the
 // interactive viewer that have no headless meaning are accepted and ignored.
 // Headless additions: --synthetic, --num-gaussians, --seed, --frames, --out,
-// --format, --orbit, --eye/--center, --taa-current-depth.
+// --format, --orbit, --eye/--center, --taa-current-depth,
+// --frames-per-camera-keyframe, --total-test-keyframes.
+//
+// Camera paths (--camera-path): the first camera sets the resolution (the
+// width is kept, the height follows its aspect ratio) and every camera its
+// fovy, unless --resolution or --fovy forces them (camera_loader.cu:73-88).
+// --play-camera-path plays the keyframes 1/24 s apart at one keyframe per
+// frame; --run-tests steps through the cameras in main.cu's order (the
+// selection advances after each frame, clamped at the last camera, and
+// restarts at 0 every total_test_keyframes frames; the first lap is not
+// timed) and prints the per-camera timing JSON of main.cu:211-241.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -50,6 +61,12 @@ struct Options {
     bool taa_current_depth = false;
     int gaussians_per_block = 0;
     int accumulation_none = 0;
+    // camera paths (render_settings.cuh:42-79)
+    std::string camera_path;
+    bool play_camera_path = false;
+    bool override_camera_settings = false;
+    int frames_per_camera_keyframe = 10;
+    int total_test_keyframes = 0;
 };
 
 bool parse_vec(int& i, int argc, char** argv, float* out, int n) {
@@ -67,7 +84,10 @@ Options parse(int argc, char** argv) {
         else if (a == "--rotation") parse_vec(i, argc, argv, o.rotation, 3);
         else if (a == "--translation") parse_vec(i, argc, argv, o.translation, 3);
         else if (a == "--scaling") parse_vec(i, argc, argv, o.scaling, 3);
-        else if (a == "--fovy") o.fovy = std::stof(next());
+        else if (a == "--fovy") {
+            o.fovy = std::stof(next());
+            o.override_camera_settings = true;
+        }
         else if (a == "--near-view-plane" || a == "--near_view_plane") o.near_plane = std::stof(next());
         else if (a == "--far-view-plane" || a == "--far_view_plane") o.far_plane = std::stof(next());
         else if (a == "--samples-per-pixel" || a == "--samples_per_pixel") o.render_pass_count = std::stoi(next());
@@ -78,6 +98,7 @@ Options parse(int argc, char** argv) {
             parse_vec(i, argc, argv, r, 2);
             o.width = (int)r[0];
             o.height = (int)r[1];
+            o.override_camera_settings = true;
         } else if (a == "--set-target-width" || a == "--set_target_width") o.width = std::stoi(next());
         else if (a == "--cull-small") o.cull_small = true;
         else if (a == "--one_point_per_thread" || a == "--one-point-per-thread") o.reduce_point_count = false;
@@ -87,11 +108,14 @@ Options parse(int argc, char** argv) {
         else if (a == "--run_tests" || a == "--run-tests") o.run_tests = true;
         else if (a == "--model-path") o.model_path = next();
         else if (a == "--sort_morton_order" || a == "--sort-morton-order") o.sort_morton = true;
-        else if (a == "--max-eval-count" || a == "--max_eval_count" || a == "--cameras-path" || a == "--camera-path" ||
-                 a == "--sorted-model-output-path" || a == "--sorted_model_output_path") next();
+        else if (a == "--cameras-path" || a == "--camera-path") o.camera_path = next();
+        else if (a == "--play_camera_path" || a == "--play-camera-path") o.play_camera_path = true;
+        else if (a == "--frames-per-camera-keyframe") o.frames_per_camera_keyframe = std::stoi(next());
+        else if (a == "--total-test-keyframes") o.total_test_keyframes = std::stoi(next());
+        else if (a == "--max-eval-count" || a == "--max_eval_count" || a == "--sorted-model-output-path" ||
+                 a == "--sorted_model_output_path") next();
         else if (a == "--freeze-culling" || a == "--enable_profiling" || a == "--disable_gui" || a == "--disable-gui" ||
-                 a == "--play_camera_path" ||
-                 a == "--play-camera-path" || a == "--snap_window_to_corner" || a == "--snap-window-to-corner" ||
+                 a == "--snap_window_to_corner" || a == "--snap-window-to-corner" ||
                  a == "--create-evaluation-screenshots" || a == "--create_evaluation_screenshots") {
         }
         // headless additions
@@ -247,6 +271,24 @@ int main(int argc, char** argv) {
     d.shs = shs.data();
     gps_scene* scene = nullptr;
     CHECK(gps_scene_create(&d, &scene));
+    // camera path (RenderSettings::load_camera, render_settings.cuh:248-256)
+    std::vector<gps_camera_state> cams;
+    if (!o.camera_path.empty()) {
+        int32_t n = 0;
+        if (gps_camera_path_load(o.camera_path.c_str(), nullptr, 0, &n) == GPS_OK && n > 0) {
+            cams.resize((size_t)n);
+            CHECK(gps_camera_path_load(o.camera_path.c_str(), cams.data(), n, &n));
+        } else {
+            std::fprintf(stderr, "Error reading camera_states file: %s\n", o.camera_path.c_str());
+        }
+        if (!cams.empty() && !o.override_camera_settings) {
+            int32_t w = 0, h = 0;
+            CHECK(gps_camera_resolution(&cams[0], o.width, &w, &h));
+            o.width = w;
+            o.height = h;
+        }
+        if (o.total_test_keyframes <= 0) o.total_test_keyframes = (int)cams.size();
+    }
     gps_render_settings rs;
     std::memset(&rs, 0, sizeof(rs));
     rs.width = o.width;
@@ -281,22 +323,50 @@ int main(int argc, char** argv) {
     std::vector<double> times;
     std::vector<uint8_t> h_rgb(npx * 3);
     std::vector<float> h_col(npx * 3);
-    for (int fi = 1; fi <= o.frames; ++fi) {
-        float eye[3] = {o.eye[0], o.eye[1], o.eye[2]};
-        if (o.orbit_radius > 0) {
-            const float az = (float)(fi - 1) * o.orbit_step * 0.01745329251994329576923690768489f;
-            const float el = o.orbit_elevation * 0.01745329251994329576923690768489f;
-            eye[0] = o.orbit_radius * std::cos(el) * std::sin(az);
-            eye[1] = o.orbit_radius * std::sin(el);
-            eye[2] = -o.orbit_radius * std::cos(el) * std::cos(az);
-        }
-        const float up[3] = {0, 1, 0};
+    // --run-tests over a camera path: main.cu:196-243
+    const bool cam_tests = o.run_tests && !cams.empty();
+    const int n_test = std::max(1, o.total_test_keyframes);
+    const int passes_per_camera = o.frames_per_camera_keyframe + 1;
+    int n_frames = o.frames;
+    if (cam_tests) n_frames = passes_per_camera * n_test + 1;
+    else if (o.play_camera_path && !cams.empty()) n_frames = std::max(o.frames, (int)cams.size() - 1);  // to the end
+    int selected = 0, prev_selected = -1;
+    std::vector<double> test_times;
+    for (int fi = 1; fi <= n_frames; ++fi) {
         gps_frame_params f;
-        gps_frame_params_init(&f, eye, o.center, up, o.fovy, W, H, o.supersampling_factor, o.near_plane, o.far_plane);
+        bool moved = o.orbit_radius > 0;
+        float fovy = o.fovy;
+        if (!cams.empty()) {
+            if (o.play_camera_path) {
+                CHECK(gps_camera_path_frame(cams.data(), (int32_t)cams.size(), (float)fi / 24.0f,
+                                            o.override_camera_settings ? o.fovy : 0.0f, W, H, o.supersampling_factor,
+                                            o.near_plane, o.far_plane, &f, nullptr));
+                moved = true;
+            } else {
+                CHECK(gps_frame_params_from_camera(&cams[(size_t)selected], W, H,
+                                                   o.override_camera_settings ? o.fovy : 0.0f, o.supersampling_factor,
+                                                   o.near_plane, o.far_plane, &f));
+                moved = selected != prev_selected && prev_selected >= 0;
+                prev_selected = selected;
+            }
+            fovy = f.fovy_degrees;
+        } else {
+            float eye[3] = {o.eye[0], o.eye[1], o.eye[2]};
+            if (o.orbit_radius > 0) {
+                const float az = (float)(fi - 1) * o.orbit_step * 0.01745329251994329576923690768489f;
+                const float el = o.orbit_elevation * 0.01745329251994329576923690768489f;
+                eye[0] = o.orbit_radius * std::cos(el) * std::sin(az);
+                eye[1] = o.orbit_radius * std::sin(el);
+                eye[2] = -o.orbit_radius * std::cos(el) * std::cos(az);
+            }
+            const float up[3] = {0, 1, 0};
+            gps_frame_params_init(&f, eye, o.center, up, o.fovy, W, H, o.supersampling_factor, o.near_plane,
+                                  o.far_plane);
+        }
         float view_scene[16];
         gps_mat4_mul(f.view, scene_m, view_scene);
         std::memcpy(f.view, view_scene, 64);
-        gps_frustum_planes(o.fovy, f.view, float(W) / H, o.near_plane, o.far_plane, f.frustum_planes);
+        gps_frustum_planes(fovy, f.view, float(W) / H, o.near_plane, o.far_plane, f.frustum_planes);
         f.frame = fi;
         f.enable_occlusion_culling = o.enable_occlusion_culling;
         f.enable_hierarchical_culling = o.enable_hierarchical_culling;
@@ -318,7 +388,6 @@ int main(int argc, char** argv) {
             CHECK(gps_resolve_taa((const float*)color[a], cur_depth, (const float*)hist[b], (const float*)depth[b], M,
                                   0.9f, W, H, progressive_frames, (float*)hist[a], (uint8_t*)rgb, stream));
         } else {
-            const bool moved = o.orbit_radius > 0;
             progressive_frames = (moved || o.accumulation_none) ? 1 : progressive_frames + 1;
             CHECK(gps_resolve_progressive((float*)color[a], (float*)color[b], W, H, progressive_frames, (uint8_t*)rgb,
                                           stream));
@@ -327,6 +396,12 @@ int main(int argc, char** argv) {
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         times.push_back(ms);
         gps_mat4_mul(f.proj, f.view, prev_vp);
+        if (cam_tests) {
+            const int frame_index = fi - 1;
+            selected = std::min(selected + 1, (int)cams.size() - 1);  // change_camera_index(1)
+            if (frame_index >= n_test) test_times.push_back(ms);
+            if (frame_index % n_test == 0 && frame_index > 0) selected = 0;  // load_camera_state(0)
+        }
         if (!o.out.empty()) {
             CHECK(gps_memcpy_d2h(h_rgb.data(), rgb, npx * 3, stream));
             char name[64];
@@ -342,7 +417,21 @@ int main(int argc, char** argv) {
                         (unsigned long long)st.num_points[0], (unsigned long long)st.num_points[1],
                         (unsigned long long)st.samples);
     }
-    if (o.run_tests) {
+    if (cam_tests) {
+        // src/main.cu:211-241, per camera
+        std::printf("---BEGIN_TIMINGS_JSON---\n{\n  \"algorithm\": \"GaussianPointSplatting\",\n");
+        std::printf("  \"model_path\": \"%s%s\",\n  \"timings\": [\n", o.model_path.empty() ? "synthetic:" : "",
+                    o.model_path.empty() ? o.synthetic.c_str() : o.model_path.c_str());
+        for (int cam = 0; cam < n_test; ++cam) {
+            std::printf("    { \"camera\": %d, \"samples\": [", cam);
+            for (int rep = 0; rep < o.frames_per_camera_keyframe; ++rep) {
+                const size_t idx = (size_t)cam + (size_t)rep * n_test;
+                std::printf("%s%.4f", rep ? ", " : "", idx < test_times.size() ? test_times[idx] : 0.0);
+            }
+            std::printf("] }%s\n", cam + 1 < n_test ? "," : "");
+        }
+        std::printf("  ]\n}\n---END_TIMINGS_JSON---\n");
+    } else if (o.run_tests) {
         // src/main.cu:211-241 envelope
         std::printf("---BEGIN_TIMINGS_JSON---\n{\n  \"algorithm\": \"GaussianPointSplatting\",\n");
         std::printf("  \"model_path\": \"%s%s\",\n  \"timings\": [\n    { \"camera\": 0, \"samples\": [",

@@ -870,6 +870,67 @@ class Batch:
             pass
 
 
+# ----------------------------------------------------------------- camera paths
+class CameraState(C.Structure):
+    """gps_camera_state: one keyframe of a camera-path JSON file."""
+    _fields_ = [("id", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("position", C.c_float * 3),
+                ("rotation", C.c_float * 9), ("fy", C.c_float), ("fx", C.c_float), ("img_name", C.c_char * 64)]
+
+
+_sig("gps_camera_path_load", C.c_int32, C.c_char_p, C.POINTER(CameraState), C.c_int32, C.POINTER(C.c_int32))
+_sig("gps_camera_path_save", C.c_int32, C.c_char_p, C.POINTER(CameraState), C.c_int32)
+_sig("gps_camera_fovy", C.c_float, C.POINTER(CameraState))
+_sig("gps_camera_resolution", C.c_int32, C.POINTER(CameraState), C.c_int32, C.POINTER(C.c_int32),
+     C.POINTER(C.c_int32))
+_sig("gps_frame_params_from_camera", C.c_int32, C.POINTER(CameraState), C.c_int32, C.c_int32, C.c_float, C.c_int32,
+     C.c_float, C.c_float, C.POINTER(FrameParams))
+_sig("gps_camera_path_frame", C.c_int32, C.POINTER(CameraState), C.c_int32, C.c_float, C.c_float, C.c_int32,
+     C.c_int32, C.c_int32, C.c_float, C.c_float, C.POINTER(FrameParams), C.POINTER(C.c_int32))
+
+
+def camera_path_load(path: str) -> list:
+    """Cameras of a camera-path JSON file, sorted by id."""
+    n = C.c_int32()
+    check(_lib.gps_camera_path_load(path.encode(), None, 0, C.byref(n)), f"camera path {path}")
+    arr = (CameraState * max(1, n.value))()
+    check(_lib.gps_camera_path_load(path.encode(), arr, n.value, C.byref(n)), f"camera path {path}")
+    return list(arr)[:n.value]
+
+
+def camera_path_save(path: str, cams) -> None:
+    arr = (CameraState * max(1, len(cams)))(*cams)
+    check(_lib.gps_camera_path_save(path.encode(), arr, len(cams)), f"camera path {path}")
+
+
+def camera_fovy(cam: CameraState) -> float:
+    return float(_lib.gps_camera_fovy(C.byref(cam)))
+
+
+def camera_resolution(cam: CameraState, width: int) -> tuple:
+    w, h = C.c_int32(), C.c_int32()
+    check(_lib.gps_camera_resolution(C.byref(cam), width, C.byref(w), C.byref(h)), "camera resolution")
+    return w.value, h.value
+
+
+def frame_from_camera(cam: CameraState, width: int, height: int, fovy: float = 0.0, ss: int = 1,
+                      near: float = 0.2, far: float = 1e4) -> FrameParams:
+    """Frame of a camera keyframe as the viewer builds it (fovy <= 0: the camera's own)."""
+    f = FrameParams()
+    check(_lib.gps_frame_params_from_camera(C.byref(cam), width, height, fovy, ss, near, far, C.byref(f)), "camera")
+    return f
+
+
+def camera_path_frame(cams, time_s: float, width: int, height: int, fovy: float = 0.0, ss: int = 1,
+                      near: float = 0.2, far: float = 1e4) -> tuple:
+    """(frame, finished) of camera-path playback at time_s (keyframes 1/24 s apart)."""
+    arr = (CameraState * len(cams))(*cams)
+    f = FrameParams()
+    fin = C.c_int32()
+    check(_lib.gps_camera_path_frame(arr, len(cams), time_s, fovy, width, height, ss, near, far, C.byref(f),
+                                     C.byref(fin)), "camera path frame")
+    return f, bool(fin.value)
+
+
 # ----------------------------------------------------------------- PLY files
 class PlyInfo(C.Structure):
     _fields_ = [("num_gaussians", C.c_uint64), ("record_bytes", C.c_uint64), ("has_rest", C.c_int32),

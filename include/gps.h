@@ -298,6 +298,49 @@ void gps_frustum_planes(float fovy_degrees, const float view[16], float aspect, 
 void gps_frame_params_init(gps_frame_params* frame, const float eye[3], const float center[3],
                            const float up[3], float fovy_degrees, int32_t width, int32_t height,
                            int32_t supersampling_factor, float near_plane, float far_plane);
+/* ------------------------------------------------------- camera paths */
+/* One camera keyframe of a camera-path file (CameraLoader::CameraState,
+ * src/utils/camera_loader.cuh:26-37; the files under assets/cameras): rotation is
+ * the file's 3x3 camera-to-world rotation, row-major as written. */
+typedef struct gps_camera_state {
+    int32_t id;
+    int32_t width, height;
+    float position[3];
+    float rotation[9];
+    float fy, fx;
+    char img_name[64];
+} gps_camera_state;
+/* Reads a camera-path JSON file (loadCameras, camera_loader.cu:120-148):
+ * all cameras sorted by id, the first `capacity` copied to `out`, the total
+ * in *count (capacity 0 only counts).  GPS_ERR_IO for a missing or
+ * malformed file. */
+gps_status gps_camera_path_load(const char* path, gps_camera_state* out, int32_t capacity, int32_t* count);
+/* Writes cameras in the same JSON layout (saveCameras, camera_loader.cu:106-117). */
+gps_status gps_camera_path_save(const char* path, const gps_camera_state* states, int32_t count);
+/* Vertical field of view of a camera, degrees (Camera::focal2fovdeg(fy,
+ * height), src/world/camera.h:44-47). */
+float gps_camera_fovy(const gps_camera_state* state);
+/* Resolution a camera file gives the renderer when none is forced
+ * (fromCameraState, camera_loader.cu:73-85, load_cam_resolution_height_only):
+ * the width is kept and the height follows the camera's aspect ratio. */
+gps_status gps_camera_resolution(const gps_camera_state* state, int32_t width, int32_t* out_width,
+                                 int32_t* out_height);
+/* Frame of a camera state as the viewer builds it (fromCameraState + Transform
+ * + main.cu:146-158): view = scale(1,-1,-1) * affineInverse(translate(position)
+ * * mat4_cast(quat_cast(basis))), perspective projection, frustum planes;
+ * fovy_degrees <= 0 takes gps_camera_fovy(state).  Flags as
+ * gps_frame_params_init. */
+gps_status gps_frame_params_from_camera(const gps_camera_state* state, int32_t width, int32_t height,
+                                        float fovy_degrees, int32_t supersampling_factor, float near_plane,
+                                        float far_plane, gps_frame_params* out);
+/* Frame of camera-path playback at time_s seconds (startCameraPath +
+ * CameraPath::interpolate, camera_loader.cu:93-104, src/world/camera.h:53-106):
+ * keyframes 1/24 s apart, position by mix, rotation by slerp, fovy of the
+ * first camera unless given; *finished = 1 at the end of the path. */
+gps_status gps_camera_path_frame(const gps_camera_state* states, int32_t count, float time_s, float fovy_degrees,
+                                 int32_t width, int32_t height, int32_t supersampling_factor, float near_plane,
+                                 float far_plane, gps_frame_params* out, int32_t* finished);
+
 /* deltaMatrix of the TAA pass (render_server.cu:128-129). */
 void gps_taa_reprojection(const float prev_view_proj[16], const float view[16], const float proj[16],
                           float out[16]);

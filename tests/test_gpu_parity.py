@@ -577,3 +577,34 @@ def test_device_fp8_decode_matches_oracle(gpu):
     nan = np.isnan(host)
     assert nan.sum() == 2 and np.isnan(dev[nan]).all()
     assert_same("fp8 decode", dev[~nan], host[~nan])
+
+
+def test_cli_camera_path_run_tests(gpu, tmp_path):
+    """The headless driver on a camera-path file with --run-tests: the
+    cameras in main.cu's order, the per-camera timing JSON of
+    main.cu:211-241, and the resolution the first camera sets (width kept,
+    height from its 4946 x 3286 aspect ratio)."""
+    import json
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "gaussian-point-splatting_amd", "gps_render")
+    fix = os.path.join(root, "tests", "golden", "truck_cameras_head.json")
+    out = str(tmp_path / "frame")
+    flush("cli: camera path run-tests")
+    r = subprocess.run([exe, "--synthetic", "c1", "--num-gaussians", "20000", "--camera-path", fix, "--run-tests",
+                        "--frames-per-camera-keyframe", "2", "--set-target-width", "320", "--out", out],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    text = r.stdout
+    body = text[text.index("---BEGIN_TIMINGS_JSON---") + len("---BEGIN_TIMINGS_JSON---"):
+                text.index("---END_TIMINGS_JSON---")]
+    doc = json.loads(body)
+    assert doc["algorithm"] == "GaussianPointSplatting"
+    assert [t["camera"] for t in doc["timings"]] == [0, 1, 2, 3]
+    assert all(len(t["samples"]) == 2 and all(s > 0 for s in t["samples"]) for t in doc["timings"])
+    # frames 1 .. 3 * 4 + 1 were written at 320 x round(320 * 3286 / 4946)
+    with open(out + "_0001.ppm", "rb") as fh:
+        head = fh.read(32).split()
+    assert head[0] == b"P6" and (int(head[1]), int(head[2])) == (320, round(320 * 3286 / 4946))
+    assert os.path.exists(out + f"_{3 * 4 + 1:04d}.ppm")
