@@ -74,12 +74,13 @@ GPS_HD float glm_min(float x, float y) { return (y < x) ? y : x; }
 GPS_HD float glm_max(float x, float y) { return (x < y) ? y : x; }
 GPS_HD float glm_clamp(float x, float lo, float hi) { return glm_min(glm_max(x, lo), hi); }
 
-// Saturating conversions (PTX cvt.rzi / cvt.rni semantics).
+// Saturating conversions (PTX cvt.rzi / cvt.rni semantics).  det_f2i_rz is
+// written with selects (no branches in device code): the conversion is of a
+// value clamped into range, and the saturated / NaN results are selected.
 GPS_HD int32_t det_f2i_rz(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return 2147483647;
-    if (x <= -2147483648.0f) return (int32_t)0x80000000u;
-    return (int32_t)x;
+    const float c = (x != x) ? 0.0f : glm_min(glm_max(x, -2147483648.0f), 2147483520.0f);
+    const int32_t r = (int32_t)c;  // 2147483520 = the largest float below 2^31
+    return (x >= 2147483648.0f) ? 2147483647 : r;
 }
 GPS_HD int32_t det_f2i_rn(float x) { return det_f2i_rz(det_rint(x)); }
 GPS_HD uint32_t det_f2u_rz(float x) {
@@ -94,12 +95,16 @@ GPS_HD uint64_t det_f2u64_rz(float x) {
 }
 
 // ---------------------------------------------------------------- exp
+// The special cases (NaN, overflow above ln(FLT_MAX), underflow below
+// ln(min subnormal / 2)) are selected after evaluating an in-range stand-in,
+// so device code has no branches; in-range inputs take the same arithmetic.
 GPS_HD float det_expf(float x) {
-    if (x != x) return x;
-    if (x > 0x1.62e430p+6f) return u2f_bits(0x7F800000u);  // > ln(FLT_MAX)
-    if (x < -0x1.9fe368p+6f) return 0.0f;                  // < ln(min subnormal / 2)
-    const float k = det_rint(x * 0x1.715476p+0f);           // x / ln2
-    float r = det_fma(-k, 0x1.62e400p-1f, x);                // ln2 hi
+    const bool nan = x != x;
+    const bool over = x > 0x1.62e430p+6f;
+    const bool under = x < -0x1.9fe368p+6f;
+    const float xs = (nan || over || under) ? 0.0f : x;
+    const float k = det_rint(xs * 0x1.715476p+0f);          // x / ln2
+    float r = det_fma(-k, 0x1.62e400p-1f, xs);               // ln2 hi
     r = det_fma(-k, 0x1.7f7d1cp-20f, r);                     // ln2 lo
     float p = 0x1.a01a02p-13f;                               // 1/7!
     p = det_fma(p, r, 0x1.6c16c2p-10f);                      // 1/6!
@@ -114,7 +119,8 @@ GPS_HD float det_expf(float x) {
     const int32_t k2 = ki - k1;
     const float s1 = u2f_bits((uint32_t)(k1 + 127) << 23);
     const float s2 = u2f_bits((uint32_t)(k2 + 127) << 23);
-    return (p * s1) * s2;
+    const float e = (p * s1) * s2;
+    return nan ? x : over ? u2f_bits(0x7F800000u) : under ? 0.0f : e;
 }
 
 // ---------------------------------------------------------------- log
@@ -171,16 +177,14 @@ GPS_HD void det_sincosf_inrange(float x, float* sp, float* cp) {
     pc = det_fma(pc, z, 0x1.555556p-5f);                     //  1/4!
     pc = det_fma(pc, z, -0x1.000000p-1f);                    // -1/2!
     const float cr = det_fma(pc, z, 1.0f);
+    // quadrant q: (sin, cos) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr),
+    // as selects (no branches in device code)
     const uint32_t q = (uint32_t)(int32_t)k & 3u;
-    float s, c;
-    switch (q) {
-        case 0: s = sr; c = cr; break;
-        case 1: s = cr; c = -sr; break;
-        case 2: s = -sr; c = -cr; break;
-        default: s = -cr; c = sr; break;
-    }
-    *sp = s;
-    *cp = c;
+    const bool odd = (q & 1u) != 0u;
+    const float s = odd ? cr : sr;
+    const float c = odd ? sr : cr;
+    *sp = (q & 2u) ? -s : s;
+    *cp = ((q + 1u) & 2u) ? -c : c;
 }
 
 GPS_HD void det_sincosf(float x, float* sp, float* cp) {
