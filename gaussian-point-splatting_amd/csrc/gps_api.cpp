@@ -776,10 +776,8 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         sa.xcds = p->xcds;
         stage_mark(p, 4, slot, 0, st);
         BinGeom bg_geom;
-        // Binned when a point has >= 8 samples (K * spp): below that the
-        // per-point atomics are few and the binning does not pay for itself.
         const bool binned = p->splat_mode != GPS_SPLAT_POINT_PARALLEL &&
-                            (p->splat_mode == GPS_SPLAT_BINNED || f.Ki * f.spp >= 8) &&
+                            (p->splat_mode == GPS_SPLAT_BINNED || f.Ki * f.spp >= GPS_BIN_AUTO_SAMPLES) &&
                             binned_geometry(f.up_w, f.up_h, f.spp, &bg_geom);
         p->last_binned = binned ? 1 : 0;
         if (binned) {

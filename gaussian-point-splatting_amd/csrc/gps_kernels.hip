@@ -430,10 +430,45 @@ struct PreItem {
     uint32_t g;
     float op, view_z, pmx, pmy, ca, cb, cc, det;
     uint32_t bmin, bmax;
+    float mx, my, mz;  // world mean (the colour's view direction)
 };
 struct ColourItem {
     uint32_t g, depth, bmax;
+    float mx, my, mz;
 };
+
+// Two 32-bit valid-mask words per wave and the workgroup's (points, valid)
+// sums added to its 4096-Gaussian tile; n = this lane's point count.
+__device__ __forceinline__ void preprocess_epilogue(uint32_t n, uint32_t g, uint32_t g0, uint32_t N,
+                                                    uint32_t* __restrict__ mask_cur, uint32_t* __restrict__ tile_sum,
+                                                    uint32_t* __restrict__ tile_cnt) {
+    const int tid = threadIdx.x;
+    const uint64_t b = __ballot(n != 0u);
+    const int lane = tid & 63;
+    const uint32_t wave_first = g - (uint32_t)lane;
+    const uint32_t sum = wave_reduce_add(n);
+    __shared__ uint32_t s_sum[GPS_PRE_THREADS / 64], s_cnt[GPS_PRE_THREADS / 64];
+    if (lane == 0) {
+        if (wave_first < N) mask_cur[wave_first >> 5] = (uint32_t)b;
+        if (wave_first + 32u < N) mask_cur[(wave_first >> 5) + 1] = (uint32_t)(b >> 32);
+        s_sum[tid >> 6] = sum;
+        s_cnt[tid >> 6] = (uint32_t)__popcll(b);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t ts = 0, tc = 0;
+#pragma unroll
+        for (int w = 0; w < GPS_PRE_THREADS / 64; ++w) {
+            ts += s_sum[w];
+            tc += s_cnt[w];
+        }
+        const uint32_t tile = g0 >> GPS_TILE_SHIFT;
+        if (tc) {
+            atomicAdd(tile_sum + tile, ts);
+            atomicAdd(tile_cnt + tile, tc);
+        }
+    }
+}
 
 template <int FORMAT, int DEG>
 __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
@@ -493,6 +528,9 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
                     q.det = pr.det;
                     q.bmin = (uint32_t)pr.minx | ((uint32_t)pr.miny << 16);
                     q.bmax = (uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16);
+                    q.mx = ge.mx;
+                    q.my = ge.my;
+                    q.mz = ge.mz;
                 }
             }
         }
@@ -524,6 +562,9 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
             s_col[k].g = q.g;
             s_col[k].depth = (uint32_t)depth_quantize(q.view_z, f.near_plane, f.far_plane);
             s_col[k].bmax = q.bmax;
+            s_col[k].mx = q.mx;
+            s_col[k].my = q.my;
+            s_col[k].mz = q.mz;
             s_count[q.g - g0] = cnt;
         }
         weights[q.g] = cnt;
@@ -532,41 +573,35 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     // Phase C
     if ((uint32_t)tid < s_n2) {
         const ColourItem q = s_col[tid];
-        const Geo ge = load_geo<FORMAT>(s, q.g);
         float dx, dy, dz;
-        view_dir(f, ge.mx, ge.my, ge.mz, dx, dy, dz);
+        view_dir(f, q.mx, q.my, q.mz, dx, dy, dz);
         float rgb[3];
         ShEval<FORMAT, DEG>::eval(s, q.g, dx, dy, dz, rgb);
         const uint64_t packed = ((uint64_t)q.depth << GPS_DEPTH_SHIFT) | pack_color36(rgb[0], rgb[1], rgb[2]);
         reinterpret_cast<uint4*>(recs + q.g)[3] = make_uint4(q.bmax, (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
     }
-    // Two 32-bit valid-mask words per wave, tile sums.
-    const uint32_t n = s_count[tid];
-    const uint64_t b = __ballot(n != 0u);
-    const int lane = tid & 63;
-    const uint32_t wave_first = g - (uint32_t)lane;
-    const uint32_t sum = wave_reduce_add(n);
-    __shared__ uint32_t s_sum[GPS_PRE_THREADS / 64], s_cnt[GPS_PRE_THREADS / 64];
-    if (lane == 0) {
-        if (wave_first < s.N) mask_cur[wave_first >> 5] = (uint32_t)b;
-        if (wave_first + 32u < s.N) mask_cur[(wave_first >> 5) + 1] = (uint32_t)(b >> 32);
-        s_sum[tid >> 6] = sum;
-        s_cnt[tid >> 6] = (uint32_t)__popcll(b);
+    preprocess_epilogue(s_count[tid], g, g0, s.N, mask_cur, tile_sum, tile_cnt);
+}
+
+// The same outputs with preprocess_one per lane and no work lists: cheaper
+// when nearly every Gaussian survives projection (frames without block or
+// occlusion culling, where the lists only add synchronisation).
+template <int FORMAT, int DEG>
+__global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_flat_kernel(
+    DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
+    const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
+    uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h) {
+    const uint32_t g0 = blockIdx.x * GPS_PRE_THREADS;
+    const uint32_t g = g0 + threadIdx.x;
+    const uint32_t it = *iteration;
+    uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
+    const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
+    uint32_t n = 0;
+    if (g < s.N && !(f.hier && block_culled(cv, g))) {
+        n = preprocess_one<FORMAT, DEG>(s, f, g, mask_prev, h, recs + g);
+        weights[g] = n;
     }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t ts = 0, tc = 0;
-#pragma unroll
-        for (int w = 0; w < GPS_PRE_THREADS / 64; ++w) {
-            ts += s_sum[w];
-            tc += s_cnt[w];
-        }
-        const uint32_t tile = g0 >> GPS_TILE_SHIFT;
-        if (tc) {
-            atomicAdd(tile_sum + tile, ts);
-            atomicAdd(tile_cnt + tile, tc);
-        }
-    }
+    preprocess_epilogue(n, g, g0, s.N, mask_cur, tile_sum, tile_cnt);
 }
 
 // ------------------------------------------------------------------ distribution
@@ -577,22 +612,35 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
                                                          uint32_t* __restrict__ tile_voff, PassCounters* __restrict__ ctr,
                                                          uint32_t budget, uint32_t* __restrict__ iteration) {
     constexpr int PER = 8;  // tiles per thread per step
+    // A step's 8192 tiles pass through LDS so that global loads and stores
+    // are coalesced while each thread scans 8 consecutive tiles; one pad word
+    // per 64 keeps the strided LDS reads conflict-free.
+    constexpr uint32_t STEP = 1024u * PER;
+    __shared__ uint32_t s_ta[STEP + STEP / 64], s_tc[STEP + STEP / 64];
     __shared__ uint32_t s_a[16], s_c[16];
     __shared__ uint32_t s_carry_a, s_carry_c;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto pidx = [](uint32_t j) { return j + (j >> 6); };
     if (tid == 0) {
         s_carry_a = 0;
         s_carry_c = 0;
     }
     __syncthreads();
-    for (uint32_t base = 0; base < ntiles; base += 1024 * PER) {
-        const uint32_t i0 = base + (uint32_t)tid * PER;
+    for (uint32_t base = 0; base < ntiles; base += STEP) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t j = (uint32_t)k * 1024u + (uint32_t)tid;
+            const uint32_t i = base + j;
+            s_ta[pidx(j)] = i < ntiles ? tile_sum[i] : 0u;
+            s_tc[pidx(j)] = i < ntiles ? tile_cnt[i] : 0u;
+        }
+        __syncthreads();
+        const uint32_t j0 = (uint32_t)tid * PER;
         uint32_t a[PER], c[PER], sa = 0, sc = 0;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const uint32_t i = i0 + (uint32_t)k;
-            a[k] = i < ntiles ? tile_sum[i] : 0u;
-            c[k] = i < ntiles ? tile_cnt[i] : 0u;
+            a[k] = s_ta[pidx(j0 + (uint32_t)k)];
+            c[k] = s_tc[pidx(j0 + (uint32_t)k)];
             sa += a[k];
             sc += c[k];
         }
@@ -611,18 +659,24 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
         const uint32_t ca = s_carry_a, cc = s_carry_c;
         uint32_t oa = ca + wa + ia - sa, oc = cc + wc + ic - sc;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const uint32_t i = i0 + (uint32_t)k;
-            if (i < ntiles) {
-                tile_off[i] = oa;
-                tile_voff[i] = oc;
-                tile_sum[i] = 0u;
-                tile_cnt[i] = 0u;
-            }
+        for (int k = 0; k < PER; ++k) {  // exclusive prefixes back into LDS
+            s_ta[pidx(j0 + (uint32_t)k)] = oa;
+            s_tc[pidx(j0 + (uint32_t)k)] = oc;
             oa += a[k];
             oc += c[k];
         }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t j = (uint32_t)k * 1024u + (uint32_t)tid;
+            const uint32_t i = base + j;
+            if (i < ntiles) {
+                tile_off[i] = s_ta[pidx(j)];
+                tile_voff[i] = s_tc[pidx(j)];
+                tile_sum[i] = 0u;
+                tile_cnt[i] = 0u;
+            }
+        }
         if (tid == 1023) {
             s_carry_a = ca + wa + ia;
             s_carry_c = cc + wc + ic;
@@ -760,6 +814,13 @@ __device__ __forceinline__ SplatRecord load_record(const SplatRecord* __restrict
     return rec;
 }
 
+// The alpha test of a sample (gaussian_point_splatting.cu:118-123):
+// gw = sigma > 0 ? 0 : exp(sigma), rejected when opacity * gw < 1/255.
+__device__ __forceinline__ bool alpha_passes(const SplatRecord& rec, float sigma) {
+    const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
+    return !(rec.opacity * gw < 1.0f / 255.0f);
+}
+
 // The K * spp samples of point p (seed stream p of RNG frame `frame_seed`)
 // of one Gaussian (splatGaussianPoint, gaussian_point_splatting.cu:67-141):
 // sample q = layer * K + k lands in layer q / K.  Returns the atomics issued.
@@ -788,9 +849,7 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
         if (x < bminx || x >= bmaxx || y < bminy || y >= bmaxy) continue;
         const float dx = rec.pmx - fx, dy = rec.pmy - fy;
         const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
-        const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
-        const float alpha = rec.opacity * gw;
-        if (alpha < 1.0f / 255.0f) continue;
+        if (!alpha_passes(rec, sigma)) continue;
         const int idx = fb_index(x, y, layer, spp, up_w);
         if (fb[idx] <= depth_bits) continue;
         atomicMin(fb + idx, packed);
@@ -1209,9 +1268,7 @@ __device__ __forceinline__ void splat_point_region(const SplatRecord& rec, uint3
         if ((unsigned)lx >= (unsigned)rw || (unsigned)ly >= (unsigned)rh) continue;  // another region's sample
         const float dx = rec.pmx - fx, dy = rec.pmy - fy;
         const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
-        const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
-        const float alpha = rec.opacity * gw;
-        if (alpha < 1.0f / 255.0f) continue;
+        if (!alpha_passes(rec, sigma)) continue;
         const uint32_t t = (uint32_t)((ly >> GPS_TILE_BITS) * b.g.rtx + (lx >> GPS_TILE_BITS));
         const uint32_t i = (t * GPS_TILE_WORDS + morton_encode((uint32_t)(x & (GPS_TILE_SIZE - 1)),
                                                                 (uint32_t)(y & (GPS_TILE_SIZE - 1)))) *
@@ -2039,8 +2096,14 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
 template <int FORMAT, int DEG>
 static void launch_pre(const PreprocessArgs& a, hipStream_t st) {
     const uint32_t blocks = (a.scene.N + GPS_PRE_THREADS - 1) / GPS_PRE_THREADS;
-    hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st, a.scene, a.f,
-                       a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt, a.cull, a.hiz);
+    if (a.f.hier || a.f.occl)  // culled frames: work lists keep the later phases on full waves
+        hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st, a.scene,
+                           a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt, a.cull,
+                           a.hiz);
+    else
+        hipLaunchKernelGGL((preprocess_flat_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
+                           a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
+                           a.cull, a.hiz);
 }
 
 void launch_preprocess(const PreprocessArgs& a, hipStream_t st) {
