@@ -141,6 +141,8 @@ struct BatchView {
 #define GPS_SPLAT_THREADS 256
 #define GPS_SPLAT_PPT 4
 #define GPS_SPLAT_CHUNK (GPS_SPLAT_THREADS * GPS_SPLAT_PPT)
+#define GPS_SPLAT_CHUNK_SHIFT 10u  // log2(GPS_SPLAT_CHUNK)
+static_assert((1u << GPS_SPLAT_CHUNK_SHIFT) == GPS_SPLAT_CHUNK, "chunk is a power of two");
 // Binned splat: one 1024-thread workgroup renders a screen region of at most
 // GPS_BIN_WORDS framebuffer words in LDS; GPS_BIN_PPT points per thread per
 // step; Gaussians overlapping more regions than GPS_BIN_MAX_REGIONS take the

@@ -51,6 +51,30 @@ __device__ __forceinline__ void add_count(uint32_t* counter, uint32_t v) {
 
 __device__ __forceinline__ bool mask_bit(const uint32_t* m, uint32_t i) { return ((m[i >> 5] >> (i & 31u)) & 1u) != 0u; }
 
+// Block culling of one preprocess workgroup's Gaussians [g0, g0 + threads):
+// Gaussian g lies in block g / gpb, whose bit is bit b % 32 of mask word
+// b / 32 (the same bit as block_culled below).  The workgroup's first block
+// and its offset into it are found once (scalar division); a lane's block
+// is then b0 + (r0 + lane) / gpb, a quotient of two integers below 2^24
+// (exact in float: the quotient is at least 1 / gpb from the next integer,
+// more than half an ulp) or, for blocks of 2^23 or more, 0 or 1.
+struct WgBlocks {
+    uint32_t b0, r0;
+};
+__device__ __forceinline__ bool wg_all_culled(const CullView& c, uint32_t g0, uint32_t n_in_wg, WgBlocks& wb) {
+    wb.b0 = g0 / c.gpb;
+    wb.r0 = g0 - wb.b0 * c.gpb;
+    const uint32_t b1 = (g0 + n_in_wg - 1u) / c.gpb;
+    for (uint32_t b = wb.b0; b <= b1; ++b)
+        if (!mask_bit(c.mask, b)) return false;
+    return true;
+}
+__device__ __forceinline__ bool lane_block_culled(const CullView& c, const WgBlocks& wb, uint32_t lane_off) {
+    const uint32_t o = wb.r0 + lane_off;
+    const uint32_t q = c.gpb >= (1u << 23) ? (o >= c.gpb ? 1u : 0u) : (uint32_t)((float)o / (float)c.gpb);
+    return mask_bit(c.mask, wb.b0 + q);
+}
+
 // gaussian_bvh.cuh:47-53
 __device__ __forceinline__ bool block_culled(const CullView& c, uint32_t g) {
     const uint32_t span = c.gpb * 32u;
@@ -486,6 +510,13 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     const uint32_t it = *iteration;
     uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
     const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
+    WgBlocks wb{0u, 0u};
+    if (f.hier && wg_all_culled(cv, g0, min((uint32_t)GPS_PRE_THREADS, s.N - g0), wb)) {
+        // Every Gaussian here is culled: only the valid-mask words (zero)
+        // are written; weights stay untouched, the tile sums get nothing.
+        if (tid < GPS_PRE_THREADS / 32 && g0 + 32u * (uint32_t)tid < s.N) mask_cur[(g0 >> 5) + (uint32_t)tid] = 0u;
+        return;
+    }
     if (tid == 0) {
         s_n1 = 0u;
         s_n2 = 0u;
@@ -494,7 +525,7 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     __syncthreads();
     // Phase A.  Gaussians of culled blocks leave their weight slot untouched:
     // the distributor reads the same cull mask and counts them as zero.
-    if (g < s.N && !(f.hier && block_culled(cv, g))) {
+    if (g < s.N && !(f.hier && lane_block_culled(cv, wb, (uint32_t)tid))) {
         bool survives = false;
         if (!(f.only_prev && mask_bit(mask_prev, g))) {
             const Geo ge = load_geo<FORMAT>(s, g);
@@ -2147,7 +2178,7 @@ void launch_distribute(const DistributeArgs& a, hipStream_t st) {
     if (a.ntiles == 0) return;
     hipLaunchKernelGGL(distribute_kernel, dim3(a.ntiles), dim3(256), 0, st, a.weights, a.tile_off, a.tile_voff,
                        a.ntiles, a.ctr, a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.N, a.ntiles, 0u, 0u,
-                       (uint8_t*)nullptr, 10u);
+                       (uint8_t*)nullptr, GPS_SPLAT_CHUNK_SHIFT);
 }
 
 void launch_splat(const SplatArgs& a, hipStream_t st) {
