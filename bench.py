@@ -231,6 +231,8 @@ def main() -> int:
     ap.add_argument("--batch-views", type=int, default=64, help="views per batch (0 = skip the batched form)")
     ap.add_argument("--format", choices=("workload", "fp32", "half_fp8", "quantized"), default="workload",
                     help="scene storage format (default: the workload's own)")
+    ap.add_argument("--first-frame", type=int, default=-1,
+                    help="camera-path position of the first timed frame (default: right after the warm-up)")
     ap.add_argument("--no-splat-forms", action="store_true",
                     help="skip re-timing the frames with each splat form forced")
     ap.add_argument("--batch-count", type=int, default=5, help="timed batches")
@@ -252,6 +254,7 @@ def main() -> int:
         w = W.replace(w, morton=False)
     K = args.steps or max(60, w.path_frames)
     Wm = args.warmup
+    F0 = max(0, args.first_frame - Wm) if args.first_frame >= 0 else 0  # warm-up positions F0 .. F0 + Wm - 1
     t0 = time.time()
     perm = W.order(w)
     log(f"[bench] {args.config}: load order of {w.n} Gaussians in {time.time() - t0:.1f}s")
@@ -267,16 +270,16 @@ def main() -> int:
 
     stream = G.Stream()
     loop = W.FrameLoop(w, scene, stream)
-    log(f"[bench] warmup {Wm} frames, then {K} timed frames (camera positions {Wm}..{Wm + K - 1} mod "
+    log(f"[bench] warmup {Wm} frames, then {K} timed frames (camera positions {F0 + Wm}..{F0 + Wm + K - 1} mod "
         f"{w.path_frames})")
-    run(loop, range(Wm))
+    run(loop, range(F0, F0 + Wm))
     stream.synchronize()
     loop.pass_.set_timing(K + 1)
     e0, e1 = G.Event(), G.Event()
     stream.synchronize()
     t_wall = time.perf_counter()
     e0.record(stream)
-    run(loop, range(Wm, Wm + K))
+    run(loop, range(F0 + Wm, F0 + Wm + K))
     e1.record(stream)
     stream.synchronize()
     wall = time.perf_counter() - t_wall
@@ -289,8 +292,8 @@ def main() -> int:
     loop = W.FrameLoop(w, scene, stream)
     loop.pass_.set_counters(True)
     per_frame = []
-    run(loop, range(Wm))
-    run(loop, range(Wm, Wm + K), per_frame)
+    run(loop, range(F0, F0 + Wm))
+    run(loop, range(F0 + Wm, F0 + Wm + K), per_frame)
     loop.close()
     # The same frames with each splat form forced (binned vs point-parallel
     # with one global atomic per sample): the chosen and the plain form.
@@ -300,11 +303,11 @@ def main() -> int:
     for name, mode in forms:
         loop = W.FrameLoop(w, scene, stream)
         loop.pass_.set_splat_mode(mode)
-        run(loop, range(Wm))
+        run(loop, range(F0, F0 + Wm))
         stream.synchronize()
         loop.pass_.set_timing(K + 1)
         t0 = time.perf_counter()
-        run(loop, range(Wm, Wm + K))
+        run(loop, range(F0 + Wm, F0 + Wm + K))
         stream.synchronize()
         sp = loop.pass_.timing_summary()["stages"]["splat"]["ms"] / K
         splat_forms[name] = {"ms_per_frame": round((time.perf_counter() - t0) * 1e3 / K, 4),
