@@ -42,7 +42,7 @@ def test_log_accuracy():
     exact = np.log(x.astype(np.float64))
     approx = orc.detmath(1, x)
     e = ulp_err(approx, exact)
-    # log(1 +- tiny) is tiny: judge those relative to ulp(1) instead
+    # log(1 +- tiny) is tiny: compare those relative to ulp(1) instead
     near1 = np.abs(exact) < 1e-3
     assert e[~near1].max() <= 2.0, e[~near1].max()
     assert np.max(np.abs(approx[near1] - exact[near1])) < 1e-9
