@@ -165,6 +165,7 @@ _sig("gps_mat4_mul", None, _fp, _fp, _fp)
 _sig("gps_synth_generate", C.c_int32, C.POINTER(SynthParams), C.c_uint64, C.c_uint64, _vp, _vp, _vp, _vp, _vp)
 _sig("gps_synth_generate_indices", C.c_int32, C.POINTER(SynthParams), _vp, C.c_uint64, _vp, _vp, _vp, _vp, _vp)
 _sig("gps_morton_order", C.c_int32, _vp, C.c_uint64, _vp)
+_sig("gps_bvh_order", C.c_int32, _vp, _vp, _vp, C.c_uint64, _vp)
 _sig("gps_encode_half_fp8", C.c_int32, C.POINTER(SceneDesc), _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("gps_quantized_bounds_compute", C.c_int32, C.POINTER(SceneDesc), C.POINTER(QuantizedBounds))
 _sig("gps_encode_quantized", C.c_int32, C.POINTER(SceneDesc), C.POINTER(QuantizedBounds), _vp, _vp, _vp, _vp)
@@ -358,6 +359,18 @@ class SceneArrays:
 
     def morton_sorted(self) -> "SceneArrays":
         return self.subset(self.morton_order())
+
+    def bvh_order(self) -> np.ndarray:
+        """perm[new] = old of the reference's default load-time reorder
+        (GaussianBVH::build_bvh_and_reorder, gaussian_bvh.cu:146-169); runs on
+        the GPU."""
+        perm = np.empty(self.n, np.uint32)
+        check(_lib.gps_bvh_order(_ptr(self.means), _ptr(self.rotations), _ptr(self.scales), self.n, _ptr(perm)),
+              "bvh order")
+        return perm
+
+    def bvh_sorted(self) -> "SceneArrays":
+        return self.subset(self.bvh_order())
 
     def subset(self, idx) -> "SceneArrays":
         return SceneArrays(np.ascontiguousarray(self.means[idx]), np.ascontiguousarray(self.rotations[idx]),

@@ -382,6 +382,19 @@ gps_status gps_synth_generate_indices(const gps_synth_params* p, const uint64_t*
  * (src/core/gaussians/gaussian_loader.cu:363-426): writes perm[new] = old for
  * a stable sort of the 63-bit Morton codes of the bbox-normalized positions. */
 gps_status gps_morton_order(const float* means, uint64_t n, uint32_t* perm);
+/* Load-time BVH order, the reference's default for a scene that is neither
+ * Morton-sorted nor marked "bvh_sorted_order" (GaussianBVH::
+ * build_bvh_and_reorder, src/core/rendering/occlusion/gaussian_bvh.cu:146-169,
+ * called from gaussian_point_splatting.cu:791-797): the tinybvh 1.6.7
+ * binned-SAH build (packages/tinybvh/tiny_bvh.h:2292-2420) over the 3-sigma
+ * boxes, the right-first leaf walk of collect_leaf_order (:14-34) and the
+ * cycle reorder (:47-84).  Writes perm[new] = old.  Built on the current GPU
+ * from host arrays laid out as in gps_scene_desc (rotations w, x, y, z;
+ * activated scales).  GPS_ERR_UNSUPPORTED above 2^31 - 1 Gaussians, or for
+ * scenes whose node areas exceed ~1e30 / count (where tinybvh splits on a
+ * degenerate axis). */
+gps_status gps_bvh_order(const float* means, const float* rotations, const float* scales, uint64_t n,
+                         uint32_t* perm);
 
 /* ---------------------------------------------------------- PLY files */
 /* 3D Gaussian Splatting PLY (binary little-endian), as the reference's

@@ -127,6 +127,69 @@ def quantized_decode(enc: dict) -> np.ndarray:
 
 _DTYPE = {gps_amd.DBG_FRAMEBUFFER: np.uint64, gps_amd.DBG_HIZ: np.float32}
 
+_sig("orc_bvh_fragments", C.c_int, _vp, _vp, _vp, C.c_uint64, _vp)
+_sig("orc_bvh_leaves", C.c_int64, _vp, C.c_uint64, _vp, _vp)
+_sig("orc_bvh_order", C.c_int, _vp, _vp, _vp, C.c_uint64, _vp)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def bvh_fragments(means, rotations, scales) -> np.ndarray:
+    """tinybvh input of the load-time reorder: 3 vec4 vertices per Gaussian."""
+    means, rotations, scales = _f32(means), _f32(rotations), _f32(scales)
+    n = len(means)
+    v = np.empty((n, 12), np.float32)
+    _lib.orc_bvh_fragments(_ptr(means), _ptr(rotations), _ptr(scales), n, _ptr(v))
+    return v
+
+
+def bvh_leaves(verts: np.ndarray):
+    """(first, count) of the leaves of the tinybvh build, left-first order."""
+    verts = _f32(verts)
+    n = len(verts)
+    first = np.empty(n, np.uint32)
+    count = np.empty(n, np.uint32)
+    k = _lib.orc_bvh_leaves(_ptr(verts), n, _ptr(first), _ptr(count))
+    assert k >= 0
+    return first[:k], count[:k]
+
+
+def bvh_order(means, rotations, scales) -> np.ndarray:
+    """perm[new] = old of the reference's BVH reorder."""
+    means, rotations, scales = _f32(means), _f32(rotations), _f32(scales)
+    n = len(means)
+    perm = np.empty(n, np.uint32)
+    assert _lib.orc_bvh_order(_ptr(means), _ptr(rotations), _ptr(scales), n, _ptr(perm)) == 0
+    return perm
+
+
+REF_TINYBVH_PATH = os.path.join(_HERE, "_ref", "libtinybvh_ref.so")
+
+
+def tinybvh_ref():
+    """The reference's own tinybvh built into _ref/ (None where the reference
+    tree was absent at build time)."""
+    if not os.path.exists(REF_TINYBVH_PATH):
+        return None
+    lib = C.CDLL(REF_TINYBVH_PATH)
+    lib.ref_tinybvh_leaf_order.restype = C.c_int
+    lib.ref_tinybvh_leaf_order.argtypes = [_vp, C.c_uint32, _vp]
+    return lib
+
+
+def tinybvh_ref_order(lib, verts: np.ndarray) -> np.ndarray:
+    """perm[new] = old from tinybvh's leaves and the reference's reorder:
+    new[leafOrder[m]] = old[m] (gaussian_bvh.cu:47-84)."""
+    verts = _f32(verts)
+    n = len(verts)
+    lo = np.empty(n, np.uint32)
+    assert lib.ref_tinybvh_leaf_order(_ptr(verts), n, _ptr(lo)) == 0
+    perm = np.empty(n, np.uint32)
+    perm[lo] = np.arange(n, dtype=np.uint32)
+    return perm
+
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
