@@ -228,6 +228,49 @@ def batched(G, W, args):
     return out
 
 
+def bvh_order_leg(G, W, args) -> dict:
+    """The load-time BVH order (gps_bvh_order, the reference's
+    GaussianBVH::build_bvh_and_reorder) of the configured workload's scene in
+    generation order, i.e. as an un-presorted PLY would arrive; host arrays
+    in, perm out, timed end to end.  cpu_baseline: the reference's own
+    tinybvh (oracle/_ref, its threaded build) on a bounded prefix of the same
+    scene; the oracle's serial restatement where _ref is absent."""
+    w = W.WORKLOADS[args.config]
+    t0 = time.time()
+    a = W.host_scene(w, np.arange(w.n, dtype=np.uint64))
+    log(f"[bench] bvh order: generated {w.n} Gaussians in {time.time() - t0:.1f}s")
+    a.bvh_order()  # first call: device memory and code objects
+    runs = []
+    for _ in range(args.bvh_order_runs):
+        t = time.perf_counter()
+        perm = a.bvh_order()
+        runs.append(time.perf_counter() - t)
+    sec = min(runs)
+    assert np.array_equal(np.sort(perm), np.arange(w.n, dtype=np.uint32))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc  # CPU baseline only
+    ns = min(w.n, args.bvh_ref_sample)
+    m, q, sc = a.means[:ns], a.rotations[:ns], a.scales[:ns]
+    ref = orc.tinybvh_ref()
+    t = time.perf_counter()
+    if ref is not None:
+        orc.tinybvh_ref_order(ref, orc.bvh_fragments(m, q, sc))
+        kind, cores = "reference", 32  # tinybvh runs the subtrees below depth 5 on 32 threads
+    else:
+        orc.bvh_order(m, q, sc)
+        kind, cores = "port", 1
+    cpu_sec = time.perf_counter() - t
+    return {
+        "metric": "load-time BVH order (reference build_bvh_and_reorder), Gaussians/s", "value": round(w.n / sec),
+        "unit": "Gaussians/s", "n_gpus": 1, "steps": len(runs), "warmup": 1, "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {w.desc}, generation order", "gaussians": w.n},
+        "runs_ms": [round(r * 1e3, 2) for r in runs],
+        "cpu_baseline": {"value": round(ns / cpu_sec), "unit": "Gaussians/s", "cores": cores, "kind": kind,
+                         "sample": f"first {ns} Gaussians of the same scene, {cpu_sec:.2f} s"},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +294,11 @@ def main() -> int:
     ap.add_argument("--no-splat-forms", action="store_true",
                     help="skip re-timing the frames with each splat form forced")
     ap.add_argument("--batch-count", type=int, default=5, help="timed batches")
+    ap.add_argument("--bvh-order", action="store_true",
+                    help="measure only the load-time BVH order of the workload's scene (one JSON line)")
+    ap.add_argument("--bvh-order-runs", type=int, default=3)
+    ap.add_argument("--bvh-ref-sample", type=int, default=8_000_000,
+                    help="Gaussians of the reference tinybvh CPU timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
@@ -261,6 +309,9 @@ def main() -> int:
     if G.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.bvh_order:
+        print(json.dumps(bvh_order_leg(G, W, args)), flush=True)
+        return 0
     w = W.WORKLOADS[args.config]
     if args.format != "workload":
         w = W.replace(w, fmt={"fp32": G.FORMAT_FP32, "half_fp8": G.FORMAT_HALF_FP8,

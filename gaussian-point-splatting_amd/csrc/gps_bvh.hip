@@ -16,8 +16,8 @@
 // order of the Gaussians inside a node.  The build therefore runs
 // level-synchronously: one pass bins every Gaussian of every open node,
 // one thread per node evaluates the SAH exactly as tinybvh does, and a
-// partition pass scatters each Gaussian to its child's range with
-// wave-aggregated atomics.  Nodes of at most 32 Gaussians are
+// partition pass scatters each Gaussian to its child's range (ranks in
+// LDS, one cursor atomic per node side and workgroup).  Nodes of at most 32 Gaussians are
 // finished by one thread each with the sequential algorithm.
 //
 // Float min/max go through an order-preserving integer encoding (atomic
@@ -470,6 +470,7 @@ __global__ __launch_bounds__(THREADS) void bvh_partition_kernel(const Frag* __re
     __shared__ uint32_t s_k0, s_m;
     __shared__ uint32_t s_first[MAXN], s_count[MAXN], s_axis[MAXN], s_pos[MAXN], s_left[MAXN], s_kind[MAXN];
     __shared__ float s_nmin[MAXN], s_rpd[MAXN];
+    __shared__ uint32_t s_cnt[2 * MAXN], s_base[2 * MAXN];
     const uint32_t s = blockIdx.x * CHUNK, tid = threadIdx.x;
     if (tid == 0) {
         uint32_t k0;
@@ -486,49 +487,48 @@ __global__ __launch_bounds__(THREADS) void bvh_partition_kernel(const Frag* __re
         s_axis[tid] = sp.axis, s_pos[tid] = sp.pos, s_left[tid] = sp.left, s_kind[tid] = sp.kind;
         s_nmin[tid] = sp.nmin, s_rpd[tid] = sp.rpd;
     }
+    for (uint32_t i = tid; i < 2 * MAXN; i += THREADS) s_cnt[i] = 0u;
     __syncthreads();
-    const uint32_t lane = tid & 63;
-    for (uint32_t p0 = s; p0 < s + CHUNK; p0 += THREADS) {
-        const uint32_t p = p0 + tid;  // every lane of a wave runs each round
+    // ranks inside the workgroup per (node, side), then one global cursor
+    // bump per (node, side) for the whole workgroup
+    constexpr uint32_t R = CHUNK / THREADS;
+    int key[R];
+    uint32_t rank[R];
+    Frag f[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t p = s + r * THREADS + tid;
         const int j = p < n ? find_slot(s_first, s_count, m, p) : -1;
-        int key = -1;
-        Frag f;
-        if (j >= 0) {
-            if (!s_kind[j]) {
-                // leaf: positions [f, f + c) take old indices n - f - c + (q - f)
-                perm[p] = (n - s_first[j] - s_count[j]) + (p - s_first[j]);
-            } else {
-                f = src[p];
-                const uint32_t a = s_axis[j];
-                const float c = a == 0 ? (f.lx + f.hx) * 0.5f : a == 1 ? (f.ly + f.hy) * 0.5f : (f.lz + f.lz) * 0.5f;
-                const int32_t bi = clampi((int32_t)cvt_u32((c - s_nmin[j]) * s_rpd[j]), 0, BINS - 1);
-                key = 2 * j + ((uint32_t)bi <= s_pos[j] ? 0 : 1);
-            }
+        key[r] = -1;
+        if (j < 0) continue;
+        if (!s_kind[j]) {
+            // leaf: positions [f, f + c) take old indices n - f - c + (q - f)
+            perm[p] = (n - s_first[j] - s_count[j]) + (p - s_first[j]);
+            continue;
         }
-        // wave-aggregated cursor bumps, one atomic per (node, side) group
-        uint32_t slot = 0;
-        uint64_t pending = __ballot(key >= 0);
-        while (pending) {
-            const int leader = __ffsll((long long)pending) - 1;
-            const int lk = __shfl(key, leader, 64);
-            const uint64_t grp = __ballot(key == lk);
-            uint32_t base = 0;
-            if ((int)lane == leader) {
-                Split& sp = splits[s_k0 + (uint32_t)(lk >> 1)];
-                base = atomicAdd((lk & 1) ? &sp.curR : &sp.curL, (uint32_t)__popcll(grp));
-            }
-            base = (uint32_t)__shfl((int)base, leader, 64);
-            if (key == lk) slot = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
-            pending &= ~grp;
-        }
-        if (key >= 0) {
-            const uint32_t jj = (uint32_t)(key >> 1);
-            const uint32_t cap = (key & 1) ? s_count[jj] - s_left[jj] : s_left[jj];
-            if (slot < cap)
-                dst[s_first[jj] + ((key & 1) ? s_left[jj] + slot : slot)] = f;
-            else
-                atomicOr(err, 2u);
-        }
+        f[r] = src[p];
+        const uint32_t a = s_axis[j];
+        const float c = centroid(f[r], a);
+        const int32_t bi = clampi((int32_t)cvt_u32((c - s_nmin[j]) * s_rpd[j]), 0, BINS - 1);
+        key[r] = 2 * j + ((uint32_t)bi <= s_pos[j] ? 0 : 1);
+        rank[r] = atomicAdd(&s_cnt[key[r]], 1u);
+    }
+    __syncthreads();
+    if (tid < 2 * m && s_cnt[tid]) {
+        Split& sp = splits[s_k0 + (tid >> 1)];
+        s_base[tid] = atomicAdd((tid & 1) ? &sp.curR : &sp.curL, s_cnt[tid]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        if (key[r] < 0) continue;
+        const uint32_t jj = (uint32_t)(key[r] >> 1);
+        const uint32_t slot = s_base[key[r]] + rank[r];
+        const uint32_t cap = (key[r] & 1) ? s_count[jj] - s_left[jj] : s_left[jj];
+        if (slot < cap)
+            dst[s_first[jj] + ((key[r] & 1) ? s_left[jj] + slot : slot)] = f[r];
+        else
+            atomicOr(err, 2u);
     }
 }
 
@@ -539,16 +539,14 @@ struct Task {
     float mn[3], mx[3];
     uint32_t first, count, tc, depth;
 };
-__global__ __launch_bounds__(64) void bvh_small_kernel(const Frag* __restrict__ fr, uint32_t n,
+__global__ __launch_bounds__(64) void bvh_small_kernel(Frag* __restrict__ fr, uint32_t n,
                                                        const ONode* __restrict__ small, uint32_t ns,
                                                        const float* __restrict__ min_dim, int threaded,
                                                        uint32_t* __restrict__ perm) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     const ONode sn = small[k];
-    const uint32_t base = sn.first;
-    uint8_t idx[SMALL];
-    for (uint32_t i = 0; i < sn.count; ++i) idx[i] = (uint8_t)i;
+    const uint32_t base = sn.first;  // this thread owns [base, base + count)
     Task stack[SMALL];
     uint32_t sp = 0;
     Task t;
@@ -574,7 +572,7 @@ __global__ __launch_bounds__(64) void bvh_small_kernel(const Frag* __restrict__ 
                     for (int c = 0; c < 3; ++c) bmn[i][c] = FAR, bmx[i][c] = -FAR;
                 }
                 for (uint32_t i = 0; i < t.count; ++i) {
-                    const Frag f = fr[base + idx[t.first + i]];
+                    const Frag f = fr[base + t.first + i];
                     const int32_t bi = clampi(cvt_i32((centroid(f, a) - t.mn[a]) * rpd3[a]), 0, BINS - 1);
 #pragma unroll
                     for (int b = 0; b < BINS; ++b)
@@ -592,20 +590,19 @@ __global__ __launch_bounds__(64) void bvh_small_kernel(const Frag* __restrict__ 
             const float rSAV = 1.0f / node_area(t.mn, t.mx);
             const float splitCost = 1.0f + 1.0f * rSAV * best.cost;
             if (!(splitCost >= (float)t.count * 1.0f)) {
-                // in-place partition of the index list (tiny_bvh.h:2372-2380)
+                // in-place partition of the node's fragments (tiny_bvh.h:2372-2380)
                 const uint32_t a = best.axis;
                 const float rpd = rpd3[a], nmin = t.mn[a];
                 uint32_t j = t.first + t.count, src = t.first;
                 for (uint32_t i = 0; i < t.count; ++i) {
-                    const Frag f = fr[base + idx[src]];
+                    const Frag f = fr[base + src];
                     const int32_t bi = clampi((int32_t)cvt_u32((centroid(f, a) - nmin) * rpd), 0, BINS - 1);
                     if ((uint32_t)bi <= best.pos) {
                         ++src;
                     } else {
                         --j;
-                        const uint8_t tmp = idx[src];
-                        idx[src] = idx[j];
-                        idx[j] = tmp;
+                        fr[base + src] = fr[base + j];
+                        fr[base + j] = f;
                     }
                 }
                 const uint32_t left = src - t.first, right = t.count - left;

@@ -49,6 +49,7 @@ struct Options {
     float rotation[3] = {0, 0, 0}, translation[3] = {0, 0, 0}, scaling[3] = {1, 1, 1};
     std::string model_path;
     bool sort_morton = false;
+    std::string sorted_model_output_path;
     // headless
     std::string synthetic = "c1";
     long long num_gaussians = -1;
@@ -112,8 +113,9 @@ Options parse(int argc, char** argv) {
         else if (a == "--play_camera_path" || a == "--play-camera-path") o.play_camera_path = true;
         else if (a == "--frames-per-camera-keyframe") o.frames_per_camera_keyframe = std::stoi(next());
         else if (a == "--total-test-keyframes") o.total_test_keyframes = std::stoi(next());
-        else if (a == "--max-eval-count" || a == "--max_eval_count" || a == "--sorted-model-output-path" ||
-                 a == "--sorted_model_output_path") next();
+        else if (a == "--sorted-model-output-path" || a == "--sorted_model_output_path")
+            o.sorted_model_output_path = next();
+        else if (a == "--max-eval-count" || a == "--max_eval_count") next();
         else if (a == "--freeze-culling" || a == "--enable_profiling" || a == "--disable_gui" || a == "--disable-gui" ||
                  a == "--snap_window_to_corner" || a == "--snap-window-to-corner" ||
                  a == "--create-evaluation-screenshots" || a == "--create_evaluation_screenshots") {
@@ -230,8 +232,8 @@ int main(int argc, char** argv) {
         sp.dc_sigma = 0.3;
     }
     if (o.num_gaussians >= 0) sp.num_gaussians = (uint64_t)o.num_gaussians;
+    gps_ply_info info{};
     if (!o.model_path.empty()) {
-        gps_ply_info info;
         CHECK(gps_ply_info_read(o.model_path.c_str(), &info));
         sp.num_gaussians = info.num_gaussians;
         sp.sh_degree = 3;
@@ -239,36 +241,65 @@ int main(int argc, char** argv) {
     const uint64_t N = sp.num_gaussians;
     const int nc = (sp.sh_degree + 1) * (sp.sh_degree + 1);
     std::vector<float> means(3 * N), rots(4 * N), scales(3 * N), ops(N), shs((size_t)3 * nc * N);
+    auto apply_perm = [&](const std::vector<uint32_t>& perm) {
+        auto permute = [&](std::vector<float>& a, size_t w) {
+            std::vector<float> t(a.size());
+            for (uint64_t i = 0; i < N; ++i) std::memcpy(&t[w * i], &a[w * perm[i]], 4 * w);
+            a.swap(t);
+        };
+        permute(means, 3);
+        permute(rots, 4);
+        permute(scales, 3);
+        permute(ops, 1);
+        permute(shs, (size_t)3 * nc);
+    };
+    auto fill_desc = [&](gps_scene_desc& d) {
+        d.num_gaussians = N;
+        d.sh_degree = sp.sh_degree;
+        d.format = o.format;
+        d.means = means.data();
+        d.rotations = rots.data();
+        d.scales = scales.data();
+        d.opacities = ops.data();
+        d.shs = shs.data();
+    };
     if (!o.model_path.empty()) {
         uint64_t got = 0;
+        if (!o.sorted_model_output_path.empty()) {
+            // main.cu:266-273: load without Morton order, BVH reorder, save
+            // with the "bvh_sorted_order" comment
+            CHECK(gps_ply_load(o.model_path.c_str(), 0, N, means.data(), rots.data(), scales.data(), ops.data(),
+                               shs.data(), &got));
+            if (N > 1) {
+                std::vector<uint32_t> perm(N);
+                CHECK(gps_bvh_order(means.data(), rots.data(), scales.data(), N, perm.data()));
+                apply_perm(perm);
+            }
+            gps_scene_desc sd;
+            fill_desc(sd);
+            CHECK(gps_ply_write(o.sorted_model_output_path.c_str(), &sd, 1));
+            std::printf("Wrote BVH-sorted model: %s\n", o.sorted_model_output_path.c_str());
+        }
         CHECK(gps_ply_load(o.model_path.c_str(), o.sort_morton ? 1 : 0, N, means.data(), rots.data(), scales.data(),
                            ops.data(), shs.data(), &got));
+        // gaussian_point_splatting.cu:791-797: the BVH reorder unless the
+        // scene was Morton-sorted on load or carries "bvh_sorted_order"
+        if (!o.sort_morton && !info.bvh_sorted && N > 1) {
+            std::vector<uint32_t> perm(N);
+            CHECK(gps_bvh_order(means.data(), rots.data(), scales.data(), N, perm.data()));
+            apply_perm(perm);
+        }
+        std::printf("is pre-sorted: %s\n", info.bvh_sorted ? "True" : "false");
     } else {
         CHECK(gps_synth_generate(&sp, 0, N, means.data(), rots.data(), scales.data(), ops.data(), shs.data()));
         if (o.sort_morton && N > 1) {  // gaussian_loader.cu:363-426
             std::vector<uint32_t> perm(N);
             CHECK(gps_morton_order(means.data(), N, perm.data()));
-            auto permute = [&](std::vector<float>& a, size_t w) {
-                std::vector<float> t(a.size());
-                for (uint64_t i = 0; i < N; ++i) std::memcpy(&t[w * i], &a[w * perm[i]], 4 * w);
-                a.swap(t);
-            };
-            permute(means, 3);
-            permute(rots, 4);
-            permute(scales, 3);
-            permute(ops, 1);
-            permute(shs, (size_t)3 * nc);
+            apply_perm(perm);
         }
     }
     gps_scene_desc d;
-    d.num_gaussians = N;
-    d.sh_degree = sp.sh_degree;
-    d.format = o.format;
-    d.means = means.data();
-    d.rotations = rots.data();
-    d.scales = scales.data();
-    d.opacities = ops.data();
-    d.shs = shs.data();
+    fill_desc(d);
     gps_scene* scene = nullptr;
     CHECK(gps_scene_create(&d, &scene));
     // camera path (RenderSettings::load_camera, render_settings.cuh:248-256)

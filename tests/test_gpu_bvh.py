@@ -59,3 +59,30 @@ def test_bvh_order_synthetic_scene(gpu):
     a = gpu.synth_scene(p)
     print("bvh_order synth n=200000", flush=True)
     assert np.array_equal(a.bvh_order(), orc.bvh_order(a.means, a.rotations, a.scales))
+
+
+def test_cli_ply_bvh_reorder(gpu, tmp_path):
+    """The headless driver on a PLY without "bvh_sorted_order": it reorders
+    on load (gaussian_point_splatting.cu:791-797), and
+    --sorted-model-output-path writes the reordered scene with the comment
+    (main.cu:266-273); a file carrying the comment is not reordered again."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "gaussian-point-splatting_amd", "gps_render")
+    p = gpu.synth_params(0x5EED0077, 20_000, kind="clusters", clusters=8, sh_degree=3, half_extent=10.0)
+    src = str(tmp_path / "scene.ply")
+    gpu.write_ply(src, gpu.synth_scene(p))
+    loaded = gpu.load_ply(src)
+    perm = orc.bvh_order(loaded.means, loaded.rotations, loaded.scales)
+    out = str(tmp_path / "sorted.ply")
+    print("cli: ply bvh reorder", flush=True)
+    r = subprocess.run([exe, "--model-path", src, "--sorted-model-output-path", out, "--resolution", "160", "90",
+                        "--frames", "1", "--out", str(tmp_path / "a")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "is pre-sorted: false" in r.stdout
+    assert gpu.ply_info(out)["bvh_sorted"]
+    assert np.array_equal(gpu.load_ply(out).means, loaded.means[perm])
+    r = subprocess.run([exe, "--model-path", out, "--resolution", "160", "90", "--frames", "1",
+                        "--out", str(tmp_path / "b")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "is pre-sorted: True" in r.stdout
