@@ -103,9 +103,24 @@ GPS_HD float dilog(float x) {
 // Li2(1) = pi^2 / 6
 #define GPS_LI2_MAX 1.6449340668482264f
 
+// x / GPS_LI2_MAX with IEEE rounding in three operations instead of a full
+// division: q0 = x * RN(1/c), the remainder x - q0 * c exactly in one fma,
+// one fma correction (Markstein's theorem: with RN(1/c) and q0 within an ulp
+// the corrected quotient is the correctly rounded one while the remainder
+// does not underflow).  Checked exhaustively against x / c for every
+// positive normal float whose quotient is at least 2^-100; smaller, zero
+// and NaN inputs take the division.
+GPS_HD float det_div_li2max(float x) {
+    constexpr float z = 1.0f / GPS_LI2_MAX;
+    if (!(det_abs(x) >= 0x1.0p-99f)) return x / GPS_LI2_MAX;
+    const float q0 = x * z;
+    const float r = det_fma(-q0, GPS_LI2_MAX, x);
+    return det_fma(r, z, q0);
+}
+
 // Polynomial fit of the inverse of Li2 (random.cuh:110-127).
 GPS_HD float inv_dilog(float x) {
-    const float t = det_fmin(x / GPS_LI2_MAX, 1.0f);
+    const float t = det_fmin(det_div_li2max(x), 1.0f);
     float y = -1.27463503e+01f;
     y = det_fma(y, t, 5.88993459e+01f);
     y = det_fma(y, t, -1.16025780e+02f);
