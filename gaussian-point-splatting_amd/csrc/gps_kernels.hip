@@ -856,7 +856,8 @@ __device__ __forceinline__ bool alpha_passes(const SplatRecord& rec, float sigma
 // of one Gaussian (splatGaussianPoint, gaussian_point_splatting.cu:67-141):
 // sample q = layer * K + k lands in layer q / K.  Returns the atomics issued.
 __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t p, uint32_t frame_seed, int K,
-                                                int spp, int up_w, int unbiased, unsigned long long* __restrict__ fb) {
+                                                int spp, int up_w, int unbiased, unsigned long long* __restrict__ fb,
+                                                int& last) {
     const int bminx = (int)(rec.bmin & 0xFFFFu), bminy = (int)(rec.bmin >> 16);
     const int bmaxx = (int)(rec.bmax & 0xFFFFu), bmaxy = (int)(rec.bmax >> 16);
     const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
@@ -882,6 +883,10 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
         const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
         if (!alpha_passes(rec, sigma)) continue;
         const int idx = fb_index(x, y, layer, spp, up_w);
+        // The word this thread just read or wrote for the same Gaussian
+        // (same packed value): a second atomicMin would change nothing.
+        if (idx == last) continue;
+        last = idx;
         if (fb[idx] <= depth_bits) continue;
         atomicMin(fb + idx, packed);
         ++issued;
@@ -932,6 +937,7 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
             int j = lo;
             uint32_t cur_g = 0xFFFFFFFFu;
             SplatRecord rec;
+            int last = -1;
             for (int k = 0; k < GPS_SPLAT_PPT; ++k) {
                 const uint32_t p = p0 + (uint32_t)k;
                 if (p >= P) break;
@@ -941,8 +947,9 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
                 if (g != cur_g) {
                     cur_g = g;
                     rec = load_record(recs, g);
+                    last = -1;
                 }
-                issued += splat_point(rec, p, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb);
+                issued += splat_point(rec, p, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb, last);
             }
         }
         __syncthreads();
@@ -1464,8 +1471,9 @@ __global__ __launch_bounds__(256) void bin_big_kernel(const SplatRecord* __restr
         const uint32_t r = big[e];
         const SplatRecord rec = load_record(recs, cidx[r]);
         const uint32_t p0 = coff[r], w = ew[r];
+        int last = -1;  // one Gaussian throughout
         for (uint32_t k = (uint32_t)tid; k < w; k += 256u)
-            issued += splat_point(rec, p0 + k, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb);
+            issued += splat_point(rec, p0 + k, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb, last);
     }
     if (atomics_count) add_count(atomics_count, issued);
 }
@@ -1592,8 +1600,10 @@ __global__ __launch_bounds__(256) void batch_bin_big_kernel(const SplatRecord* _
         const SplatRecord rec = load_record(recs, cidx[r]);
         const uint32_t p0 = coff[r] - bv.base, w = ew[r];
         const uint32_t seed = bv.f.frame_seed + 1u;
+        int last = -1;  // one Gaussian throughout
         for (uint32_t k = (uint32_t)tid; k < w; k += 256u)
-            issued += splat_point(rec, p0 + k, seed, bv.f.Ki, bv.f.spp, bv.f.up_w, bv.f.unbiased, fb + bv.fb_offset);
+            issued += splat_point(rec, p0 + k, seed, bv.f.Ki, bv.f.spp, bv.f.up_w, bv.f.unbiased, fb + bv.fb_offset,
+                                  last);
     }
     if (atomics_count) add_count(atomics_count, issued);
 }
@@ -2115,7 +2125,8 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
                 if (vskip) continue;  // rendered by the binned kernels
                 const uint32_t pl = p - vbase;
                 if (pl >= vpoints) continue;  // beyond the view's point budget
-                issued += splat_point(rec, pl, vseed, vK, vspp, vup_w, vunb, fb + voff);
+                int last = -1;
+                issued += splat_point(rec, pl, vseed, vK, vspp, vup_w, vunb, fb + voff, last);
             }
         }
         __syncthreads();
