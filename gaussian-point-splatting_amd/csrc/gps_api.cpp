@@ -521,7 +521,7 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     // MI300-class parts (gfx942 / gfx950) have 8 XCDs of 32 or 38 CUs; the
     // chunk schedule only needs the count to be a divisor of the grid.
     p->xcds = (cus % 8 == 0) ? 8 : 1;
-    p->splat_grid = (uint32_t)cus * 16u;  // persistent; finer chunk interleave than one wave per SIMD slot
+    p->splat_grid = (uint32_t)cus * GPS_SPLAT_WGS_PER_CU;  // persistent; finer chunk interleave than one wave per SIMD slot
     p->cus = (uint32_t)cus;
     if (s == GPS_OK) {
         hipError_t e = hipDeviceSynchronize();

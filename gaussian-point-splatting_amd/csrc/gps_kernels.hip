@@ -857,7 +857,8 @@ __device__ __forceinline__ bool alpha_passes(const SplatRecord& rec, float sigma
 // sample q = layer * K + k lands in layer q / K.  Returns the atomics issued.
 __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t p, uint32_t frame_seed, int K,
                                                 int spp, int up_w, int unbiased, unsigned long long* __restrict__ fb,
-                                                int& last) {
+                                                int& last, uint32_t* s_key = nullptr,
+                                                unsigned long long* s_val = nullptr) {
     const int bminx = (int)(rec.bmin & 0xFFFFu), bminy = (int)(rec.bmin >> 16);
     const int bmaxx = (int)(rec.bmax & 0xFFFFu), bmaxy = (int)(rec.bmax >> 16);
     const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
@@ -887,6 +888,22 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
         // (same packed value): a second atomicMin would change nothing.
         if (idx == last) continue;
         last = idx;
+#if GPS_SPLAT_SLOTS
+        if (s_key) {
+            // merge into the chunk's LDS word table when the slot is free or
+            // already holds this word; otherwise go to the framebuffer
+            const uint32_t slot = ((uint32_t)idx ^ ((uint32_t)idx >> 11)) & (GPS_SPLAT_SLOTS - 1u);
+            uint32_t key = s_key[slot];
+            if (key == 0xFFFFFFFFu) {
+                const uint32_t old = atomicCAS(&s_key[slot], 0xFFFFFFFFu, (uint32_t)idx);
+                key = old == 0xFFFFFFFFu ? (uint32_t)idx : old;
+            }
+            if (key == (uint32_t)idx) {
+                atomicMin(&s_val[slot], packed);
+                continue;
+            }
+        }
+#endif
         if (fb[idx] <= depth_bits) continue;
         atomicMin(fb + idx, packed);
         ++issued;
@@ -904,12 +921,25 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
                                                                   uint32_t* __restrict__ atomics_count,
                                                                   const uint32_t* __restrict__ only_if) {
     __shared__ uint32_t s_off[GPS_SPLAT_CHUNK + 1];
+#if GPS_SPLAT_SLOTS
+    __shared__ uint32_t s_key[GPS_SPLAT_SLOTS];
+    __shared__ unsigned long long s_val[GPS_SPLAT_SLOTS];
+#else
+    uint32_t* const s_key = nullptr;
+    unsigned long long* const s_val = nullptr;
+#endif
     if (only_if && *only_if == 0u) return;  // the binned splat covers this pass
     const uint32_t P = ctr->points;
     const uint32_t V = ctr->valid;
     const uint32_t nchunks = ctr->chunks;
     const int tid = threadIdx.x;
     uint32_t issued = 0;
+#if GPS_SPLAT_SLOTS
+    for (int i = tid; i < GPS_SPLAT_SLOTS; i += GPS_SPLAT_THREADS) {
+        s_key[i] = 0xFFFFFFFFu;
+        s_val[i] = ~0ull;
+    }  // ordered before the first insert by the barrier after the offsets
+#endif
     // Workgroups are dispatched round-robin over the XCDs (blockIdx % xcds);
     // give every XCD one contiguous range of chunks so the framebuffer lines
     // of a compact scene region stay in that XCD's L2.
@@ -949,10 +979,27 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
                     rec = load_record(recs, g);
                     last = -1;
                 }
-                issued += splat_point(rec, p, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb, last);
+                issued += splat_point(rec, p, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb, last, s_key,
+                                      s_val);
             }
         }
         __syncthreads();
+#if GPS_SPLAT_SLOTS
+        // flush the chunk's merged words and clear the table; the next
+        // chunk inserts only after the barrier that follows its offsets
+        for (int i = tid; i < GPS_SPLAT_SLOTS; i += GPS_SPLAT_THREADS) {
+            const uint32_t key = s_key[i];
+            if (key != 0xFFFFFFFFu) {
+                const unsigned long long v = s_val[i];
+                if (fb[key] > v) {
+                    atomicMin(fb + key, v);
+                    ++issued;
+                }
+                s_key[i] = 0xFFFFFFFFu;
+                s_val[i] = ~0ull;
+            }
+        }
+#endif
     }
     if (atomics_count) add_count(atomics_count, issued);
 }
