@@ -189,6 +189,7 @@ BatchArgs args(gps_batch* b, int nviews) {
     a.hier = b->hier_available ? 1 : 0;
     a.fb = b->fb;
     a.max_fb_words = b->fb_view_words;
+    a.fb_key32 = (uint64_t)(uint32_t)a.nviews * b->fb_view_words < 0xFFFFFFFFull ? 1 : 0;
     a.grid = b->grid;
     a.xcds = b->xcds;
     a.threads = b->threads;

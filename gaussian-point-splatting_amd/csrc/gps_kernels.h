@@ -121,6 +121,7 @@ struct BatchArgs {
     int32_t hier;
     uint64_t* fb;            // pool; views address it by fb_offset
     uint64_t max_fb_words;   // largest view
+    int32_t fb_key32;        // every word of the batch's views has a 32-bit index (below 2^32 - 1)
     uint32_t grid;
     int32_t xcds;
     int32_t threads, ppt;    // splat variant

@@ -858,7 +858,7 @@ __device__ __forceinline__ bool alpha_passes(const SplatRecord& rec, float sigma
 __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t p, uint32_t frame_seed, int K,
                                                 int spp, int up_w, int unbiased, unsigned long long* __restrict__ fb,
                                                 int& last, uint32_t* s_key = nullptr,
-                                                unsigned long long* s_val = nullptr) {
+                                                unsigned long long* s_val = nullptr, uint32_t key_base = 0u) {
     const int bminx = (int)(rec.bmin & 0xFFFFu), bminy = (int)(rec.bmin >> 16);
     const int bmaxx = (int)(rec.bmax & 0xFFFFu), bmaxy = (int)(rec.bmax >> 16);
     const unsigned long long packed = ((unsigned long long)rec.packed_hi << 32) | rec.packed_lo;
@@ -892,13 +892,15 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
         if (s_key) {
             // merge into the chunk's LDS word table when the slot is free or
             // already holds this word; otherwise go to the framebuffer
-            const uint32_t slot = ((uint32_t)idx ^ ((uint32_t)idx >> 11)) & (GPS_SPLAT_SLOTS - 1u);
+            // key: the word's index from the table's framebuffer base
+            const uint32_t word = key_base + (uint32_t)idx;
+            const uint32_t slot = (word ^ (word >> 11)) & (GPS_SPLAT_SLOTS - 1u);
             uint32_t key = s_key[slot];
             if (key == 0xFFFFFFFFu) {
-                const uint32_t old = atomicCAS(&s_key[slot], 0xFFFFFFFFu, (uint32_t)idx);
-                key = old == 0xFFFFFFFFu ? (uint32_t)idx : old;
+                const uint32_t old = atomicCAS(&s_key[slot], 0xFFFFFFFFu, word);
+                key = old == 0xFFFFFFFFu ? word : old;
             }
-            if (key == (uint32_t)idx) {
+            if (key == word) {
                 atomicMin(&s_val[slot], packed);
                 continue;
             }
@@ -2107,15 +2109,34 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
                                                               const BatchView* __restrict__ views,
                                                               unsigned long long* __restrict__ fb, int32_t xcds,
                                                               uint32_t* __restrict__ atomics_count,
-                                                              const uint32_t* __restrict__ bin_state) {
+                                                              const uint32_t* __restrict__ bin_state,
+                                                              int use_table) {
     constexpr int CHUNK = THREADS * PPT;
     const bool bin_ran = bin_state && bin_state[3] == 0u;
     __shared__ uint32_t s_off[CHUNK + 1];
+#if GPS_SPLAT_SLOTS
+    // word table as in splat_kernel, keyed by the word's index in the whole
+    // batch framebuffer (use_table: that index fits 32 bits)
+    __shared__ uint32_t s_key_[GPS_SPLAT_SLOTS];
+    __shared__ unsigned long long s_val_[GPS_SPLAT_SLOTS];
+    uint32_t* const s_key = use_table ? s_key_ : nullptr;
+    unsigned long long* const s_val = s_val_;
+#else
+    uint32_t* const s_key = nullptr;
+    unsigned long long* const s_val = nullptr;
+#endif
     const uint32_t P = ctr->points;
     const uint32_t V = ctr->valid;
     const uint32_t nchunks = (uint32_t)(((uint64_t)P + CHUNK - 1) / CHUNK);
     const int tid = threadIdx.x;
     uint32_t issued = 0;
+#if GPS_SPLAT_SLOTS
+    if (s_key)
+        for (int i = tid; i < GPS_SPLAT_SLOTS; i += THREADS) {
+            s_key[i] = 0xFFFFFFFFu;
+            s_val[i] = ~0ull;
+        }
+#endif
     const uint32_t nx = (uint32_t)xcds;
     const uint32_t xcd = blockIdx.x % nx;
     const uint32_t per_xcd = gridDim.x / nx;
@@ -2144,6 +2165,7 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
             int vK = 1, vspp = 1, vup_w = 1, vunb = 1;
             bool vskip = false;
             uint64_t voff = 0;
+            int last = -1;
             for (int k = 0; k < PPT; ++k) {
                 const uint32_t p = p0 + (uint32_t)k;
                 if (p >= P) break;
@@ -2168,15 +2190,31 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
                         vskip = bin_ran && bv.bg.nregions != 0u;
                     }
                     if (!vskip) rec = load_record(recs, g);
+                    last = -1;
                 }
                 if (vskip) continue;  // rendered by the binned kernels
                 const uint32_t pl = p - vbase;
                 if (pl >= vpoints) continue;  // beyond the view's point budget
-                int last = -1;
-                issued += splat_point(rec, pl, vseed, vK, vspp, vup_w, vunb, fb + voff, last);
+                issued += splat_point(rec, pl, vseed, vK, vspp, vup_w, vunb, fb + voff, last, s_key, s_val,
+                                      (uint32_t)voff);
             }
         }
         __syncthreads();
+#if GPS_SPLAT_SLOTS
+        if (s_key)
+            for (int i = tid; i < GPS_SPLAT_SLOTS; i += THREADS) {
+                const uint32_t key = s_key[i];
+                if (key != 0xFFFFFFFFu) {
+                    const unsigned long long v = s_val[i];
+                    if (fb[key] > v) {
+                        atomicMin(fb + key, v);
+                        ++issued;
+                    }
+                    s_key[i] = 0xFFFFFFFFu;
+                    s_val[i] = ~0ull;
+                }
+            }
+#endif
     }
     if (atomics_count) add_count(atomics_count, issued);
 }
@@ -2385,7 +2423,7 @@ template <int THREADS, int PPT> static void launch_batch_splat_t(const BatchArgs
                        (const SplatRecord*)a.recs, (const uint32_t*)a.cidx, (const uint32_t*)a.coff,
                        (const uint8_t*)a.cview, (const uint32_t*)a.chunk_start, (const PassCounters*)a.ctr,
                        (const BatchView*)a.views, reinterpret_cast<unsigned long long*>(a.fb), a.xcds, a.atomics,
-                       a.bin.any ? (const uint32_t*)a.bin.state : nullptr);
+                       a.bin.any ? (const uint32_t*)a.bin.state : nullptr, a.fb_key32);
 }
 
 // Splat variants the batch autotuner chooses from: (threads, points per thread).
