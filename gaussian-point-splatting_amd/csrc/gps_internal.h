@@ -155,6 +155,11 @@ static_assert((1u << GPS_SPLAT_CHUNK_SHIFT) == GPS_SPLAT_CHUNK, "chunk is a powe
 #ifndef GPS_SPLAT_WGS_PER_CU
 #define GPS_SPLAT_WGS_PER_CU 10
 #endif
+// A sample whose slot holds another word tries the neighbouring slot before
+// the framebuffer (0.986 -> 0.979 ms; 16 workgroups per CU: 1.012).
+#ifndef GPS_SPLAT_PROBE2
+#define GPS_SPLAT_PROBE2 1
+#endif
 static_assert((GPS_SPLAT_SLOTS & (GPS_SPLAT_SLOTS - 1)) == 0, "slots: a power of two");
 // Binned splat: one 1024-thread workgroup renders a screen region of at most
 // GPS_BIN_WORDS framebuffer words in LDS; GPS_BIN_PPT points per thread per

@@ -904,6 +904,18 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
                 atomicMin(&s_val[slot], packed);
                 continue;
             }
+#if GPS_SPLAT_PROBE2
+            const uint32_t slot2 = slot ^ 1u;
+            key = s_key[slot2];
+            if (key == 0xFFFFFFFFu) {
+                const uint32_t old = atomicCAS(&s_key[slot2], 0xFFFFFFFFu, word);
+                key = old == 0xFFFFFFFFu ? word : old;
+            }
+            if (key == word) {
+                atomicMin(&s_val[slot2], packed);
+                continue;
+            }
+#endif
         }
 #endif
         if (fb[idx] <= depth_bits) continue;
