@@ -500,7 +500,10 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
     uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h) {
     __shared__ PreItem s_item[GPS_PRE_THREADS];
-    __shared__ ColourItem s_col[GPS_PRE_THREADS];
+    // list 2 reuses list 1's storage (read into registers before a barrier):
+    // 16 KB of LDS per workgroup, eight workgroups per CU
+    static_assert(sizeof(ColourItem) <= sizeof(PreItem), "list 2 fits in list 1");
+    ColourItem* const s_col = reinterpret_cast<ColourItem*>(s_item);
     __shared__ uint32_t s_count[GPS_PRE_THREADS];
     __shared__ uint32_t s_n1, s_n2;
     const int tid = threadIdx.x;
@@ -569,8 +572,11 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     }
     __syncthreads();
     // Phase B
-    if ((uint32_t)tid < s_n1) {
-        const PreItem q = s_item[tid];
+    const bool in1 = (uint32_t)tid < s_n1;
+    PreItem q;
+    if (in1) q = s_item[tid];
+    __syncthreads();  // list 1 is in registers before list 2 overwrites it
+    if (in1) {
         u32x2 seed = make_seed(q.g, f.frame_seed);
         const float s_det = det_sqrt(q.det);
         const float dl = dilog(q.op);
