@@ -645,6 +645,7 @@ void make_frame_consts(const gps_frame_params* fp, int spp, FrameConsts* out) {
     f.up_w = fp->width * ss;
     f.up_h = fp->height * ss;
     f.ss = ss;
+    f.ss_shift = (ss > 0 && (ss & (ss - 1)) == 0) ? __builtin_ctz((unsigned)ss) : -1;
     f.spp = spp;
     f.Ki = fp->reduce_point_count ? ss * ss : 1;
     f.K = fp->reduce_point_count ? (float)(ss * ss) : 1.0f;

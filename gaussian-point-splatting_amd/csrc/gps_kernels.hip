@@ -105,16 +105,25 @@ __device__ __forceinline__ bool hiz_occluded(const HiZView& h, float depth, int 
     lod = min(max(lod, 0), h.levels - 1);
     const int l = (h.levels - 1) - lod;
     const int wl = h.lw[l], hl = h.lh[l];
-    const int tileW = max(1, min(h.width, 1 << (h.levels - l - 1)));
-    const int tileH = max(1, min(h.height, 1 << (h.levels - l - 1)));
-    const int tx0 = max(0, min(wl - 1, minx / tileW));
-    const int ty0 = max(0, min(hl - 1, miny / tileH)) * wl;
-    const int tx1 = max(0, min(wl - 1, maxx / tileW));
-    const int ty1 = max(0, min(hl - 1, maxy / tileH)) * wl;
+    // The tile is max(1, min(size, 2^k)) texels with k = levels - l - 1 >= 0:
+    // a power of two (a shift), or the whole image, which the coordinates
+    // (clamped to [0, size - 1] above) divide to 0.
+    const int k = h.levels - l - 1;
+    const bool kx = (1 << k) <= h.width, ky = (1 << k) <= h.height;
+    const int tx0 = max(0, min(wl - 1, kx ? minx >> k : 0));
+    const int ty0 = max(0, min(hl - 1, ky ? miny >> k : 0)) * wl;
+    const int tx1 = max(0, min(wl - 1, kx ? maxx >> k : 0));
+    const int ty1 = max(0, min(hl - 1, ky ? maxy >> k : 0)) * wl;
     const float* m = h.mip + h.off[l];
     const float zmax = det_fmax(m[tx0 + ty0], det_fmax(m[tx1 + ty0], det_fmax(m[tx0 + ty1], m[tx1 + ty1])));
     return zmax < depth;
 }
+
+// A supersampled pixel bound in Hi-Z pixels: x / ss, as a shift when ss is a
+// power of two.  For negative x the shift rounds down where the division
+// truncates; hiz_occluded clamps both to 0, and only the minima can be
+// negative (a Gaussian with a negative maximum was rejected before).
+__device__ __forceinline__ int div_ss(const FrameConsts& f, int x) { return f.ss_shift >= 0 ? x >> f.ss_shift : x / f.ss; }
 
 // ------------------------------------------------------------------ scene access
 __device__ __forceinline__ float plane(const DevScene& s, int p, uint32_t g) {
@@ -412,7 +421,7 @@ __device__ __forceinline__ uint32_t preprocess_one(const DevScene& s, const Fram
     }
     // (The Hi-Z test is a pure rejection, so it is taken before the
     // importance terms it does not depend on.)
-    if (f.occl && hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss)) return 0u;
+    if (f.occl && hiz_occluded(h, pr.pz, div_ss(f, pr.minx), div_ss(f, pr.miny), div_ss(f, pr.maxx), div_ss(f, pr.maxy))) return 0u;
     const float s_det = det_sqrt(pr.det);
     const float dl = dilog(op);
     const float importance = f.unbiased ? GPS_TWO_PI_F * s_det * dl : op * GPS_TWO_PI_F * s_det;
@@ -546,7 +555,7 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
                     if (e1 < 0.02f || e2 < 0.02f) survives = false;
                 }
                 if (survives && f.occl &&
-                    hiz_occluded(h, pr.pz, pr.minx / f.ss, pr.miny / f.ss, pr.maxx / f.ss, pr.maxy / f.ss))
+                    hiz_occluded(h, pr.pz, div_ss(f, pr.minx), div_ss(f, pr.miny), div_ss(f, pr.maxx), div_ss(f, pr.maxy)))
                     survives = false;
                 if (survives) {
                     const uint32_t k = atomicAdd(&s_n1, 1u);

@@ -28,6 +28,7 @@ struct FrameConsts {
     int32_t ss, spp, Ki;
     uint32_t frame_seed;
     int32_t occl, hier, cull_small, unbiased, reduce, only_prev;
+    int32_t ss_shift;   // log2(ss) when ss is a power of two, else -1
 };
 
 struct Projected {
