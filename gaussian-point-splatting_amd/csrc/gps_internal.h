@@ -138,10 +138,16 @@ struct BatchView {
 
 // Splat work decomposition: one workgroup processes CHUNK consecutive points,
 // PPT consecutive points per thread.
+#ifndef GPS_SPLAT_THREADS
 #define GPS_SPLAT_THREADS 256
+#endif
+#ifndef GPS_SPLAT_PPT
 #define GPS_SPLAT_PPT 4
+#endif
 #define GPS_SPLAT_CHUNK (GPS_SPLAT_THREADS * GPS_SPLAT_PPT)
+#ifndef GPS_SPLAT_CHUNK_SHIFT
 #define GPS_SPLAT_CHUNK_SHIFT 10u  // log2(GPS_SPLAT_CHUNK)
+#endif
 static_assert((1u << GPS_SPLAT_CHUNK_SHIFT) == GPS_SPLAT_CHUNK, "chunk is a power of two");
 // Point-parallel splat: samples of one chunk that land on the same
 // framebuffer word merge in a direct-mapped LDS table of this many words
@@ -156,7 +162,9 @@ static_assert((1u << GPS_SPLAT_CHUNK_SHIFT) == GPS_SPLAT_CHUNK, "chunk is a powe
 #define GPS_SPLAT_WGS_PER_CU 10
 #endif
 // A sample whose slot holds another word tries the neighbouring slot before
-// the framebuffer (0.986 -> 0.979 ms; 16 workgroups per CU: 1.012).
+// the framebuffer (0.986 -> 0.979 ms; 16 workgroups per CU: 1.012).  Larger
+// chunks with the table measured slower: 512 threads x 4 points 1.107 ms,
+// 256 x 8 1.175, 1024 x 2 with 4096 words 1.058.
 #ifndef GPS_SPLAT_PROBE2
 #define GPS_SPLAT_PROBE2 1
 #endif
