@@ -78,6 +78,26 @@ def stage_bytes(w, st):
     }
 
 
+def atomic_rate(G, words: int) -> float:
+    """Random 64-bit atomicMin per second over a `words`-word buffer
+    (gps_selftest_atomics, best of 3): the memory-side update rate the
+    point-parallel splat's framebuffer atomics run against."""
+    buf = G.DeviceArray((words,), np.uint64, zero=True)
+    st = G.Stream()
+    e0, e1 = G.Event(), G.Event()
+    blocks, iters = 8192, 32
+    G.check(G.lib.gps_selftest_atomics(buf.ptr, words, blocks, iters, st.handle), "atomics")
+    st.synchronize()
+    best = 0.0
+    for _ in range(3):
+        e0.record(st)
+        G.check(G.lib.gps_selftest_atomics(buf.ptr, words, blocks, iters, st.handle), "atomics")
+        e1.record(st)
+        st.synchronize()
+        best = max(best, blocks * 256 * iters / (e0.elapsed_ms(e1) / 1e3))
+    return best
+
+
 def cpu_baseline(G, W, w, sample_n, threads, perm):
     """Oracle on a bounded sample: every (n/sample_n)-th Gaussian of the load
     order (a spatially uniform subset), first timed camera of the workload,
@@ -401,6 +421,14 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "algorithmic_bytes_per_launch": round(dom_bytes / dom_launches), "avg_launch_ms": dom_avg_ms}
+    if dom == "splat" and atomics:
+        # The splat's own bound: memory-side 64-bit atomics at random
+        # framebuffer words, priced against the measured random-atomicMin rate.
+        peak_atomics = atomic_rate(G, w.width * w.ss * w.height * w.ss * w.spp)
+        done = atomics / (stages["splat"]["ms"] / 1e3)
+        roofline["atomics"] = {"achieved_per_s": round(done), "peak_per_s": round(peak_atomics),
+                               "frac": round(done / peak_atomics, 4), "per_launch": round(atomics / dom_launches),
+                               "peak": "gps_selftest_atomics: random 64-bit atomicMin over the framebuffer's words"}
     stage_report = {k: {"ms_per_frame": round(v["ms"] / K, 4), "launches": v["launches"],
                         "GBps": round(sum(b[k] for b in bytes_frames) / max(v["ms"], 1e-9) / 1e6, 1)}
                     for k, v in stages.items() if v["launches"]}

@@ -57,7 +57,28 @@ __global__ __launch_bounds__(256) void selftest_copy_kernel(const uint4* __restr
         dst[i] = src[i];
 }
 
+// Random 64-bit atomicMin over `words` framebuffer-sized words: the
+// memory-side atomic rate the splat's framebuffer updates run against.  Word
+// index = (hash * words) >> 32, always in [0, words).
+__global__ __launch_bounds__(256) void selftest_atomics_kernel(unsigned long long* __restrict__ buf, uint64_t words,
+                                                               uint32_t iters, uint32_t seed) {
+    uint32_t x = (blockIdx.x * blockDim.x + threadIdx.x) ^ seed;
+    for (uint32_t i = 0; i < iters; ++i) {
+        x = hash32(x + 0x9E3779B9u);
+        const uint64_t idx = ((uint64_t)x * words) >> 32;
+        atomicMin(buf + idx, ((unsigned long long)hash32(x) << 32) | x);
+    }
+}
+
 }  // namespace gps
+
+extern "C" gps_status gps_selftest_atomics(uint64_t* d_buf, uint64_t words, uint32_t blocks, uint32_t iters,
+                                           void* stream) {
+    if (!d_buf || words == 0 || words > 0xFFFFFFFFull || blocks == 0) return GPS_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(gps::selftest_atomics_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<unsigned long long*>(d_buf), words, iters, 0x5EEDu);
+    return hipGetLastError() == hipSuccess ? GPS_OK : GPS_ERR_HIP;
+}
 
 extern "C" gps_status gps_selftest_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream) {
     if (!d_dst || !d_src || bytes % 16) return GPS_ERR_INVALID_ARG;

@@ -197,6 +197,7 @@ _sig("gps_selftest_math", C.c_int32, C.c_int32, _vp, _vp, C.c_uint64, _vp)
 _sig("gps_selftest_rng", C.c_int32, C.c_uint32, C.c_uint32, C.c_int32, _vp, _vp)
 _sig("gps_selftest_poisson", C.c_int32, C.c_uint32, _vp, _vp, _vp)
 _sig("gps_selftest_copy", C.c_int32, _vp, _vp, C.c_uint64, _vp)
+_sig("gps_selftest_atomics", C.c_int32, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp)
 
 lib = _lib
 

@@ -557,6 +557,10 @@ gps_status gps_selftest_rng(uint32_t n, uint32_t frame, int32_t draws, uint32_t*
 gps_status gps_selftest_poisson(uint32_t n, const float* d_lambda, uint32_t* d_out, void* stream);
 /* Streaming device copy (bytes % 16 == 0): reference kernel for HBM byte counters. */
 gps_status gps_selftest_copy(void* d_dst, const void* d_src, uint64_t bytes, void* stream);
+/* blocks x 256 threads each issue `iters` 64-bit atomicMin at hashed random
+ * words of d_buf[0, words) (words < 2^32): the random framebuffer-update rate
+ * the point-parallel splat's atomics are priced against. */
+gps_status gps_selftest_atomics(uint64_t* d_buf, uint64_t words, uint32_t blocks, uint32_t iters, void* stream);
 
 /* ------------------------------------------------------- device plumbing */
 /* Minimal HIP plumbing for hosts without a HIP binding (ctypes, cgo, JNI). */

@@ -608,3 +608,32 @@ def test_cli_camera_path_run_tests(gpu, tmp_path):
         head = fh.read(32).split()
     assert head[0] == b"P6" and (int(head[1]), int(head[2])) == (320, round(320 * 3286 / 4946))
     assert os.path.exists(out + f"_{3 * 4 + 1:04d}.ppm")
+
+
+def test_selftest_atomics_min(gpu):
+    """gps_selftest_atomics (the atomic-rate probe bench.py prices the splat
+    against) leaves every word at the minimum of the values hashed to it."""
+    words, blocks, iters = 1000, 3, 5
+    flush("selftest atomics")
+    buf = gpu.DeviceArray((words,), np.uint64)
+    buf.upload(np.full(words, np.iinfo(np.uint64).max, np.uint64))
+    gpu.check(gpu.lib.gps_selftest_atomics(buf.ptr, words, blocks, iters, None), "atomics")
+    got = buf.download()
+
+    def h32(x):  # lowbias32
+        x = np.uint32(x)
+        x ^= x >> np.uint32(16)
+        x = np.uint32((int(x) * 0x7FEB352D) & 0xFFFFFFFF)
+        x ^= x >> np.uint32(15)
+        x = np.uint32((int(x) * 0x846CA68B) & 0xFFFFFFFF)
+        x ^= x >> np.uint32(16)
+        return int(x)
+
+    want = np.full(words, np.iinfo(np.uint64).max, np.uint64)
+    for t in range(blocks * 256):
+        x = t ^ 0x5EED
+        for _ in range(iters):
+            x = h32((x + 0x9E3779B9) & 0xFFFFFFFF)
+            idx = (x * words) >> 32
+            want[idx] = min(int(want[idx]), (h32(x) << 32) | x)
+    assert np.array_equal(got, want)
