@@ -135,6 +135,30 @@ def test_frame_c1_small(gpu):
     assert H.psnr(rg.color, ro.color) == float("inf")
 
 
+@pytest.mark.parametrize("case", ["one_word", "grid64"])
+def test_frame_splat_word_table_contention(gpu, case):
+    """Point-parallel splat under heavy merging in its LDS word table: every
+    Gaussian on one sub-pixel spot (all lanes race for one slot, the
+    per-thread repeat skip), or on a coarse grid of 64 spots at two depths
+    (slot collisions, the neighbour probe and the framebuffer fallback)."""
+    flush(f"word table {case}")
+    arrays = H.scene(0x5EED0300 + len(case), 30_000, H.C1)
+    n = arrays.n
+    if case == "one_word":
+        arrays.means[:] = 0.0
+        arrays.scales[:] = 0.002
+    else:
+        k = np.arange(n)
+        arrays.means[:, 0] = ((k % 8) - 3.5) * 2.0
+        arrays.means[:, 1] = (((k // 8) % 8) - 3.5) * 2.0
+        arrays.means[:, 2] = np.where(k % 2 == 0, 0.0, 4.0)
+        arrays.scales[:] = 0.05
+    arrays.opacities[:] = 0.95
+    st = gpu.RenderSettings(320, 200, 1, 2, fully_disable_hierarchical_culling=True)
+    (rg, ro), = run_pair(arrays, st, [c1_frame(320, 200, ss=2)])
+    assert rg.stats["num_points"][0] > 1000 and rg.stats["splat_binned"] == 0
+
+
 def test_frame_c2_small_sh3_ss2_spp4(gpu):
     arrays = H.scene(0x5EED0002, 40_000, H.C2, clusters=200)
     st = gpu.RenderSettings(480, 270, 4, 2, fully_disable_hierarchical_culling=True)
