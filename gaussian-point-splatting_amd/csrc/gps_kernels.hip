@@ -941,6 +941,40 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
 }
 
 
+#if GPS_SPLAT_SLOTS
+// End of a chunk: every stored word is read once and lowered with one
+// atomicMin if the chunk's minimum is smaller, and the table is cleared.  A
+// thread's reads are all issued before any is used (empty slots read word 0,
+// which is always allocated), so the chunk waits for one memory round trip
+// instead of one per slot.
+template <int THREADS>
+__device__ __forceinline__ uint32_t flush_word_table(uint32_t* s_key, unsigned long long* s_val,
+                                                     unsigned long long* __restrict__ fb) {
+    constexpr int PER = GPS_SPLAT_SLOTS / THREADS;
+    static_assert(PER * THREADS == GPS_SPLAT_SLOTS, "slots split evenly over the threads");
+    const int tid = threadIdx.x;
+    uint32_t key[PER];
+    unsigned long long cur[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) key[k] = s_key[tid + k * THREADS];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) cur[k] = fb[key[k] != 0xFFFFFFFFu ? key[k] : 0u];
+    uint32_t issued = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        if (key[k] == 0xFFFFFFFFu) continue;
+        const unsigned long long v = s_val[tid + k * THREADS];
+        if (cur[k] > v) {
+            atomicMin(fb + key[k], v);
+            ++issued;
+        }
+        s_key[tid + k * THREADS] = 0xFFFFFFFFu;
+        s_val[tid + k * THREADS] = ~0ull;
+    }
+    return issued;
+}
+#endif
+
 __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRecord* __restrict__ recs,
                                                                   const uint32_t* __restrict__ cidx,
                                                                   const uint32_t* __restrict__ coff,
@@ -1016,18 +1050,7 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
 #if GPS_SPLAT_SLOTS
         // flush the chunk's merged words and clear the table; the next
         // chunk inserts only after the barrier that follows its offsets
-        for (int i = tid; i < GPS_SPLAT_SLOTS; i += GPS_SPLAT_THREADS) {
-            const uint32_t key = s_key[i];
-            if (key != 0xFFFFFFFFu) {
-                const unsigned long long v = s_val[i];
-                if (fb[key] > v) {
-                    atomicMin(fb + key, v);
-                    ++issued;
-                }
-                s_key[i] = 0xFFFFFFFFu;
-                s_val[i] = ~0ull;
-            }
-        }
+        issued += flush_word_table<GPS_SPLAT_THREADS>(s_key, s_val, fb);
 #endif
     }
     if (atomics_count) add_count(atomics_count, issued);
@@ -2228,19 +2251,7 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
         }
         __syncthreads();
 #if GPS_SPLAT_SLOTS
-        if (s_key)
-            for (int i = tid; i < GPS_SPLAT_SLOTS; i += THREADS) {
-                const uint32_t key = s_key[i];
-                if (key != 0xFFFFFFFFu) {
-                    const unsigned long long v = s_val[i];
-                    if (fb[key] > v) {
-                        atomicMin(fb + key, v);
-                        ++issued;
-                    }
-                    s_key[i] = 0xFFFFFFFFu;
-                    s_val[i] = ~0ull;
-                }
-            }
+        if (s_key) issued += flush_word_table<THREADS>(s_key, s_val, fb);
 #endif
     }
     if (atomics_count) add_count(atomics_count, issued);
