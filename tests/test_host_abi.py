@@ -54,6 +54,11 @@ def test_entry_points_reject_bad_arguments():
     s = G.RenderSettingsC()
     s.width, s.height, s.render_pass_count, s.supersampling_factor = 100, 100, 0, 1
     assert G.lib.gps_pass_create(C.byref(s), C.c_void_p(1), C.byref(h)) == 1
+    # load-time BVH order: argument checks come before any device work
+    assert G.lib.gps_bvh_order(None, None, None, 5, None) == 1
+    assert G.lib.gps_bvh_order(None, None, None, 0, None) == 0  # nothing to order
+    assert G.lib.gps_selftest_atomics(None, 16, 1, 1, None) == 1
+    assert G.lib.gps_selftest_atomics(C.c_void_p(8), 0, 1, 1, None) == 1
 
 
 def lookat_ref(eye, center, up):
