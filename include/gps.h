@@ -393,7 +393,8 @@ gps_status gps_morton_order(const float* means, uint64_t n, uint32_t* perm);
  * activated scales).  GPS_ERR_UNSUPPORTED above 2^31 - 1 Gaussians, or for
  * scenes whose node areas exceed ~1e30 / count (where tinybvh splits on a
  * degenerate axis).  A load-time call: it allocates its own device memory
- * (about 75 bytes per Gaussian), works on the null stream and returns after
+ * (about 80 bytes per Gaussian plus an upload buffer of at most 336 MB),
+ * works on the null stream and returns after
  * the device has finished. */
 gps_status gps_bvh_order(const float* means, const float* rotations, const float* scales, uint64_t n,
                          uint32_t* perm);
