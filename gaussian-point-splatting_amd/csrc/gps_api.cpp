@@ -441,6 +441,10 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
         return GPS_ERR_INVALID_ARG;
     if ((int64_t)st->width * st->supersampling_factor >= 65536 || (int64_t)st->height * st->supersampling_factor >= 65536)
         return GPS_ERR_UNSUPPORTED;
+    // framebuffer word indices are 32-bit signed (fb_index)
+    if (fb_words(st->width * st->supersampling_factor, st->height * st->supersampling_factor,
+                 st->render_pass_count) >= (1ull << 31))
+        return GPS_ERR_UNSUPPORTED;
     gps_pass* p = new gps_pass();
     p->scene = scene;
     p->settings = *st;
@@ -684,6 +688,7 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
     if (fp->supersampling_factor < 1 || !(fp->far_plane > fp->near_plane)) return GPS_ERR_INVALID_ARG;
     const int ss = fp->supersampling_factor;
     if ((int64_t)fp->width * ss >= 65536 || (int64_t)fp->height * ss >= 65536) return GPS_ERR_UNSUPPORTED;
+    if (fb_words(fp->width * ss, fp->height * ss, spp) >= (1ull << 31)) return GPS_ERR_UNSUPPORTED;
     hipStream_t st = (hipStream_t)stream;
     gps_status s = ensure_fb(p, ss, fb_words(fp->width * ss, fp->height * ss, spp));
     if (s != GPS_OK) return s;

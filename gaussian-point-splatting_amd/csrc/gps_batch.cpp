@@ -246,6 +246,10 @@ gps_status gps_batch_create(const gps_batch_settings* st, gps_scene* scene, gps_
     }
     b->fb_view_words = fb_words(st->max_width * st->max_supersampling, st->max_height * st->max_supersampling,
                                 st->max_spp);
+    if (b->fb_view_words >= (1ull << 31)) {
+        delete b;
+        return GPS_ERR_UNSUPPORTED;  // framebuffer word indices are 32-bit signed
+    }
     gps_status s = GPS_OK;
     auto chk = [&](gps_status r) {
         if (s == GPS_OK) s = r;

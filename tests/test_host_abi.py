@@ -54,6 +54,10 @@ def test_entry_points_reject_bad_arguments():
     s = G.RenderSettingsC()
     s.width, s.height, s.render_pass_count, s.supersampling_factor = 100, 100, 0, 1
     assert G.lib.gps_pass_create(C.byref(s), C.c_void_p(1), C.byref(h)) == 1
+    # a framebuffer of 2^31 words or more (32-bit word indices) is refused
+    # before anything is allocated: 40000^2 pixels at 2 samples = 3.2e9 words
+    s.width, s.height, s.render_pass_count, s.supersampling_factor = 40000, 40000, 2, 1
+    assert G.lib.gps_pass_create(C.byref(s), C.c_void_p(1), C.byref(h)) == 4
     # load-time BVH order: argument checks come before any device work
     assert G.lib.gps_bvh_order(None, None, None, 5, None) == 1
     assert G.lib.gps_bvh_order(None, None, None, 0, None) == 0  # nothing to order
