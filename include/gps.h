@@ -192,7 +192,10 @@ gps_status gps_encode_quantized(const gps_scene_desc* desc, const gps_quantized_
 /* ------------------------------------------------------------------ passes */
 /* Replaces GaussianPointSplatting::GaussianPointSplatting(const RenderSettings&)
  * (gaussian_point_splatting.cu:777-816) for an already-loaded scene.  The scene
- * must outlive the pass; several passes may share one scene. */
+ * must outlive the pass; several passes may share one scene.
+ * GPS_ERR_UNSUPPORTED when width or height times the supersampling factor
+ * reaches 65536, or the framebuffer (64x64 tiles x samples per pixel) would
+ * hold 2^31 words or more; the same limits apply per frame and per batch. */
 gps_status gps_pass_create(const gps_render_settings* settings, gps_scene* scene, gps_pass** out);
 void gps_pass_destroy(gps_pass* pass);
 /* Splat kernel choice.  GPS_SPLAT_BINNED: screen regions rendered in LDS,
