@@ -191,7 +191,10 @@ static_assert((GPS_SPLAT_SLOTS & (GPS_SPLAT_SLOTS - 1)) == 0, "slots: a power of
 // Distributor tile: 256 threads x 16 Gaussians.
 #define GPS_TILE_SHIFT 12
 #define GPS_TILE (1u << GPS_TILE_SHIFT)
+#ifndef GPS_PRE_THREADS
 #define GPS_PRE_THREADS 256
+#endif
+static_assert(GPS_PRE_THREADS % 64 == 0 && (GPS_TILE % GPS_PRE_THREADS) == 0, "whole waves, whole tiles");
 
 #if defined(__HIP_DEVICE_COMPILE__)
 // Byte `sel` (a constant 0..3) of `word` as an OCP FP8 E4M3 value, converted
