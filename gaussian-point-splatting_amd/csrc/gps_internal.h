@@ -142,11 +142,11 @@ struct BatchView {
 #define GPS_SPLAT_THREADS 256
 #endif
 #ifndef GPS_SPLAT_PPT
-#define GPS_SPLAT_PPT 4
+#define GPS_SPLAT_PPT 2
 #endif
 #define GPS_SPLAT_CHUNK (GPS_SPLAT_THREADS * GPS_SPLAT_PPT)
 #ifndef GPS_SPLAT_CHUNK_SHIFT
-#define GPS_SPLAT_CHUNK_SHIFT 10u  // log2(GPS_SPLAT_CHUNK)
+#define GPS_SPLAT_CHUNK_SHIFT 9u  // log2(GPS_SPLAT_CHUNK)
 #endif
 static_assert((1u << GPS_SPLAT_CHUNK_SHIFT) == GPS_SPLAT_CHUNK, "chunk is a power of two");
 // Point-parallel splat: samples of one chunk that land on the same
@@ -162,9 +162,12 @@ static_assert((1u << GPS_SPLAT_CHUNK_SHIFT) == GPS_SPLAT_CHUNK, "chunk is a powe
 #define GPS_SPLAT_WGS_PER_CU 10
 #endif
 // A sample whose slot holds another word tries the neighbouring slot before
-// the framebuffer (0.986 -> 0.979 ms; 16 workgroups per CU: 1.012).  Larger
-// chunks with the table measured slower: 512 threads x 4 points 1.107 ms,
-// 256 x 8 1.175, 1024 x 2 with 4096 words 1.058.
+// the framebuffer (0.986 -> 0.979 ms; 16 workgroups per CU: 1.012).  Chunk
+// size against the 2048-word table (splat ms per frame): 256 threads x 4
+// points 0.972, x 2 points 0.913 (the table overflows less), x 1 point 0.940
+// (the distributor's chunk starts cost more); 512 x 4 1.107, 256 x 8 1.175,
+// 1024 x 2 with 4096 words 1.058; 256 x 2 with 1024 words 1.043; 512 x 1
+// (eight waves per table) 0.962.
 #ifndef GPS_SPLAT_PROBE2
 #define GPS_SPLAT_PROBE2 1
 #endif
