@@ -171,6 +171,10 @@ static_assert((1u << GPS_SPLAT_CHUNK_SHIFT) == GPS_SPLAT_CHUNK, "chunk is a powe
 #ifndef GPS_SPLAT_PROBE2
 #define GPS_SPLAT_PROBE2 1
 #endif
+// ...and then slot ^ 2 (512-point chunks: 0.913 -> 0.908 ms).
+#ifndef GPS_SPLAT_PROBE3
+#define GPS_SPLAT_PROBE3 1
+#endif
 static_assert((GPS_SPLAT_SLOTS & (GPS_SPLAT_SLOTS - 1)) == 0, "slots: a power of two");
 // Binned splat: one 1024-thread workgroup renders a screen region of at most
 // GPS_BIN_WORDS framebuffer words in LDS; GPS_BIN_PPT points per thread per
