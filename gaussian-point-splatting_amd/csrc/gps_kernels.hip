@@ -660,27 +660,45 @@ __global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ 
     constexpr int PER = 8;  // tiles per thread per step
     // A step's 8192 tiles pass through LDS so that global loads and stores
     // are coalesced while each thread scans 8 consecutive tiles; one pad word
-    // per 64 keeps the strided LDS reads conflict-free.
+    // per 64 keeps the strided LDS reads conflict-free.  The next step's 16
+    // words per thread are loaded while the current step is scanned; the
+    // loads take a clamped index instead of a guard, so all 16 are in flight
+    // together (a guarded load per tile waited for each one in turn).
     constexpr uint32_t STEP = 1024u * PER;
     __shared__ uint32_t s_ta[STEP + STEP / 64], s_tc[STEP + STEP / 64];
     __shared__ uint32_t s_a[16], s_c[16];
     __shared__ uint32_t s_carry_a, s_carry_c;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto pidx = [](uint32_t j) { return j + (j >> 6); };
+    uint32_t na[PER], nc[PER];
+    // Reads tiles [b, b + STEP) into na / nc, 0 past ntiles (ntiles >= 1).
+    auto load_step = [&](uint32_t b) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t i = b + (uint32_t)k * 1024u + (uint32_t)tid;
+            const uint32_t ic = i < ntiles ? i : ntiles - 1u;
+            const uint32_t va = tile_sum[ic], vc = tile_cnt[ic];
+            na[k] = i < ntiles ? va : 0u;
+            nc[k] = i < ntiles ? vc : 0u;
+        }
+    };
     if (tid == 0) {
         s_carry_a = 0;
         s_carry_c = 0;
     }
+    if (ntiles) load_step(0u);
     __syncthreads();
     for (uint32_t base = 0; base < ntiles; base += STEP) {
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const uint32_t j = (uint32_t)k * 1024u + (uint32_t)tid;
-            const uint32_t i = base + j;
-            s_ta[pidx(j)] = i < ntiles ? tile_sum[i] : 0u;
-            s_tc[pidx(j)] = i < ntiles ? tile_cnt[i] : 0u;
+            s_ta[pidx(j)] = na[k];
+            s_tc[pidx(j)] = nc[k];
         }
         __syncthreads();
+        // Past the last step this reads tile ntiles - 1 (possibly while this
+        // step clears it) and discards the value.
+        load_step(base + STEP);
         const uint32_t j0 = (uint32_t)tid * PER;
         uint32_t a[PER], c[PER], sa = 0, sc = 0;
 #pragma unroll
