@@ -785,8 +785,12 @@ __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restr
     __shared__ uint32_t s_a[4], s_c[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t tile = blockIdx.x;
+    // The workgroup's offsets and the pass totals are read up front, together.
+    const uint32_t v0 = tile_voff[tile];
     const uint32_t vend = tile + 1 < ntiles ? tile_voff[tile + 1] : ctr->valid;
-    if (vend == tile_voff[tile]) return;  // uniform over the workgroup
+    const uint32_t a0 = tile_off[tile];
+    const uint64_t P = ctr->points;
+    if (vend == v0) return;  // uniform over the workgroup
     const uint32_t view = tile / tiles_per_view;
     const uint32_t* cmask = cv.mask + (uint64_t)view * mask_stride;
     const uint32_t g0 = (tile << GPS_TILE_SHIFT) + (uint32_t)tid * 16u;
@@ -794,24 +798,45 @@ __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restr
     // bit k set: element g0 + k counts (inside the scene and not culled)
     uint32_t live = l0 >= N ? 0u : (N - l0 >= 16u ? 0xFFFFu : ((1u << (N - l0)) - 1u));
     if (hier && live) {
-        uint32_t b = l0 / cv.gpb;
-        uint32_t next = (b + 1u) * cv.gpb;
-        bool c = mask_bit(cmask, b);
         const int kmax = N - l0 >= 16u ? 16 : (int)(N - l0);
+        // The mask words of the thread's first and last block are read
+        // together; blocks in between (fewer than 16 Gaussians per block)
+        // are read when reached.
+        const uint32_t bf = l0 / cv.gpb, bl = (l0 + (uint32_t)kmax - 1u) / cv.gpb;
+        const uint32_t mf = cmask[bf >> 5], ml = cmask[bl >> 5];
+        auto culled = [&](uint32_t b) {
+            const uint32_t m = b == bf ? mf : (b == bl ? ml : cmask[b >> 5]);
+            return ((m >> (b & 31u)) & 1u) != 0u;
+        };
+        uint32_t b = bf;
+        uint32_t next = (b + 1u) * cv.gpb;
+        bool c = culled(b);
         for (int k = 0; k < kmax; ++k) {
             if (l0 + (uint32_t)k == next) {
                 ++b;
                 next += cv.gpb;
-                c = mask_bit(cmask, b);
+                c = culled(b);
             }
             if (c) live &= ~(1u << k);
         }
     }
     const uint4* wp = reinterpret_cast<const uint4*>(weights + g0);
+    // Groups of 4 without a live element re-read the first live group's
+    // address (no extra cache line) so that the 4 loads need no branch and
+    // are in flight together; their values are discarded.
+    uint4 v4[4];
+    if (live) {
+        const int kl = __builtin_ctz(live) >> 2;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v4[k] = wp[((live >> (4 * k)) & 15u) ? k : kl];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v4[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
     uint32_t w[16];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const uint4 v = ((live >> (4 * k)) & 15u) ? wp[k] : make_uint4(0u, 0u, 0u, 0u);
+        const uint4 v = v4[k];
         w[4 * k] = (live >> (4 * k)) & 1u ? v.x : 0u;
         w[4 * k + 1] = (live >> (4 * k + 1)) & 1u ? v.y : 0u;
         w[4 * k + 2] = (live >> (4 * k + 2)) & 1u ? v.z : 0u;
@@ -835,9 +860,8 @@ __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restr
         wa += s_a[q];
         wc += s_c[q];
     }
-    uint32_t off = tile_off[tile] + wa + ia - sa;
-    uint32_t r = tile_voff[tile] + wc + ic - sc;
-    const uint64_t P = ctr->points;
+    uint32_t off = a0 + wa + ia - sa;
+    uint32_t r = v0 + wc + ic - sc;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         if (w[k]) {
