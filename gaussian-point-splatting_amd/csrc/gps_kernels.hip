@@ -651,107 +651,114 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_flat_kernel(
 }
 
 // ------------------------------------------------------------------ distribution
-// Exclusive scan of the per-tile (point, valid) sums; one workgroup.  Also
-// clears the tile sums for the next pass and publishes the pass totals.
-__global__ __launch_bounds__(1024) void tile_scan_kernel(uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt,
-                                                         uint32_t ntiles, uint32_t* __restrict__ tile_off,
+// Exclusive scan of the per-tile (point, valid) sums, one workgroup per step
+// of 8192 tiles; the last workgroup publishes the pass totals.  Workgroup s
+// takes its carry by reducing the raw sums of steps 0 .. s-1 itself, so no
+// workgroup waits for another (sums are modulo 2^32, as a sequential scan's).
+// The tile sums are cleared for the next pass by distribute_kernel, which
+// runs after this kernel on every tile.
+__global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* __restrict__ tile_sum,
+                                                         const uint32_t* __restrict__ tile_cnt, uint32_t ntiles,
+                                                         uint32_t* __restrict__ tile_off,
                                                          uint32_t* __restrict__ tile_voff, PassCounters* __restrict__ ctr,
                                                          uint32_t budget, uint32_t* __restrict__ iteration) {
-    constexpr int PER = 8;  // tiles per thread per step
-    // A step's 8192 tiles pass through LDS so that global loads and stores
+    constexpr int PER = 8;  // tiles per thread
+    // The step's 8192 tiles pass through LDS so that global loads and stores
     // are coalesced while each thread scans 8 consecutive tiles; one pad word
-    // per 64 keeps the strided LDS reads conflict-free.  The next step's 16
-    // words per thread are loaded while the current step is scanned; the
-    // loads take a clamped index instead of a guard, so all 16 are in flight
-    // together (a guarded load per tile waited for each one in turn).
+    // per 64 keeps the strided LDS reads conflict-free.  Loads take a clamped
+    // index instead of a guard so that all of a thread's loads are in flight
+    // together.
     constexpr uint32_t STEP = 1024u * PER;
     __shared__ uint32_t s_ta[STEP + STEP / 64], s_tc[STEP + STEP / 64];
-    __shared__ uint32_t s_a[16], s_c[16];
-    __shared__ uint32_t s_carry_a, s_carry_c;
+    __shared__ uint32_t s_a[16], s_c[16], s_ra[16], s_rc[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto pidx = [](uint32_t j) { return j + (j >> 6); };
-    uint32_t na[PER], nc[PER];
-    // Reads tiles [b, b + STEP) into na / nc, 0 past ntiles (ntiles >= 1).
-    auto load_step = [&](uint32_t b) {
+    const uint32_t base = blockIdx.x * STEP;  // base < ntiles unless ntiles == 0 (one workgroup)
+    uint32_t va[PER], vc[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t i = base + (uint32_t)k * 1024u + (uint32_t)tid;
+        const uint32_t ic = i < ntiles ? i : (ntiles ? ntiles - 1u : 0u);
+        const uint32_t x = ntiles ? tile_sum[ic] : 0u, y = ntiles ? tile_cnt[ic] : 0u;
+        va[k] = i < ntiles ? x : 0u;
+        vc[k] = i < ntiles ? y : 0u;
+    }
+    // Carry: tiles [0, base) are whole steps.
+    uint32_t ra = 0, rc = 0;
+    for (uint32_t b = 0; b < base; b += STEP) {
+        uint32_t xa[PER], xc[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
-            const uint32_t i = b + (uint32_t)k * 1024u + (uint32_t)tid;
-            const uint32_t ic = i < ntiles ? i : ntiles - 1u;
-            const uint32_t va = tile_sum[ic], vc = tile_cnt[ic];
-            na[k] = i < ntiles ? va : 0u;
-            nc[k] = i < ntiles ? vc : 0u;
+            xa[k] = tile_sum[b + (uint32_t)k * 1024u + (uint32_t)tid];
+            xc[k] = tile_cnt[b + (uint32_t)k * 1024u + (uint32_t)tid];
         }
-    };
-    if (tid == 0) {
-        s_carry_a = 0;
-        s_carry_c = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            ra += xa[k];
+            rc += xc[k];
+        }
     }
-    if (ntiles) load_step(0u);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t j = (uint32_t)k * 1024u + (uint32_t)tid;
+        s_ta[pidx(j)] = va[k];
+        s_tc[pidx(j)] = vc[k];
+    }
+    ra = wave_reduce_add(ra);
+    rc = wave_reduce_add(rc);
+    if (lane == 0) {
+        s_ra[wave] = ra;
+        s_rc[wave] = rc;
+    }
     __syncthreads();
-    for (uint32_t base = 0; base < ntiles; base += STEP) {
+    const uint32_t j0 = (uint32_t)tid * PER;
+    uint32_t a[PER], c[PER], sa = 0, sc = 0;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const uint32_t j = (uint32_t)k * 1024u + (uint32_t)tid;
-            s_ta[pidx(j)] = na[k];
-            s_tc[pidx(j)] = nc[k];
-        }
-        __syncthreads();
-        // Past the last step this reads tile ntiles - 1 (possibly while this
-        // step clears it) and discards the value.
-        load_step(base + STEP);
-        const uint32_t j0 = (uint32_t)tid * PER;
-        uint32_t a[PER], c[PER], sa = 0, sc = 0;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            a[k] = s_ta[pidx(j0 + (uint32_t)k)];
-            c[k] = s_tc[pidx(j0 + (uint32_t)k)];
-            sa += a[k];
-            sc += c[k];
-        }
-        const uint32_t ia = wave_scan_incl(sa);
-        const uint32_t ic = wave_scan_incl(sc);
-        if (lane == 63) {
-            s_a[wave] = ia;
-            s_c[wave] = ic;
-        }
-        __syncthreads();
-        uint32_t wa = 0, wc = 0;
-        for (int w = 0; w < wave; ++w) {
-            wa += s_a[w];
-            wc += s_c[w];
-        }
-        const uint32_t ca = s_carry_a, cc = s_carry_c;
-        uint32_t oa = ca + wa + ia - sa, oc = cc + wc + ic - sc;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {  // exclusive prefixes back into LDS
-            s_ta[pidx(j0 + (uint32_t)k)] = oa;
-            s_tc[pidx(j0 + (uint32_t)k)] = oc;
-            oa += a[k];
-            oc += c[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const uint32_t j = (uint32_t)k * 1024u + (uint32_t)tid;
-            const uint32_t i = base + j;
-            if (i < ntiles) {
-                tile_off[i] = s_ta[pidx(j)];
-                tile_voff[i] = s_tc[pidx(j)];
-                tile_sum[i] = 0u;
-                tile_cnt[i] = 0u;
-            }
-        }
-        if (tid == 1023) {
-            s_carry_a = ca + wa + ia;
-            s_carry_c = cc + wc + ic;
-        }
-        __syncthreads();
+    for (int k = 0; k < PER; ++k) {
+        a[k] = s_ta[pidx(j0 + (uint32_t)k)];
+        c[k] = s_tc[pidx(j0 + (uint32_t)k)];
+        sa += a[k];
+        sc += c[k];
     }
-    if (tid == 0) {
-        const uint32_t total = s_carry_a;
+    const uint32_t ia = wave_scan_incl(sa);
+    const uint32_t ic = wave_scan_incl(sc);
+    if (lane == 63) {
+        s_a[wave] = ia;
+        s_c[wave] = ic;
+    }
+    __syncthreads();
+    uint32_t ca = 0, cc = 0, wa = 0, wc = 0;
+    for (int w = 0; w < 16; ++w) {
+        ca += s_ra[w];
+        cc += s_rc[w];
+    }
+    for (int w = 0; w < wave; ++w) {
+        wa += s_a[w];
+        wc += s_c[w];
+    }
+    uint32_t oa = ca + wa + ia - sa, oc = cc + wc + ic - sc;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {  // exclusive prefixes back into LDS
+        s_ta[pidx(j0 + (uint32_t)k)] = oa;
+        s_tc[pidx(j0 + (uint32_t)k)] = oc;
+        oa += a[k];
+        oc += c[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t j = (uint32_t)k * 1024u + (uint32_t)tid;
+        const uint32_t i = base + j;
+        if (i < ntiles) {
+            tile_off[i] = s_ta[pidx(j)];
+            tile_voff[i] = s_tc[pidx(j)];
+        }
+    }
+    if (blockIdx.x == gridDim.x - 1u && tid == 1023) {
+        const uint32_t total = ca + wa + ia;
         const uint32_t P = total < budget ? total : budget;
         ctr->total = total;
-        ctr->valid = s_carry_c;
+        ctr->valid = cc + wc + ic;
         ctr->points = P;
         ctr->chunks = (uint32_t)(((uint64_t)P + GPS_SPLAT_CHUNK - 1) / GPS_SPLAT_CHUNK);
         // The reference flips the mask ping-pong only when the pass produced
@@ -781,10 +788,15 @@ __global__ __launch_bounds__(256) void distribute_kernel(const uint32_t* __restr
                                                          uint32_t* __restrict__ coff, uint32_t* __restrict__ chunk_start,
                                                          CullView cv, int hier, uint32_t N, uint32_t tiles_per_view,
                                                          uint32_t npad, uint32_t mask_stride,
-                                                         uint8_t* __restrict__ cview, uint32_t chunk_shift) {
+                                                         uint8_t* __restrict__ cview, uint32_t chunk_shift,
+                                                         uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt) {
     __shared__ uint32_t s_a[4], s_c[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t tile = blockIdx.x;
+    if (tid == 0) {  // the scan has consumed them: clear for the next pass
+        tile_sum[tile] = 0u;
+        tile_cnt[tile] = 0u;
+    }
     // The workgroup's offsets and the pass totals are read up front, together.
     const uint32_t v0 = tile_voff[tile];
     const uint32_t vend = tile + 1 < ntiles ? tile_voff[tile + 1] : ctr->valid;
@@ -2362,13 +2374,17 @@ void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, con
     hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, st, cv, pl, vp, h, occl, culled_count);
 }
 
+// One tile-scan workgroup per 8192 tiles, at least one (it publishes the
+// pass totals also when there are no tiles).
+static uint32_t tile_scan_grid(uint32_t ntiles) { return ntiles ? (ntiles + 8191u) / 8192u : 1u; }
+
 void launch_distribute(const DistributeArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, st, a.tile_sum, a.tile_cnt, a.ntiles, a.tile_off,
-                       a.tile_voff, a.ctr, a.budget, a.iteration);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(tile_scan_grid(a.ntiles)), dim3(1024), 0, st, a.tile_sum, a.tile_cnt,
+                       a.ntiles, a.tile_off, a.tile_voff, a.ctr, a.budget, a.iteration);
     if (a.ntiles == 0) return;
     hipLaunchKernelGGL(distribute_kernel, dim3(a.ntiles), dim3(256), 0, st, a.weights, a.tile_off, a.tile_voff,
                        a.ntiles, a.ctr, a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.N, a.ntiles, 0u, 0u,
-                       (uint8_t*)nullptr, GPS_SPLAT_CHUNK_SHIFT);
+                       (uint8_t*)nullptr, GPS_SPLAT_CHUNK_SHIFT, a.tile_sum, a.tile_cnt);
 }
 
 void launch_splat(const SplatArgs& a, hipStream_t st) {
@@ -2498,8 +2514,8 @@ void launch_batch_preprocess(const BatchArgs& a, hipStream_t st) {
 
 void launch_batch_distribute(const BatchArgs& a, hipStream_t st) {
     const uint32_t ntiles = a.tiles_per_view * (uint32_t)a.nviews;
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, st, a.tile_sum, a.tile_cnt, ntiles, a.tile_off,
-                       a.tile_voff, a.ctr, 0xFFFFFFFFu, a.iteration);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(tile_scan_grid(ntiles)), dim3(1024), 0, st, a.tile_sum, a.tile_cnt,
+                       ntiles, a.tile_off, a.tile_voff, a.ctr, 0xFFFFFFFFu, a.iteration);
     hipLaunchKernelGGL(batch_views_kernel, dim3((a.nviews + 255) / 256), dim3(256), 0, st, a.views, a.nviews,
                        (const uint32_t*)a.tile_off, (const uint32_t*)a.tile_voff, a.tiles_per_view,
                        (const PassCounters*)a.ctr);
@@ -2509,7 +2525,7 @@ void launch_batch_distribute(const BatchArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(distribute_kernel, dim3(ntiles), dim3(256), 0, st, (const uint32_t*)a.weights,
                        (const uint32_t*)a.tile_off, (const uint32_t*)a.tile_voff, ntiles, (const PassCounters*)a.ctr,
                        a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.scene.N, a.tiles_per_view, a.npad,
-                       a.mask_stride, a.cview, shift);
+                       a.mask_stride, a.cview, shift, a.tile_sum, a.tile_cnt);
 }
 
 template <int THREADS, int PPT> static void launch_batch_splat_t(const BatchArgs& a, hipStream_t st) {

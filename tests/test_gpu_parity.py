@@ -490,6 +490,42 @@ def test_batch_views_identical_to_single_renders(gpu):
         check_batch(fr, cols, deps, r, f"graph {k}")
 
 
+def test_batch_tile_scan_across_workgroups(gpu):
+    """64 views of a 1.1M-Gaussian scene concatenate to about 17 200 tiles, so
+    the tile scan runs three workgroups whose carries come from the steps
+    before them; every view must match a pass rendering it alone (one view is
+    269 tiles: one workgroup, no carry)."""
+    from gps_amd import workloads as W
+    w = W.scaled(W.WORKLOADS["c2"], 1_100_000, fmt=gpu.FORMAT_HALF_FP8)
+    arrays = W.host_scene(w)
+    scene = gpu.Scene(arrays, w.fmt)
+    nv = 64
+    bw = W.BatchWorkload(scene=w, views=nv, radius=150.0, resolutions=((64, 48), (96, 64)), spps=(1, 2), ss=2)
+    stream = gpu.Stream()
+    batch = gpu.Batch(scene, nv, 96, 64, max_spp=2, max_supersampling=2)
+    single = gpu.Pass(gpu.RenderSettings(96, 64, 2, 2, flexible_frame_size=True), scene)
+    frames = W.batch_views(bw, 0)
+    cols = [gpu.DeviceArray((f.height, f.width, 3), np.float32, zero=True) for f in frames]
+    deps = [gpu.DeviceArray((f.height, f.width), np.float32, zero=True) for f in frames]
+    flush(f"batch of {nv} views, {w.n} Gaussians")
+    batch.render(frames, cols, deps, stream)
+    stream.synchronize()
+    total = 0
+    for v, f in enumerate(frames):
+        c = gpu.DeviceArray((f.height, f.width, 3), np.float32)
+        d = gpu.DeviceArray((f.height, f.width), np.float32)
+        single.render(f, c, d, stream)
+        stream.synchronize()
+        ref = single.stats(stream)
+        assert_same(f"view {v} colour", cols[v].download(), c.download())
+        assert_same(f"view {v} depth", deps[v].download(), d.download())
+        st = batch.view_stats(v, stream)
+        for key in ("valid_gaussians", "num_points", "total_weight"):
+            assert st[key][0] == ref[key][0], (v, key)
+        total += ref["num_points"][0]
+    assert total > 0
+
+
 @pytest.mark.parametrize("case", ["mixed", "closeup_big", "budget", "quantized"])
 def test_batch_binned_views_identical(gpu, case):
     """Binned batch views (the regions of all binned views in one job queue,
