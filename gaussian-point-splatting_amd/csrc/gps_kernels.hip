@@ -390,9 +390,8 @@ __global__ __launch_bounds__(256) void cull_kernel(CullView cv, Planes pl, Mat4 
         const uint32_t w0 = (idx - lane) >> 5;
         if (w0 < cv.mask_words) cv.mask[w0] = (uint32_t)b;
         if (w0 + 1 < cv.mask_words) cv.mask[w0 + 1] = (uint32_t)(b >> 32);
-        const uint32_t c = (uint32_t)__popcll(b);
-        if (c) atomicAdd(culled_count, c);
     }
+    add_count(culled_count, culled ? 1u : 0u);  // one atomic per workgroup on the single counter
 }
 
 // ------------------------------------------------------------------ preprocess
