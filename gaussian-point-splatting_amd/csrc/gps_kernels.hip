@@ -2136,9 +2136,8 @@ __global__ __launch_bounds__(256) void batch_cull_kernel(BatchView* __restrict__
         const uint32_t w0 = (idx - lane) >> 5;
         if (w0 < cv.mask_words) mask[w0] = (uint32_t)b;
         if (w0 + 1 < cv.mask_words) mask[w0 + 1] = (uint32_t)(b >> 32);
-        const uint32_t c = (uint32_t)__popcll(b);
-        if (c) atomicAdd(&views[v].culled_blocks, c);
     }
+    add_count(&views[v].culled_blocks, culled ? 1u : 0u);  // the workgroup's view: one atomic per workgroup
 }
 
 // One workgroup per tile of 4096 Gaussians; each thread evaluates its 16
