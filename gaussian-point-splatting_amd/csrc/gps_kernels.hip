@@ -1938,9 +1938,44 @@ __global__ __launch_bounds__(256) void hiz_build_kernel(HiZView h, const float* 
 
 // Levels above the per-workgroup ones, in one workgroup (second launch; a
 // device-wide ticket on one word serialized the 2000 workgroups).
+// The level below `top` is staged in LDS once when it fits (up to 8192
+// texels, frames to about 4K), so each higher level costs a barrier instead of
+// a round trip through memory; larger frames read every level from memory.
 __global__ __launch_bounds__(256) void hiz_top_kernel(HiZView h, int kmin, int top) {
+    constexpr int CAP = 8192;
+    __shared__ float s_a[CAP], s_b[CAP];
     const int L = h.levels;
     const int tid = threadIdx.x;
+    const int n1 = h.lw[top + 1] * h.lh[top + 1];
+    if (n1 <= CAP) {
+        float* cur = s_a;
+        float* nxt = s_b;
+        for (int i = tid; i < n1; i += 256) cur[i] = h.mip[h.off[top + 1] + (uint32_t)i];
+        __syncthreads();
+        for (int l = top; l >= 0; --l) {
+            const bool use_min = ((L - 2) - l) < kmin;
+            const int n = h.lw[l] * h.lh[l];
+            for (int i = tid; i < n; i += 256) {
+                const int X = i % h.lw[l], Y = i / h.lw[l];
+                float v = use_min ? __builtin_bit_cast(float, 0x7F800000u) : __builtin_bit_cast(float, 0xFF800000u);
+                for (int dy = 0; dy < 2; ++dy)
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const int CX = 2 * X + dx, CY = 2 * Y + dy;
+                        if (CX < h.lw[l + 1] && CY < h.lh[l + 1]) {
+                            const float c = cur[CY * h.lw[l + 1] + CX];
+                            v = use_min ? det_fmin(v, c) : det_fmax(v, c);
+                        }
+                    }
+                h.mip[h.off[l] + (uint32_t)i] = v;
+                nxt[i] = v;
+            }
+            __syncthreads();
+            float* t = cur;
+            cur = nxt;
+            nxt = t;
+        }
+        return;
+    }
     for (int l = top; l >= 0; --l) {
         const bool use_min = ((L - 2) - l) < kmin;
         const int n = h.lw[l] * h.lh[l];
