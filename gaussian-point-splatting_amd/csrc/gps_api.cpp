@@ -712,12 +712,10 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         p->timing_slot = -1;
     }
     stage_mark(p, 0, 0, 0, st);
-    launch_clear(p->cur_fb, p->fb_words, bg, st);
+    // The clear also resets the per-frame counters (pass results, culled
+    // counts, atomics) and keeps the Hi-Z ticket and the mask iteration.
+    launch_clear(p->cur_fb, p->fb_words, bg, st, reinterpret_cast<uint32_t*>(p->ctr), 10, &p->ctr->atomics, 1);
     stage_mark(p, 0, 0, 1, st);
-    // reset per-frame counters (pass results, culled counts, atomics), keep
-    // the Hi-Z ticket and the mask iteration
-    launch_zero_u32(reinterpret_cast<uint32_t*>(p->ctr), 10, st);
-    launch_zero_u32(&p->ctr->atomics, 1, st);
 
     Mat4 vpm;
     std::memcpy(vpm.m, VP.m, 64);

@@ -138,7 +138,9 @@ bool launch_batch_splat(const BatchArgs& a, hipStream_t st);
 void launch_batch_binned(const BatchArgs& a, hipStream_t st);
 void launch_batch_bin_big(const BatchArgs& a, hipStream_t st);
 
-void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st);
+// Fills the framebuffer with `value`; also zeroes n0 words at z0 and n1 at z1.
+void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st, uint32_t* z0 = nullptr,
+                  uint32_t n0 = 0, uint32_t* z1 = nullptr, uint32_t n1 = 0);
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
                  uint32_t* culled_count, hipStream_t st);
 void launch_preprocess(const PreprocessArgs& a, hipStream_t st);

@@ -322,7 +322,15 @@ template <int DEG> struct ShEval<GPS_FORMAT_QUANTIZED, DEG> {
 };
 
 // ------------------------------------------------------------------ clear
-__global__ __launch_bounds__(256) void clear_kernel(uint4* __restrict__ fb, uint64_t n2, uint64_t value) {
+// Also zeroes up to 256 words at z0 and at z1 (the pass counters) in
+// workgroup 0, which saves two one-workgroup launches per frame.
+__global__ __launch_bounds__(256) void clear_kernel(uint4* __restrict__ fb, uint64_t n2, uint64_t value,
+                                                    uint32_t* __restrict__ z0, uint32_t n0, uint32_t* __restrict__ z1,
+                                                    uint32_t n1) {
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < n0) z0[threadIdx.x] = 0u;
+        if (threadIdx.x < n1) z1[threadIdx.x] = 0u;
+    }
     const uint4 v = make_uint4((uint32_t)value, (uint32_t)(value >> 32), (uint32_t)value, (uint32_t)(value >> 32));
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (uint64_t)gridDim.x * blockDim.x)
         fb[i] = v;
@@ -2390,12 +2398,20 @@ void launch_preprocess(const PreprocessArgs& a, hipStream_t st) {
     }
 }
 
-void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st) {
+void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st, uint32_t* z0, uint32_t n0,
+                  uint32_t* z1, uint32_t n1) {
     const uint64_t n2 = words / 2;
     uint64_t blocks = (n2 + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    if (blocks == 0) return;
-    hipLaunchKernelGGL(clear_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, reinterpret_cast<uint4*>(fb), n2, value);
+    if (blocks == 0 && n0 + n1 == 0) return;
+    if (n0 > 256 || n1 > 256) {  // more than one workgroup zeroes: separate launches
+        if (n0) launch_zero_u32(z0, n0, st);
+        if (n1) launch_zero_u32(z1, n1, st);
+        n0 = n1 = 0;
+        if (blocks == 0) return;
+    }
+    hipLaunchKernelGGL(clear_kernel, dim3(blocks ? (uint32_t)blocks : 1u), dim3(256), 0, st,
+                       reinterpret_cast<uint4*>(fb), n2, value, z0, n0, z1, n1);
 }
 
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
