@@ -780,7 +780,9 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         sa.xcds = p->xcds;
         stage_mark(p, 4, slot, 0, st);
         BinGeom bg_geom;
-        const bool binned = p->splat_mode != GPS_SPLAT_POINT_PARALLEL &&
+        // the biased splat adds colours per sample and has only the
+        // point-parallel form
+        const bool binned = f.unbiased && p->splat_mode != GPS_SPLAT_POINT_PARALLEL &&
                             (p->splat_mode == GPS_SPLAT_BINNED || f.Ki * f.spp >= GPS_BIN_AUTO_SAMPLES) &&
                             binned_geometry(f.up_w, f.up_h, f.spp, &bg_geom);
         p->last_binned = binned ? 1 : 0;

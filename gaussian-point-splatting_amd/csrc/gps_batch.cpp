@@ -339,9 +339,10 @@ gps_status gps_batch_render(gps_batch* b, int32_t nviews, const gps_frame_params
         bv.fb_words = fb_words(f.width * f.supersampling_factor, f.height * f.supersampling_factor, spp);
         bv.background = background_word(f.background[0], f.background[1], f.background[2]);
         bv.budget = (uint32_t)std::min<int64_t>(b->settings.max_number_of_points, 0x7FFFFFFFll);
-        // Binned splat for the view under the same rule as a single pass.
+        // Binned splat for the view under the same rule as a single pass
+        // (unbiased frames only: the biased splat adds colours per sample).
         BinGeom g;
-        if (b->splat_mode != GPS_SPLAT_POINT_PARALLEL &&
+        if (bv.f.unbiased && b->splat_mode != GPS_SPLAT_POINT_PARALLEL &&
             (b->splat_mode == GPS_SPLAT_BINNED || bv.f.Ki * bv.f.spp >= GPS_BIN_AUTO_SAMPLES) &&
             binned_geometry(bv.f.up_w, bv.f.up_h, bv.f.spp, &g) && ensure_binned(b) &&
             regions + g.nregions <= b->bin_regions_cap) {

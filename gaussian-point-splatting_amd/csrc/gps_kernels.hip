@@ -928,6 +928,29 @@ __device__ __forceinline__ bool alpha_passes(const SplatRecord& rec, float sigma
     return !(rec.opacity * gw < 1.0f / 255.0f);
 }
 
+// One sample of the biased splat (use_unbiased_2d_splatting = 0) on its
+// framebuffer word, as gaussian_point_splatting.cu:130-138 does it: the
+// pre-check (a stored word that is nearer, or at the same depth with colour
+// 0, skips the sample), then atomicAddColor (post_processing_kernels.cuh:13-42)
+// as a 64-bit CAS loop: a nearer sample replaces the word, a farther one
+// leaves it, one at equal depth adds its 36 colour bits modulo 2^36.  The
+// pre-check's read is the loop's first guess.  Returns 1 if the loop ran.
+__device__ __forceinline__ uint32_t add_color_sample(unsigned long long* __restrict__ w, unsigned long long value) {
+    const unsigned long long depth = value & GPS_DEPTH_MASK;
+    unsigned long long old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old <= depth) return 0u;
+    unsigned long long assumed;
+    do {
+        assumed = old;
+        const unsigned long long assumed_depth = assumed & GPS_DEPTH_MASK;
+        if (depth > assumed_depth) break;
+        const unsigned long long nv =
+            depth == assumed_depth ? (assumed_depth | ((assumed + value) & GPS_COLOR_MASK)) : value;
+        old = atomicCAS(w, assumed, nv);
+    } while (old != assumed);
+    return 1u;
+}
+
 // The K * spp samples of point p (seed stream p of RNG frame `frame_seed`)
 // of one Gaussian (splatGaussianPoint, gaussian_point_splatting.cu:67-141):
 // sample q = layer * K + k lands in layer q / K.  Returns the atomics issued.
@@ -960,6 +983,11 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
         const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
         if (!alpha_passes(rec, sigma)) continue;
         const int idx = fb_index(x, y, layer, spp, up_w);
+        if (!unbiased) {
+            // every sample adds: no repeat skip, no word table
+            issued += add_color_sample(fb + idx, packed);
+            continue;
+        }
         // The word this thread just read or wrote for the same Gaussian
         // (same packed value): a second atomicMin would change nothing.
         if (idx == last) continue;

@@ -7,7 +7,8 @@
 // interactive viewer that have no headless meaning are accepted and ignored.
 // Headless additions: --synthetic, --num-gaussians, --seed, --frames, --out,
 // --format, --orbit, --eye/--center, --taa-current-depth,
-// --frames-per-camera-keyframe, --total-test-keyframes.
+// --frames-per-camera-keyframe, --total-test-keyframes, --screenshot-dir
+// (base folder of the evaluation screenshots, default out/screenshots).
 //
 // Camera paths (--camera-path): the first camera sets the resolution (the
 // width is kept, the height follows its aspect ratio) and every camera its
@@ -27,6 +28,9 @@
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <errno.h>
+#include <sys/stat.h>
 
 #include "../../include/gps.h"
 
@@ -68,6 +72,10 @@ struct Options {
     bool override_camera_settings = false;
     int frames_per_camera_keyframe = 10;
     int total_test_keyframes = 0;
+    // evaluation screenshots (main.cu:72-120, render_server.cu:263-315)
+    bool create_evaluation_screenshots = false;
+    int max_eval_count = 40;
+    std::string screenshot_dir = "out/screenshots";
 };
 
 bool parse_vec(int& i, int argc, char** argv, float* out, int n) {
@@ -115,10 +123,12 @@ Options parse(int argc, char** argv) {
         else if (a == "--total-test-keyframes") o.total_test_keyframes = std::stoi(next());
         else if (a == "--sorted-model-output-path" || a == "--sorted_model_output_path")
             o.sorted_model_output_path = next();
-        else if (a == "--max-eval-count" || a == "--max_eval_count") next();
+        else if (a == "--max-eval-count" || a == "--max_eval_count") o.max_eval_count = std::stoi(next());
+        else if (a == "--create-evaluation-screenshots" || a == "--create_evaluation_screenshots")
+            o.create_evaluation_screenshots = true;
+        else if (a == "--screenshot-dir") o.screenshot_dir = next();
         else if (a == "--freeze-culling" || a == "--enable_profiling" || a == "--disable_gui" || a == "--disable-gui" ||
-                 a == "--snap_window_to_corner" || a == "--snap-window-to-corner" ||
-                 a == "--create-evaluation-screenshots" || a == "--create_evaluation_screenshots") {
+                 a == "--snap_window_to_corner" || a == "--snap-window-to-corner") {
         }
         // headless additions
         else if (a == "--synthetic") o.synthetic = next();
@@ -198,6 +208,107 @@ void write_pfm(const std::string& path, const std::vector<float>& c, int w, int 
         std::fwrite(row.data(), 4, row.size(), f);
     }
     std::fclose(f);
+}
+
+// PNG (RGB8, no interlace) with the image data in stored (uncompressed)
+// deflate blocks: a valid zlib stream without a compressor.  Rows are
+// written last row first, as save_image_to_disk flips the resolve's image
+// (render_server.cu:268-277) before stbi_write_png.
+uint32_t crc32_update(uint32_t c, const uint8_t* p, size_t n) {
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; ++k) v = (v & 1u) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
+            table[i] = v;
+        }
+        init = true;
+    }
+    c = ~c;
+    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFFu] ^ (c >> 8);
+    return ~c;
+}
+
+bool write_png(const std::string& path, const std::vector<uint8_t>& rgb, int w, int h) {
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)h * ((size_t)w * 3 + 1));
+    for (int y = h - 1; y >= 0; --y) {
+        raw.push_back(0);  // filter: none
+        raw.insert(raw.end(), rgb.begin() + (size_t)y * w * 3, rgb.begin() + (size_t)(y + 1) * w * 3);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};
+    for (size_t off = 0; off < raw.size() || off == 0;) {
+        const size_t len = std::min<size_t>(65535, raw.size() - off);
+        const bool last = off + len == raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back((uint8_t)(len & 0xFF));
+        z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)(~len & 0xFF));
+        z.push_back((uint8_t)((~len >> 8) & 0xFF));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        off += len;
+        if (last) break;
+    }
+    uint32_t a = 1, b = 0;  // Adler-32 of the uncompressed data
+    for (uint8_t v : raw) {
+        a = (a + v) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    const uint32_t adler = (b << 16) | a;
+    for (int k = 3; k >= 0; --k) z.push_back((uint8_t)(adler >> (8 * k)));
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    auto be32 = [&](uint32_t v) {
+        const uint8_t q[4] = {(uint8_t)(v >> 24), (uint8_t)(v >> 16), (uint8_t)(v >> 8), (uint8_t)v};
+        std::fwrite(q, 1, 4, f);
+    };
+    auto chunk = [&](const char* type, const std::vector<uint8_t>& data) {
+        be32((uint32_t)data.size());
+        std::fwrite(type, 1, 4, f);
+        if (!data.empty()) std::fwrite(data.data(), 1, data.size(), f);
+        uint32_t c = crc32_update(0, reinterpret_cast<const uint8_t*>(type), 4);
+        c = crc32_update(c, data.data(), data.size());
+        be32(c);
+    };
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+    std::fwrite(sig, 1, 8, f);
+    std::vector<uint8_t> ihdr = {(uint8_t)(w >> 24), (uint8_t)(w >> 16), (uint8_t)(w >> 8), (uint8_t)w,
+                                 (uint8_t)(h >> 24), (uint8_t)(h >> 16), (uint8_t)(h >> 8), (uint8_t)h,
+                                 8, 2, 0, 0, 0};
+    chunk("IHDR", ihdr);
+    chunk("IDAT", z);
+    chunk("IEND", {});
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    return ok;
+}
+
+// RenderSettings::model_name (render_settings.cuh:155-198): a model
+// directory's name or the PLY file's stem; synthetic scenes are named after
+// their configuration.
+std::string model_name(const Options& o) {
+    if (o.model_path.empty()) return "synthetic_" + o.synthetic;
+    std::string p = o.model_path;
+    while (!p.empty() && (p.back() == '/' || p.back() == '\\')) p.pop_back();
+    const size_t slash = p.find_last_of("/\\");
+    std::string base = slash == std::string::npos ? p : p.substr(slash + 1);
+    const size_t dot = base.find_last_of('.');
+    if (dot != std::string::npos && dot > 0) base = base.substr(0, dot);
+    return base;
+}
+
+bool make_dirs(const std::string& path) {
+    std::string cur;
+    for (size_t i = 0; i <= path.size(); ++i) {
+        if (i == path.size() || path[i] == '/') {
+            if (!cur.empty() && cur != ".") {
+                if (mkdir(cur.c_str(), 0755) != 0 && errno != EEXIST) return false;
+            }
+        }
+        if (i < path.size()) cur.push_back(path[i]);
+    }
+    return true;
 }
 
 }  // namespace
@@ -363,7 +474,53 @@ int main(int argc, char** argv) {
     else if (o.play_camera_path && !cams.empty()) n_frames = std::max(o.frames, (int)cams.size() - 1);  // to the end
     int selected = 0, prev_selected = -1;
     std::vector<double> test_times;
+    // --create-evaluation-screenshots (main.cu:72-120): per camera, the
+    // progressive image after 1, 4, 16, 64, 256 and 1024 accumulated frames,
+    // saved as <image_name>_sppNNNN.png under <dir>/eval/<model name>/; then
+    // the next camera, up to camera max_eval_count or the last one.
+    static const int kSppTargets[6] = {1, 4, 16, 64, 256, 1024};
+    const bool eval = o.create_evaluation_screenshots;
+    int eval_idx = 0;
+    bool eval_next_camera = false;
+    const std::string eval_dir = o.screenshot_dir + "/eval/" + model_name(o);
+    if (eval) {
+        if (cams.empty()) {
+            std::fprintf(stderr, "--create-evaluation-screenshots needs cameras (--camera-path)\n");
+            return 1;
+        }
+        // the reference's harness counts progressive frames: with TAA the
+        // count is never reset per camera and with --accumulation-none it
+        // stays 1, so its next target is never reached
+        if (o.taa || o.accumulation_none) {
+            std::fprintf(stderr, "--create-evaluation-screenshots needs progressive accumulation (no --taa, "
+                                 "no --accumulation-none)\n");
+            return 1;
+        }
+        if (!make_dirs(eval_dir)) {
+            std::fprintf(stderr, "ERROR: Failed to ensure directory structure for screenshots.\n");
+            return 1;
+        }
+        n_frames = 0x7FFFFFFF;
+    }
     for (int fi = 1; fi <= n_frames; ++fi) {
+        std::string save_name;
+        if (eval) {
+            int next_spp = progressive_frames + 1;
+            if (eval_next_camera) {
+                if (selected >= (int)cams.size() - 1 || selected >= o.max_eval_count) break;
+                selected = selected + 1;  // change_camera_index(1); the accumulation restarts
+                eval_idx = 0;
+                eval_next_camera = false;
+                next_spp = 1;
+            }
+            if (eval_idx < 6 && next_spp == kSppTargets[eval_idx]) {
+                char suffix[16];
+                std::snprintf(suffix, sizeof(suffix), "_spp%04d", next_spp);
+                save_name = std::string(cams[(size_t)selected].img_name) + suffix;
+                ++eval_idx;
+            }
+            if (eval_idx >= 6) eval_next_camera = true;
+        }
         gps_frame_params f;
         bool moved = o.orbit_radius > 0;
         float fovy = o.fovy;
@@ -433,6 +590,15 @@ int main(int argc, char** argv) {
             if (frame_index >= n_test) test_times.push_back(ms);
             if (frame_index % n_test == 0 && frame_index > 0) selected = 0;  // load_camera_state(0)
         }
+        if (!save_name.empty()) {
+            CHECK(gps_memcpy_d2h(h_rgb.data(), rgb, npx * 3, stream));
+            const std::string path = eval_dir + "/" + save_name + ".png";
+            std::printf("Saving screenshot to: %s\n", path.c_str());
+            if (!write_png(path, h_rgb, W, H)) {
+                std::fprintf(stderr, "ERROR: Failed to save screenshot to %s\n", path.c_str());
+                return 1;
+            }
+        }
         if (!o.out.empty()) {
             CHECK(gps_memcpy_d2h(h_rgb.data(), rgb, npx * 3, stream));
             char name[64];
@@ -443,7 +609,7 @@ int main(int argc, char** argv) {
         }
         gps_stats st;
         CHECK(gps_pass_stats(pass, &st, stream));
-        if (!o.run_tests)
+        if (!o.run_tests && !eval)
             std::printf("frame %d: %.3f ms, points %llu + %llu, samples %llu\n", fi, ms,
                         (unsigned long long)st.num_points[0], (unsigned long long)st.num_points[1],
                         (unsigned long long)st.samples);

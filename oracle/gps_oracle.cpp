@@ -17,7 +17,10 @@
 //   resolve           src/core/rendering/passes/post_processing_kernels.cuh:108-285
 // Parallel loops use std::thread over index ranges; the framebuffer min uses a
 // CAS loop on std::atomic<uint64_t>, which is order-independent, so results
-// equal a serial run.
+// equal a serial run.  The biased splat (use_unbiased_2d_splatting = 0) runs
+// the reference's pre-check + atomicAddColor per sample on the same atomics;
+// its words are order-independent unless orc_pass_biased_events reports a
+// zero-colour sample or a colour-sum carry out of bit 35.
 //
 // Defined behaviours where the reference reads uninitialised memory:
 //   * the Hi-Z chain starts at +inf (nothing occluded before the first build);
@@ -451,6 +454,7 @@ struct orc_pass {
         iteration++;
     }
     uint64_t stats_total_weight = 0;
+    BiasedEvents biased_events;
 
     // gaussian_point_splatting.cu:67-141, 383-432
     void splat(const Frame& f, int frame_seed, uint32_t numPoints) {
@@ -492,7 +496,14 @@ struct orc_pass {
                     const int index = posToMemory(x, y, q / K, spp, f.up_w, f.up_h);
                     const uint64_t val = vdb | color;
                     std::atomic_ref<uint64_t> ref(fb[index]);
+                    if (!unbiased) {
+                        // :130-138, atomicAddColor branch
+                        splat_sample_biased(ref, val, &biased_events);
+                        continue;
+                    }
+                    // :130-135: the pre-check, then atomicMin (CAS loop)
                     uint64_t old = ref.load(std::memory_order_relaxed);
+                    if (old <= vdb) continue;
                     while (val < old && !ref.compare_exchange_weak(old, val, std::memory_order_relaxed)) {
                     }
                 }
@@ -550,6 +561,8 @@ struct orc_pass {
         stage_ms[0] += now_ms() - t0;
         stats = gps_stats{};
         stats.num_gaussians = N;
+        biased_events.zero_colour = 0;
+        biased_events.wrapped = 0;
         int K = f.ss * f.ss;
         if (!p->reduce_point_count) K = 1;
         stats.K = K;
@@ -748,6 +761,20 @@ int32_t orc_pass_render(orc_pass* p, const gps_frame_params* f, float* color, fl
 int32_t orc_pass_stats(orc_pass* p, gps_stats* out) {
     *out = p->stats;
     return GPS_OK;
+}
+
+void orc_pass_biased_events(orc_pass* p, uint64_t out[2]) {
+    out[0] = p->biased_events.zero_colour.load();
+    out[1] = p->biased_events.wrapped.load();
+}
+
+uint64_t orc_biased_sample(uint64_t* word, uint64_t value, int32_t precheck, int32_t add_if_equal) {
+    std::atomic_ref<uint64_t> ref(*word);
+    if (precheck) {
+        splat_sample_biased(ref, value);
+        return 1;
+    }
+    return atomicAddColor(ref, value, add_if_equal != 0) ? 1 : 0;
 }
 
 void orc_pass_stage_ms(orc_pass* p, double out[8]) {

@@ -60,6 +60,16 @@ int32_t orc_pass_render(orc_pass* pass, const gps_frame_params* frame, float* co
 int32_t orc_pass_stats(orc_pass* pass, gps_stats* out);
 uint64_t orc_pass_debug_size(orc_pass* pass, int32_t which);
 int32_t orc_pass_debug_copy(orc_pass* pass, int32_t which, void* dst, uint64_t count);
+/* Biased-splat events of the last render (use_unbiased_2d_splatting = 0):
+ * out[0] = samples of packed colour 0 that passed the pre-check, out[1] =
+ * equal-depth adds that carried out of the 36 colour bits.  When both are 0
+ * every framebuffer word is independent of the order of its samples. */
+void orc_pass_biased_events(orc_pass* pass, uint64_t out[2]);
+/* One biased sample on one word: precheck != 0 applies the splat's pre-check
+ * and atomicAddColor(add_if_equal = true) (gaussian_point_splatting.cu:130-138);
+ * precheck == 0 applies atomicAddColor alone with the given add_if_equal and
+ * returns its result (post_processing_kernels.cuh:13-42). */
+uint64_t orc_biased_sample(uint64_t* word, uint64_t value, int32_t precheck, int32_t add_if_equal);
 /* Per-stage wall times (ms) of the last render: clear, cull, preprocess,
  * distribute, splat, combine, hiz. */
 void orc_pass_stage_ms(orc_pass* pass, double out[8]);

@@ -68,6 +68,8 @@ _sig("orc_pass_stats", C.c_int32, _vp, C.POINTER(Stats))
 _sig("orc_pass_debug_size", C.c_uint64, _vp, C.c_int32)
 _sig("orc_pass_debug_copy", C.c_int32, _vp, C.c_int32, _vp, C.c_uint64)
 _sig("orc_pass_stage_ms", None, _vp, C.POINTER(C.c_double))
+_sig("orc_pass_biased_events", None, _vp, _vp)
+_sig("orc_biased_sample", C.c_uint64, _vp, C.c_uint64, C.c_int32, C.c_int32)
 _sig("orc_pass_cull", C.c_int32, _vp, C.POINTER(FrameParams))
 _sig("orc_project", C.c_int32, _vp, C.POINTER(FrameParams), C.c_int32, _vp)
 _sig("orc_resolve_taa", None, _vp, _vp, _vp, _vp, C.POINTER(C.c_float), C.c_float, C.c_int32, C.c_int32, C.c_int32,
@@ -87,6 +89,16 @@ _sig("orc_sh_eval", None, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
 _sig("orc_sh_eval_compact", None, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_double))
 
 lib = _lib
+
+
+def biased_sample(word: int, value: int, precheck: bool = True, add_if_equal: bool = True):
+    """Apply one biased-splat sample to a framebuffer word.  precheck: the
+    splat's pre-check + atomicAddColor (gaussian_point_splatting.cu:130-138);
+    otherwise atomicAddColor alone (post_processing_kernels.cuh:13-42).
+    Returns (new word, atomicAddColor's result or 1)."""
+    w = np.array([word], np.uint64)
+    r = _lib.orc_biased_sample(_ptr(w), int(value), int(precheck), int(add_if_equal))
+    return int(w[0]), int(r)
 
 
 def detmath(kind: int, x: np.ndarray) -> np.ndarray:
@@ -239,6 +251,13 @@ class OraclePass:
         s = Stats()
         _lib.orc_pass_stats(self.handle, C.byref(s))
         return s.as_dict()
+
+    def biased_events(self) -> dict:
+        """Order-sensitive events of the last render's biased splat: samples of
+        colour 0 past the pre-check, colour sums carried out of bit 35."""
+        out = np.zeros(2, np.uint64)
+        _lib.orc_pass_biased_events(self.handle, _ptr(out))
+        return {"zero_colour": int(out[0]), "wrapped": int(out[1])}
 
     def stage_ms(self) -> list:
         out = (C.c_double * 8)()

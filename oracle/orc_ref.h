@@ -6,6 +6,8 @@
 // TEST INFRASTRUCTURE: part of the CPU oracle.
 #pragma once
 
+#include <atomic>
+
 #include "../include/gps.h"
 #include "orc_glm.h"
 
@@ -430,6 +432,62 @@ inline uint64_t get_color_packed(vec3<float> c) {
     uint32_t b12 = gps::det_f2u_rz(c.z * 255.0f) & 0xFFF;
     return (uint64_t(r12) << 24) | (uint64_t(g12) << 12) | (uint64_t(b12) << 0);
 }
+// config.h:36-41
+constexpr uint64_t COLOR_MASK = (1ull << 36) - 1;
+constexpr uint64_t DEPTH_MASK = ((1ull << 28) - 1) << 36;
+
+// Events of the biased splat whose outcome depends on the order in which
+// samples reach a word (see splat_sample_biased).
+struct BiasedEvents {
+    std::atomic<uint64_t> zero_colour{0};  // a sample of packed colour 0 passed the pre-check
+    std::atomic<uint64_t> wrapped{0};      // an equal-depth add carried out of bit 35
+};
+
+// post_processing_kernels.cuh:13-42, atomicAddColor, on any word type with
+// load() / compare_exchange_strong() (std::atomic_ref<uint64_t>): a nearer
+// sample replaces the word, a farther one is ignored, one at equal depth adds
+// its 36 colour bits to the stored ones modulo 2^36 (a 12-bit channel that
+// overflows carries into the next; the top one is dropped).  Returns false
+// only when add_if_equal is false and the depths are equal.
+template <typename Word> inline bool atomicAddColor(Word& w, uint64_t value, bool add_if_equal,
+                                                   BiasedEvents* ev = nullptr) {
+    const uint64_t target_depth = value & DEPTH_MASK;
+    uint64_t old = w.load(std::memory_order_relaxed), assumed;
+    do {
+        assumed = old;
+        const uint64_t assumed_depth = assumed & DEPTH_MASK;
+        uint64_t new_value = value;
+        if (target_depth > assumed_depth) {
+            return true;
+        } else if (target_depth == assumed_depth) {
+            if (!add_if_equal) return false;
+            const uint64_t new_lower = (assumed & COLOR_MASK) + (value & COLOR_MASK);
+            if (ev && (new_lower >> 36)) ev->wrapped.fetch_add(1, std::memory_order_relaxed);
+            new_value = (assumed & DEPTH_MASK) | (new_lower & COLOR_MASK);
+        } else {
+            new_value = value;
+        }
+        old = assumed;
+        w.compare_exchange_strong(old, new_value, std::memory_order_relaxed);
+    } while (assumed != old);
+    return true;
+}
+
+// gaussian_point_splatting.cu:130-138, biased branch: the racy pre-check
+// `imageBuffer[index] <= view_depth_bits` (a stored word that is nearer, or
+// at the same depth with colour 0, skips the sample), then atomicAddColor
+// with add_if_equal.  The word's final value is independent of the order of
+// its samples unless a sample of colour 0 arrives first at the word's final
+// depth (the pre-check then skips every later add there) or the colours
+// summed at one depth wrap to 0 modulo 2^36; `ev` counts the samples that
+// could start either case.
+template <typename Word> inline void splat_sample_biased(Word& w, uint64_t value, BiasedEvents* ev = nullptr) {
+    const uint64_t view_depth_bits = value & DEPTH_MASK;
+    if (w.load(std::memory_order_relaxed) <= view_depth_bits) return;
+    if (ev && (value & COLOR_MASK) == 0) ev->zero_colour.fetch_add(1, std::memory_order_relaxed);
+    atomicAddColor(w, value, true, ev);
+}
+
 // sh.cuh:124-133
 inline uint64_t vec3To36Bits(vec3<float> rgb) {
     const float a = 255.f;

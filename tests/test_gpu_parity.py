@@ -106,6 +106,8 @@ def run_pair(arrays, settings, frames, fmt=0, which=H.INTERMEDIATES, check_recor
         rg = g.render(f, which)
         flush(f"frame {fi}: render oracle")
         ro = o.render(f, which)
+        if not f.use_unbiased_2d_splatting:
+            assert ro.stats["biased_events"] == {"zero_colour": 0, "wrapped": 0}, ro.stats["biased_events"]
         for k in which:
             assert_same(f"frame {fi} debug[{k}]", rg.debug[k], ro.debug[k])
         assert_same(f"frame {fi} color", rg.color, ro.color)
@@ -202,6 +204,39 @@ def test_frame_edge_cases(gpu, case):
         assert rg.stats["budget_truncated"] == 1 and rg.stats["num_points"][0] == 1000
     if case in ("empty", "behind"):
         assert rg.stats["num_points"][0] == 0
+    if case == "biased":
+        # equal-depth samples added their colours: some word's colour exceeds
+        # every single Gaussian's (the min-splat would never do that)
+        fb = rg.debug[H.G.DBG_FRAMEBUFFER]
+        assert rg.stats["splat_binned"] == 0
+        assert np.count_nonzero(fb != ro.debug[H.G.DBG_FRAMEBUFFER]) == 0
+
+
+@pytest.mark.parametrize("ss,spp,occl", [(1, 1, False), (2, 4, False), (2, 1, True)])
+def test_frame_biased_splat(gpu, ss, spp, occl):
+    """Biased splat (use_unbiased_2d_splatting = 0): the reference's
+    pre-check + atomicAddColor per sample.  Large Gaussians with many samples
+    per pixel so equal-depth adds are common; the oracle reports no
+    zero-colour sample and no colour carry out of bit 35, so every word is
+    independent of sample order and the framebuffers must match bit for bit.
+    Also with 4 samples per point per layer (where the unbiased splat would
+    take the binned form) and with two-pass occlusion + block culling."""
+    flush(f"biased ss={ss} spp={spp} occl={occl}")
+    arrays = H.scene(0x5EED0B00 + 10 * ss + spp, 8_000, dict(H.C1, log_scale_mu=-1.5, log_scale_sigma=0.3))
+    arrays.opacities[:] = np.clip(arrays.opacities, 0.3, 0.95)
+    st = gpu.RenderSettings(320, 200, spp, ss, fully_disable_hierarchical_culling=not occl, gaussians_per_block=64)
+    frames = moving_frames(2, 320, 200, ss, unbiased=False, occlusion=occl, hierarchical=occl)
+    which = H.INTERMEDIATES + ([H.G.DBG_HIZ] if occl else [])
+    res = run_pair(arrays, st, frames, which=which)
+    for rg, ro in res:
+        assert rg.stats["splat_binned"] == 0
+        fb = rg.debug[H.G.DBG_FRAMEBUFFER]
+        bg = np.uint64(((1 << 28) - 1) << 36)
+        hit = fb[fb != bg]
+        assert hit.size > 1000
+        # some word holds more than one sample's colour: its 12-bit red
+        # channel exceeds 255 (one sample's colour is at most ~1.5 * 255 here)
+        assert int(((hit >> np.uint64(24)) & np.uint64(0xFFF)).max()) > 600
 
 
 def moving_frames(n_frames, w, h, ss, radius=40.0, **kw):
@@ -462,6 +497,7 @@ def test_batch_views_identical_to_single_renders(gpu):
     frames = W.batch_views(bw, 0)
     frames[1].enable_hierarchical_culling = 0
     frames[2].supersampling_factor = 1
+    frames[4].use_unbiased_2d_splatting = 0  # biased splat (atomicAddColor) in the mix
     ref = alone(frames)
     cols, deps = outputs(frames)
     for t, q in ((256, 4), (256, 2), (256, 8), (128, 4), (512, 2), (512, 4)):
@@ -469,12 +505,13 @@ def test_batch_views_identical_to_single_renders(gpu):
         batch.set_tuning(t, q, False)
         batch.render(frames, cols, deps, stream)
         check_batch(frames, cols, deps, ref, f"eager {t}x{q}")
-    # oracle for two views at their exact sizes
-    for v in (0, 3):
+    # oracle for three views at their exact sizes (view 4: biased)
+    for v in (0, 3, 4):
         f = frames[v]
         o = orc.OraclePass(gpu.RenderSettings(f.width, f.height, f.samples_per_pixel, f.supersampling_factor),
                            arrays, w.fmt, 8)
         c, _ = o.render(f)
+        assert o.biased_events() == {"zero_colour": 0, "wrapped": 0}
         assert_same(f"view {v} vs oracle", ref[v][0], c)
     tuned = batch.autotune(frames, cols, deps, stream, repeats=1)
     flush(f"autotune: {tuned}")
@@ -668,6 +705,86 @@ def test_cli_camera_path_run_tests(gpu, tmp_path):
         head = fh.read(32).split()
     assert head[0] == b"P6" and (int(head[1]), int(head[2])) == (320, round(320 * 3286 / 4946))
     assert os.path.exists(out + f"_{3 * 4 + 1:04d}.ppm")
+
+
+def read_png_rgb(path):
+    """RGB8 PNG (8-bit, colour type 2, no interlace) -> (h, w, 3) array."""
+    import struct
+    import zlib
+    with open(path, "rb") as fh:
+        d = fh.read()
+    assert d[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, hdr = 8, b"", None
+    while pos < len(d):
+        n = struct.unpack(">I", d[pos:pos + 4])[0]
+        t, data = d[pos + 4:pos + 8], d[pos + 8:pos + 8 + n]
+        assert zlib.crc32(t + data) & 0xFFFFFFFF == struct.unpack(">I", d[pos + 8 + n:pos + 12 + n])[0]
+        if t == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", data)
+        elif t == b"IDAT":
+            idat += data
+        pos += 12 + n
+    w, h, depth, ctype = hdr[0], hdr[1], hdr[2], hdr[3]
+    assert depth == 8 and ctype == 2
+    rows = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, w * 3 + 1)
+    assert (rows[:, 0] == 0).all()  # filter "none"
+    return rows[:, 1:].reshape(h, w, 3)
+
+
+def test_cli_evaluation_screenshots(gpu, tmp_path):
+    """--create-evaluation-screenshots / --max-eval-count (main.cu:72-120,
+    render_server.cu:263-315): per camera the progressive image after 1, 4,
+    16, 64, 256 and 1024 frames, named <image_name>_sppNNNN.png under
+    <dir>/eval/<model name>/, for cameras 0 .. max_eval_count; the first
+    image equals the same frame rendered and resolved through the library,
+    rows flipped as save_image_to_disk flips them."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "gaussian-point-splatting_amd", "gps_render")
+    fix = os.path.join(root, "tests", "golden", "truck_cameras_head.json")
+    flush("cli: evaluation screenshots")
+    r = subprocess.run([exe, "--synthetic", "c1", "--num-gaussians", "20000", "--camera-path", fix,
+                        "--create-evaluation-screenshots", "--max-eval-count", "1", "--set-target-width", "160",
+                        "--screenshot-dir", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    cams = gpu.camera_path_load(fix)
+    names = [c.img_name.decode() for c in cams]
+    d = tmp_path / "eval" / "synthetic_c1"
+    want = sorted(f"{names[c]}_spp{s:04d}.png" for c in (0, 1) for s in (1, 4, 16, 64, 256, 1024))
+    assert sorted(os.listdir(d)) == want
+    w, h = 160, round(160 * 3286 / 4946)
+    img = {s: read_png_rgb(str(d / f"{names[0]}_spp{s:04d}.png")) for s in (1, 4, 256, 1024)}
+    assert img[1].shape == (h, w, 3)
+    # frame 1 of camera 0 through the library: same scene, pass, camera, resolve
+    arrays = H.scene(0x5EED0001, 20_000, H.C1)
+    scene = gpu.Scene(arrays)
+    ps = gpu.Pass(gpu.RenderSettings(w, h, 1, 2), scene)
+    f = gpu.frame_from_camera(cams[0], w, h, 0.0, 2, 0.2, 1e4)
+    view = gpu.mat4_mul(np.array(f.view, np.float32), np.eye(4, dtype=np.float32).reshape(-1))  # scene matrix I
+    for i in range(16):
+        f.view[i] = float(view[i])
+    planes = np.empty(24, np.float32)
+    gpu._lib.gps_frustum_planes(f.fovy_degrees, view.ctypes.data_as(gpu._fp), w / h, 0.2, 1e4,
+                                planes.ctypes.data_as(gpu._fp))
+    for i in range(24):
+        f.frustum_planes[i] = float(planes[i])
+    f.frame = 1
+    f.enable_occlusion_culling = 1
+    f.enable_hierarchical_culling = 1
+    f.cull_small_gaussians = 0
+    f.reduce_point_count = 1
+    f.use_unbiased_2d_splatting = 1
+    color = gpu.DeviceArray((h, w, 3), np.float32, zero=True)
+    accum = gpu.DeviceArray((h, w, 3), np.float32, zero=True)
+    depth = gpu.DeviceArray((h, w), np.float32, zero=True)
+    rgb = gpu.DeviceArray((h, w, 3), np.uint8, zero=True)
+    ps.render(f, color, depth)
+    gpu.resolve_progressive(color, accum, w, h, 1, rgb)
+    assert_same("eval spp0001", img[1], rgb.download()[::-1])
+    # the accumulation converges: 1024 frames are closer to 256 than 4 are
+    err = {s: np.abs(img[s].astype(np.int32) - img[256].astype(np.int32)).mean() for s in (4, 1024)}
+    assert err[1024] < err[4], err
 
 
 def test_selftest_atomics_min(gpu):
