@@ -12,8 +12,13 @@ RNG point draws per frame, summed over the timed frames / timed seconds).
 The per-kernel roofline uses the HIP events the library records around every
 stage on the render stream during the timed region.
 
-The CPU baseline is the oracle (plain C++ transcription, test
-infrastructure) timed on a bounded sample of the same workload.
+Parity (unless --no-parity): the workload at its full size against the oracle
+(plain C++ transcription, test infrastructure, used only as the checker):
+the first path positions in order from a fresh pass, later ones with the
+frame state carried from the GPU; bit-exactness and PSNR go into the line.
+The CPU baseline is the oracle's time for the first timed frame of that
+check (the full scene, one frame); with --no-parity, the oracle on every
+50-th Gaussian of the scene.
 """
 from __future__ import annotations
 
@@ -118,6 +123,240 @@ def cpu_baseline(G, W, w, sample_n, threads, perm):
             "frame_s": dt}
 
 
+def _psnr(a, b):
+    """PSNR in dB against a peak of 1.0 ("inf" for identical images)."""
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return "inf" if mse == 0.0 else round(10.0 * math.log10(1.0 / mse), 2)
+
+
+def _host_scene_for_oracle(G, W, w, perm, orc, threads):
+    """The oracle's copy of the resident scene, at its full size, generated
+    in the same load order and encoded in chunks into the storage format the
+    GPU holds (the compact format is never expanded to fp32 on the host)."""
+    n = w.n
+    if w.fmt != G.FORMAT_HALF_FP8:
+        return orc.OraclePass(w.settings(), W.host_scene(w, perm), w.fmt, threads)
+    means = np.empty((n, 3), np.float32)
+    enc = {"rotations": np.empty((n, 4), np.uint16), "scales": np.empty((n, 3), np.uint16),
+           "opacities": np.empty((n,), np.uint16), "sh_dc": np.empty((n, 3), np.uint16),
+           "sh_rest": np.empty((n, 45), np.uint8), "sh_scale": np.empty((n,), np.float32)}
+    ch = 1 << 22
+    for b in range(0, n, ch):
+        e = min(n, b + ch)
+        a = W._generate(w, perm[b:e])
+        means[b:e] = a.means
+        ce = G.encode_half_fp8(a)
+        for k in enc:
+            enc[k][b:e] = ce[k]
+        del a, ce
+    return orc.OraclePass.from_half_fp8(w.settings(), means, enc, w.sh_degree, threads)
+
+
+def parity_leg(G, W, w, scene, perm, threads, in_order, carried):
+    """Parity at the workload's full size: the GPU path against the oracle
+    on the same scene, frame by frame.  Positions in `in_order` are rendered
+    by both from a fresh pass in path order; for a position in `carried` the
+    GPU renders the path up to it and hands the oracle the state a frame
+    carries into the next (Hi-Z chain, the valid mask the next first pass
+    leaves in place) and the TAA history, then both render that position.
+    Compared: framebuffer words, colour and depth images (bit for bit and as
+    PSNR), the TAA output and its RGB8 image.  The oracle's render time of
+    each frame is kept for the CPU baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc  # the checker, test infrastructure
+    t_start = time.time()
+    o = _host_scene_for_oracle(G, W, w, perm, orc, threads)
+    log(f"[parity] oracle scene of {w.n} Gaussians ready in {time.time() - t_start:.1f}s")
+    Wd, Hd = w.width, w.height
+    stream = G.Stream()
+    loop = W.FrameLoop(w, scene, stream)
+    p = loop.pass_
+    o_color = [np.zeros((Hd, Wd, 3), np.float32) for _ in range(2)]
+    o_depth = [np.zeros((Hd, Wd), np.float32) for _ in range(2)]
+    o_hist = [np.zeros((Hd, Wd, 3), np.float32) for _ in range(2)]
+    check = set(in_order) | set(carried)
+    frames = []
+    oracle_in_order_done = 0
+    for pos in range(max(check) + 1):
+        f = W.frame(w, pos % w.path_frames, pos + 1)
+        if pos not in check:
+            loop.step(f)
+            continue
+        k = loop.frames_done + 1
+        a, b = k % 2, 1 - k % 2
+        prev_vp = loop.prev_vp.copy()
+        if pos in carried:
+            stream.synchronize()
+            st = p.stats(stream)
+            passes = st["passes"]
+            flipped = passes > 0 and st["num_points"][passes - 1] > 0
+            other = p.debug(G.DBG_MASK_CURRENT if flipped else G.DBG_MASK_PREVIOUS, stream)
+            o.set_state(p.debug(G.DBG_HIZ, stream), other)
+            if w.taa:
+                o_depth[a] = loop.depth[a].download(stream)
+                o_hist[b] = loop.hist[b].download(stream)
+            state = f"Hi-Z + mask + TAA history carried from the GPU after position {pos - 1}"
+        else:
+            assert oracle_in_order_done == pos, "in-order positions must start at 0 and be consecutive"
+            oracle_in_order_done += 1
+            state = "in path order from a fresh pass"
+        loop.step(f)
+        stream.synchronize()
+        fb = p.debug(G.DBG_FRAMEBUFFER, stream)
+        g_color = loop.color[a].download(stream)
+        g_depth = loop.depth[b].download(stream)
+        st = p.stats(stream)
+        t0 = time.perf_counter()
+        oc, od = o.render(f)
+        dt = time.perf_counter() - t0
+        so = o.stats()
+        ofb = o.debug(G.DBG_FRAMEBUFFER)
+        o_color[a], o_depth[b] = oc, od
+        rec = {"position": pos, "state": state, "samples": int(so["samples"]), "oracle_s": round(dt, 3),
+               "fb_words": int(fb.size), "fb_words_differ": int(np.count_nonzero(fb != ofb)),
+               "color_bit_exact": bool(np.array_equal(g_color.view(np.uint32), oc.view(np.uint32))),
+               "color_psnr_db": _psnr(g_color, oc),
+               "depth_bit_exact": bool(np.array_equal(g_depth.view(np.uint32), od.view(np.uint32))),
+               "counters_equal": all(st[key] == so[key] for key in ("valid_gaussians", "num_points",
+                                                                     "culled_blocks", "passes"))}
+        if w.taa:
+            M = G.taa_reprojection(prev_vp, np.array(f.view, np.float32), np.array(f.proj, np.float32))
+            o_hist[a], o_rgb = orc.resolve_taa(o_color[a], o_depth[a], o_hist[b], o_depth[b], M, 0.9, Wd, Hd, k)
+            g_hist = loop.hist[a].download(stream)
+            rec["taa_bit_exact"] = bool(np.array_equal(g_hist.view(np.uint32), o_hist[a].view(np.uint32)))
+            rec["taa_psnr_db"] = _psnr(g_hist, o_hist[a])
+            rec["rgb8_equal"] = bool(np.array_equal(loop.rgb.download(stream), o_rgb))
+        log(f"[parity] position {pos}: {rec}")
+        frames.append(rec)
+    loop.close()
+    o.close()
+    exact = all(r["fb_words_differ"] == 0 and r["color_bit_exact"] and r["depth_bit_exact"] and r["counters_equal"]
+                and r.get("taa_bit_exact", True) and r.get("rgb8_equal", True) for r in frames)
+    psnrs = [r[k] for r in frames for k in ("color_psnr_db", "taa_psnr_db") if k in r and r[k] != "inf"]
+    return {"workload": f"{w.name} at full size ({w.n} Gaussians, {w.width}x{w.height}, ss {w.ss}, spp {w.spp})",
+            "oracle_threads": threads, "bit_exact": exact, "min_psnr_db": min(psnrs) if psnrs else "inf",
+            "frames": frames, "seconds": round(time.time() - t_start, 1)}
+
+
+def _read_csv(path):
+    import csv
+    with open(path, newline="") as fh:
+        return list(csv.DictReader(fh))
+
+
+COPY_CAL_BYTES = 1 << 30  # the calibration copy of --profile-window: 1 GiB read + 1 GiB written
+
+
+def _kname(r):
+    return r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+
+
+def _window(trace_csv, gap_s):
+    """Dispatches after the last idle gap of more than gap_s seconds."""
+    rows = _read_csv(trace_csv)
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    cut = 0
+    end = int(rows[0]["End_Timestamp"]) if rows else 0
+    for i in range(1, len(rows)):
+        if int(rows[i]["Start_Timestamp"]) - end > gap_s * 1e9:
+            cut = i
+        end = max(end, int(rows[i]["End_Timestamp"]))
+    return rows, rows[cut:]
+
+
+def summarize_trace(traces, counters=(), gap_s=0.5, traffic_out=None, window_name=""):
+    """Per-kernel statistics of the timed frames of --profile-window runs
+    under rocprofv3: the dispatches after the last idle gap of more than
+    `gap_s` seconds (the run sleeps between the warm-up and the timed
+    frames).  The first trace gives the kernel times.  Each further trace
+    comes with the counter CSV of the same run (--counters, in order), whose
+    per-dispatch means over the same window are listed; FETCH_SIZE and
+    WRITE_SIZE (KiB) are also scaled by what they read for the calibration
+    copy of exactly 1 GiB each way that closes the window (selftest_copy_kernel)
+    and, with `traffic_out`, written as HBM bytes per dispatch per stage."""
+    rows, win = _window(traces[0], gap_s)
+    per = {}
+    for r in win:
+        d = per.setdefault(_kname(r), {"calls": 0, "ns": 0})
+        d["calls"] += 1
+        d["ns"] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    total = sum(d["ns"] for k, d in per.items() if k != "selftest_copy_kernel") or 1
+    t0, t1 = int(win[0]["Start_Timestamp"]), max(int(r["End_Timestamp"]) for r in win)
+    lines = [f"window: {len(win)} of {len(rows)} dispatches (after an idle gap > {gap_s} s), "
+             f"{(t1 - t0) / 1e6:.3f} ms first start to last end",
+             f"{'kernel':50s} {'calls':>6s} {'avg us':>10s} {'total ms':>10s} {'share':>6s}"]
+    for k, d in sorted(per.items(), key=lambda kv: -kv[1]["ns"]):
+        share = f"{100.0 * d['ns'] / total:5.1f}%" if k != "selftest_copy_kernel" else "  cal."
+        lines.append(f"{k[:50]:50s} {d['calls']:6d} {d['ns'] / d['calls'] / 1e3:10.1f} {d['ns'] / 1e6:10.3f} {share}")
+    means = {}  # (kernel, counter) -> mean per dispatch
+    for tr, cc in zip(traces[1:], counters):
+        _, w2 = _window(tr, gap_s)
+        ids = {}
+        for r in w2:
+            ids.setdefault(_kname(r), set()).add(r["Dispatch_Id"])
+        crows = _read_csv(cc)
+        sums = {}
+        for r in crows:
+            k = _kname(r)
+            if r["Dispatch_Id"] not in ids.get(k, ()):
+                continue
+            acc = sums.setdefault((k, r["Counter_Name"]), [0.0, set()])
+            acc[0] += float(r["Counter_Value"])
+            acc[1].add(r["Dispatch_Id"])
+        for key, (v, dset) in sums.items():
+            means[key] = (v / len(dset), len(dset))
+    if means:
+        lines.append("")
+        lines.append(f"{'kernel':50s} {'counter':24s} {'per dispatch':>16s} {'dispatches':>10s}")
+        for (k, c), (v, n) in sorted(means.items(), key=lambda kv: (-per.get(kv[0][0], {"ns": 0})["ns"], kv[0][1])):
+            lines.append(f"{k[:50]:50s} {c:24s} {v:16.1f} {n:10d}")
+    cal = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        m = means.get(("selftest_copy_kernel", c))
+        if m and m[0] > 0:
+            cal[c] = COPY_CAL_BYTES / (m[0] * 1024.0)
+    if cal:
+        lines.append("")
+        lines.append("calibration (1 GiB copy each way): " +
+                     ", ".join(f"{c} x {f:.3f} per KiB counted = bytes" for c, f in cal.items()))
+        stage_of = {"preprocess_kernel": "preprocess", "splat_kernel": "splat", "distribute_kernel": "distribute",
+                    "tile_scan_kernel": "distribute", "combine_staged_kernel": "combine", "combine_kernel": "combine",
+                    "hiz_build_kernel": "hiz", "hiz_top_kernel": "hiz", "cull_kernel": "cull", "clear_kernel": "clear",
+                    "taa_kernel": "taa", "progressive_kernel": "resolve"}
+        traffic = {}
+        for (k, c), (v, n) in means.items():
+            base = k.split("<")[0]
+            if c not in cal or base not in stage_of:
+                continue
+            calls = per.get(k, {"calls": n})["calls"]
+            st = traffic.setdefault(stage_of[base], {"fetch_bytes": 0.0, "write_bytes": 0.0, "kernels": []})
+            # a stage's bytes per launch: its kernels' bytes per dispatch times their dispatches per stage launch
+            st["fetch_bytes" if c == "FETCH_SIZE" else "write_bytes"] += v * 1024.0 * cal[c] * calls
+            if k not in st["kernels"]:
+                st["kernels"].append(k)
+            st["_calls"] = max(st.get("_calls", 0), calls)
+        out = {"_meta": {"window": window_name, "calibration": {c: round(f, 4) for c, f in cal.items()},
+                         "unit": "HBM bytes per stage launch = sum over the stage's kernels of "
+                                 "(FETCH_SIZE KiB x 1024 x fetch factor + WRITE_SIZE KiB x 1024 x write factor) "
+                                 "per dispatch, x their dispatches per stage launch"}}
+        for stg, d in traffic.items():
+            calls = d.pop("_calls")
+            launches = max(per[k]["calls"] for k in d["kernels"] if k in per) if any(k in per for k in d["kernels"]) \
+                else calls
+            # kernels that run once per stage launch (the common case): per-dispatch bytes
+            f = d["fetch_bytes"] / launches
+            wb = d["write_bytes"] / launches
+            out[stg] = round(f + wb)
+            out.setdefault("_detail", {})[stg] = {"fetch_bytes": round(f), "write_bytes": round(wb),
+                                                  "kernels": d["kernels"]}
+        lines.append("traffic per stage launch (bytes): " +
+                     ", ".join(f"{k} {v}" for k, v in out.items() if not k.startswith("_")))
+        if traffic_out:
+            with open(traffic_out, "w") as fh:
+                json.dump(out, fh, indent=1)
+    return "\n".join(lines)
+
+
 def serving(G, W, args, resident):
     """Serving workload: resident 1M / 10M / 100M scenes, Poisson requests at
     200/s of mixed resolution and spp for `--serve-seconds`, admitted by the
@@ -190,6 +429,101 @@ def serving(G, W, args, resident):
             "closed_loop_p99_ms": round(batched["p99_ms"], 3), "sustained": sustained,
             "max_batch": int(st["max_batch"]),
             "hbm_used_gb": round(st["hbm_used_bytes"] / 1e9, 2)}
+
+
+def _e4m3_table():
+    """FP8 E4M3 (OCP, finite-only) value of every code, from its bit fields."""
+    codes = np.arange(256)
+    sign = np.where(codes & 0x80, -1.0, 1.0)
+    e = (codes >> 3) & 0xF
+    m = codes & 0x7
+    val = np.where(e == 0, m / 8.0 * 2.0 ** -6, (1.0 + m / 8.0) * 2.0 ** (e.astype(np.float64) - 7))
+    val = np.where((e == 15) & (m == 7), np.nan, val)
+    return sign * val
+
+
+def attribute_errors(G, W, w, perm, chunk=1 << 21):
+    """Largest error of the compact storage (fp16 rotation / scale / opacity,
+    fp8-E4M3 SH coefficients with a per-Gaussian fp32 scale) against the
+    original values, all in float64, over the Gaussians `perm` lists
+    (relative rotation error over components in fp16's normal range)."""
+    lut = _e4m3_table()
+    worst = {"rotation_abs": 0.0, "rotation_rel": 0.0, "scale_rel": 0.0, "opacity_abs": 0.0, "sh_dc_abs": 0.0,
+             "sh_rest_abs": 0.0, "sh_rest_rel_to_max": 0.0}
+    nc = (w.sh_degree + 1) ** 2
+    for b in range(0, w.n, chunk):
+        a = W._generate(w, perm[b:min(w.n, b + chunk)])
+        e = G.encode_half_fp8(a)
+        q = a.rotations.astype(np.float64)
+        dq = e["rotations"].view(np.float16).astype(np.float64) - q
+        worst["rotation_abs"] = max(worst["rotation_abs"], float(np.abs(dq).max()))
+        nz = np.abs(q) >= 2.0 ** -14  # fp16 normal range
+        worst["rotation_rel"] = max(worst["rotation_rel"], float((np.abs(dq[nz]) / np.abs(q[nz])).max()))
+        sc = a.scales.astype(np.float64)
+        worst["scale_rel"] = max(worst["scale_rel"],
+                                 float((np.abs(e["scales"].view(np.float16).astype(np.float64) - sc) / sc).max()))
+        worst["opacity_abs"] = max(worst["opacity_abs"], float(np.abs(
+            e["opacities"].view(np.float16).astype(np.float64) - a.opacities.astype(np.float64)).max()))
+        sh = a.shs.reshape(-1, nc * 3).astype(np.float64)
+        worst["sh_dc_abs"] = max(worst["sh_dc_abs"], float(np.abs(
+            e["sh_dc"].view(np.float16).astype(np.float64) - sh[:, :3]).max()))
+        if nc > 1:
+            rest = sh[:, 3:]
+            dec = lut[e["sh_rest"][:, :rest.shape[1]]] * e["sh_scale"].astype(np.float64)[:, None]
+            err = np.abs(dec - rest)
+            mx = np.abs(rest).max(axis=1)
+            worst["sh_rest_abs"] = max(worst["sh_rest_abs"], float(err.max()))
+            ok = mx > 0
+            worst["sh_rest_rel_to_max"] = max(worst["sh_rest_rel_to_max"],
+                                              float((err[ok].max(axis=1) / mx[ok]).max()))
+        del a, e
+    return {k: float(f"{v:.4g}") for k, v in worst.items()}
+
+
+def precision_leg(G, W, args):
+    """The plain (fp32) and the chosen (compact: fp16 rotation / scale /
+    opacity, fp8-E4M3 SH with a per-Gaussian scale; positions, projection
+    and depth fp32) storage of the 10M SH3 scene on the same GPU: the same
+    camera and RNG frames rendered `--precision-frames` times and
+    accumulated progressively in each form; PSNR of the compact form's
+    converged image against the fp32 one, beside the Monte-Carlo noise floor
+    (two independent fp32 accumulations of the same length); frames/s of
+    each form; the largest per-attribute storage error against float64."""
+    w32 = W.WORKLOADS["c2"]
+    w8 = W.replace(w32, fmt=G.FORMAT_HALF_FP8)
+    M = args.precision_frames
+    perm = W.order(w32)
+    stream = G.Stream()
+    out = {"workload": f"{w32.n} clustered SH3 Gaussians, {w32.width}x{w32.height} ss {w32.ss} spp {w32.spp}, orbit "
+                       f"position 0, {M} frames accumulated (progressive), RNG frames 1..{M} in both forms",
+           "frames": M}
+    imgs = {}
+    for name, w in (("fp32", w32), ("half_fp8", w8)):
+        scene = W.device_scene(w, perm, log=lambda m: log("[precision] " + m))
+        seqs = [range(1, M + 1)] + ([range(M + 1, 2 * M + 1)] if name == "fp32" else [])
+        for k, frames in enumerate(seqs):
+            loop = W.FrameLoop(w, scene, stream)
+            loop.step(W.frame(w, 0, frames[0]))  # warm-up frame, restarted below
+            loop.close()
+            loop = W.FrameLoop(w, scene, stream)
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for fr in frames:
+                loop.step(W.frame(w, 0, fr))
+            stream.synchronize()
+            dt = time.perf_counter() - t0
+            imgs[(name, k)] = loop.output().astype(np.float64) / M
+            loop.close()
+            if k == 0:
+                out[name] = {"fps": round(M / dt, 2), "device_gb": round(scene.device_bytes / 1e9, 3)}
+        scene.close()
+    out["psnr_db_half_fp8_vs_fp32"] = _psnr(imgs[("half_fp8", 0)], imgs[("fp32", 0)])
+    out["psnr_db_noise_floor_fp32_vs_fp32"] = _psnr(imgs[("fp32", 1)], imgs[("fp32", 0)])
+    out["half_fp8_over_fp32_fps"] = round(out["half_fp8"]["fps"] / out["fp32"]["fps"], 3)
+    out["max_attribute_error_vs_fp64"] = attribute_errors(G, W, w32, perm[::5])
+    out["attribute_sample"] = f"every 5th Gaussian of the scene ({len(perm[::5])})"
+    log(f"[precision] {out}")
+    return out
 
 
 def batched(G, W, args):
@@ -314,12 +648,39 @@ def main() -> int:
     ap.add_argument("--no-splat-forms", action="store_true",
                     help="skip re-timing the frames with each splat form forced")
     ap.add_argument("--batch-count", type=int, default=5, help="timed batches")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched-views comparison and print its report (for a profiler: its "
+                         "kernels per form)")
     ap.add_argument("--bvh-order", action="store_true",
                     help="measure only the load-time BVH order of the workload's scene (one JSON line)")
     ap.add_argument("--bvh-order-runs", type=int, default=3)
     ap.add_argument("--bvh-ref-sample", type=int, default=8_000_000,
                     help="Gaussians of the reference tinybvh CPU timing")
+    ap.add_argument("--profile-window", action="store_true",
+                    help="render the warm-up, sleep 1 s, render the timed frames, print the per-stage times and "
+                         "exit: the run to put under rocprofv3 (the sleep marks the timed frames in its trace)")
+    ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV", action="append",
+                    help="print per-kernel statistics of the timed frames of --profile-window traces and exit: "
+                         "the first gives the times, each further one pairs with a --counters file")
+    ap.add_argument("--counters", metavar="COUNTER_CSV", action="append", default=[],
+                    help="with --summarize-trace: the counter CSV of the same rocprofv3 run as the trace it follows")
+    ap.add_argument("--traffic-out", metavar="JSON",
+                    help="with --summarize-trace: write the calibrated HBM bytes per stage launch here")
+    ap.add_argument("--precision-frames", type=int, default=256,
+                    help="frames accumulated per storage form in the precision comparison (0 = skip)")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the full-size parity check against the oracle (the CPU baseline then times the "
+                         "oracle on a sample of the scene)")
     args = ap.parse_args()
+    if args.summarize_trace:
+        if len(args.counters) != len(args.summarize_trace) - 1:
+            print("--summarize-trace: one trace for the times, then one per --counters file", file=sys.stderr)
+            return 2
+        print(summarize_trace(args.summarize_trace, args.counters, traffic_out=args.traffic_out,
+                              window_name=f"{args.config} positions "
+                                          f"{max(0, args.first_frame) if args.first_frame >= 0 else args.warmup}.."
+                                          f"{(max(0, args.first_frame) if args.first_frame >= 0 else args.warmup) + (args.steps or 0) - 1}"))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
               "Run with --gpus 1.", file=sys.stderr)
@@ -331,6 +692,9 @@ def main() -> int:
         return 1
     if args.bvh_order:
         print(json.dumps(bvh_order_leg(G, W, args)), flush=True)
+        return 0
+    if args.batch_only:
+        print(json.dumps({"batched": batched(G, W, args)}), flush=True)
         return 0
     w = W.WORKLOADS[args.config]
     if args.format != "workload":
@@ -360,6 +724,8 @@ def main() -> int:
         f"{w.path_frames})")
     run(loop, range(F0, F0 + Wm))
     stream.synchronize()
+    if args.profile_window:
+        time.sleep(1.0)  # an idle gap that separates the timed frames in a profiler's trace
     loop.pass_.set_timing(K + 1)
     e0, e1 = G.Event(), G.Event()
     stream.synchronize()
@@ -373,14 +739,34 @@ def main() -> int:
     timing = loop.pass_.timing_summary()
     loop.close()
     log(f"[bench] timed {K} frames: {gpu_ms:.2f} ms GPU, {wall * 1e3:.2f} ms wall")
+    if args.profile_window:
+        # the counter calibration copy (1 GiB each way), inside the window
+        buf_a = G.DeviceArray((COPY_CAL_BYTES,), np.uint8, zero=True)
+        buf_b = G.DeviceArray((COPY_CAL_BYTES,), np.uint8)
+        G.check(G.lib.gps_selftest_copy(buf_b.ptr, buf_a.ptr, COPY_CAL_BYTES, stream.handle), "copy")
+        stream.synchronize()
+        st_ = timing["stages"]
+        print(json.dumps({"profile_window": f"{args.config} positions {F0 + Wm}..{F0 + Wm + K - 1}",
+                          "ms_per_frame": round(max(gpu_ms, wall * 1e3) / K, 4),
+                          "stages_us_per_launch": {k: round(v["ms"] * 1e3 / v["launches"], 1)
+                                                   for k, v in st_.items() if v["launches"]},
+                          "launches": {k: v["launches"] for k, v in st_.items() if v["launches"]}}), flush=True)
+        return 0
     # Work counters: replay the same deterministic frame sequence on a fresh
     # pass (same initial state), reading the counters after every frame.
     loop = W.FrameLoop(w, scene, stream)
-    loop.pass_.set_counters(True)
+    loop.pass_.set_counters(True, count_rejects=True)
     per_frame = []
+    rejects = [dict.fromkeys(G.REJECT_REASONS, 0) for _ in range(2)]
     run(loop, range(F0, F0 + Wm))
-    run(loop, range(F0 + Wm, F0 + Wm + K), per_frame)
+    for i in range(F0 + Wm, F0 + Wm + K):
+        loop.step(W.frame(w, i % w.path_frames, i + 1))
+        per_frame.append(loop.pass_.stats(loop.stream))
+        for acc, r in zip(rejects, loop.pass_.rejects(loop.stream)):
+            for key, v in r.items():
+                acc[key] += v
     loop.close()
+    reject_report = [{key: round(v / K) for key, v in acc.items()} for acc in rejects]
     # The same frames with each splat form forced (binned vs point-parallel
     # with one global atomic per sample): the chosen and the plain form.
     splat_forms = {}
@@ -413,13 +799,20 @@ def main() -> int:
     dom_launches = max(1, stages[dom]["launches"])
     dom_avg_ms = stages[dom]["ms"] / dom_launches
     achieved = dom_bytes / dom_launches / (dom_avg_ms / 1e3) / 1e9
+    # HBM traffic of the dominant kernel, from rocprofv3 counters of this same
+    # window (bench.py --profile-window under rocprofv3; bench.py
+    # --summarize-trace ... --traffic-out writes the file); null for a window
+    # that was not profiled.
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
+    window = f"{args.config} positions {F0 + Wm}..{F0 + Wm + K - 1}"
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.config}_positions_{F0 + Wm}_{F0 + Wm + K - 1}.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get(dom)
+            tj = json.load(fh)
+        if tj.get("_meta", {}).get("window") == window:
+            traffic = tj.get(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "window": window,
                 "algorithmic_bytes_per_launch": round(dom_bytes / dom_launches), "avg_launch_ms": dom_avg_ms}
     if dom == "splat" and atomics:
         # The splat's own bound: memory-side 64-bit atomics at random
@@ -432,12 +825,35 @@ def main() -> int:
     stage_report = {k: {"ms_per_frame": round(v["ms"] / K, 4), "launches": v["launches"],
                         "GBps": round(sum(b[k] for b in bytes_frames) / max(v["ms"], 1e-9) / 1e6, 1)}
                     for k, v in stages.items() if v["launches"]}
+    threads = min(16, os.cpu_count() or 1)
+    parity = None
+    first_timed = F0 + Wm
+    if not args.no_parity:
+        in_order = tuple(range(min(2, w.path_frames)))
+        carried = tuple(sorted({first_timed, w.path_frames // 2, w.path_frames - 1} - set(in_order))) \
+            if w.path_frames > 2 else ()
+        if perm is None:
+            perm = W.order(w)
+        log(f"[parity] {args.config} at full size: positions {in_order} in order, {carried} with carried state, "
+            f"oracle on {threads} threads")
+        parity = parity_leg(G, W, w, scene, perm, threads, in_order, carried)
     cpu = None
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+    timed_frames = [r for r in (parity or {}).get("frames", []) if r["position"] == first_timed]
+    if timed_frames and not args.no_cpu_baseline:
+        r = timed_frames[0]
+        cpu = {"value": round(r["samples"] / r["oracle_s"] / 1e9, 5), "unit": "Gpoints/s", "cores": threads,
+               "kind": "port",
+               "sample": f"one frame of the timed window (camera position {first_timed}) on the full {w.n}-Gaussian "
+                         f"scene, {w.width}x{w.height} ss{w.ss} spp{w.spp}, frame state carried from the GPU: "
+                         f"{r['samples']} point draws in {r['oracle_s']:.2f} s (render pass only, no resolve)",
+               "frame_s": r["oracle_s"]}
+    elif not args.no_cpu_baseline:
         sample_n = args.cpu_sample or min(w.n, 2_000_000)
         log(f"[bench] cpu baseline on {sample_n} Gaussians, {threads} threads")
         cpu = cpu_baseline(G, W, w, sample_n, threads, perm)
+    precision = None
+    if args.precision_frames > 0:
+        precision = precision_leg(G, W, args)
     batch_report = None
     if args.batch_views > 0:
         batch_report = batched(G, W, args)
@@ -460,9 +876,12 @@ def main() -> int:
         "two_pass_frames": sum(1 for p in passes if p == 2),
         "valid_gaussians_per_frame": round(sum(sum(st["valid_gaussians"][:st["passes"]]) for st in per_frame) / K),
         "culled_blocks_per_frame": round(sum(sum(st["culled_blocks"][:st["passes"]]) for st in per_frame) / K),
+        "preprocess_rejects_per_frame": {"first_pass": reject_report[0], "second_pass": reject_report[1]},
         "splat_forms": splat_forms,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
+        "precision": precision,
         "batched": batch_report,
         "serving": serve,
         "stages": stage_report,

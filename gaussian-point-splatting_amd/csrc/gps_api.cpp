@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -83,6 +84,7 @@ struct gps_pass {
     // binned splat
     int32_t splat_mode = GPS_SPLAT_AUTO;
     int32_t count_atomics = 0;
+    int32_t count_rejects = 0;
     uint32_t cus = 256;
     uint2* erect = nullptr;
     uint32_t* ew = nullptr;
@@ -546,9 +548,10 @@ gps_status gps_pass_set_splat_mode(gps_pass* p, int32_t mode) {
     return GPS_OK;
 }
 
-gps_status gps_pass_set_counters(gps_pass* p, int32_t count_atomics) {
-    if (!p) return GPS_ERR_INVALID_ARG;
-    p->count_atomics = count_atomics ? 1 : 0;
+gps_status gps_pass_set_counters(gps_pass* p, int32_t flags) {
+    if (!p || (flags & ~(GPS_COUNT_ATOMICS | GPS_COUNT_REJECTS))) return GPS_ERR_INVALID_ARG;
+    p->count_atomics = (flags & GPS_COUNT_ATOMICS) ? 1 : 0;
+    p->count_rejects = (flags & GPS_COUNT_REJECTS) ? 1 : 0;
     return GPS_OK;
 }
 
@@ -714,7 +717,8 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
     stage_mark(p, 0, 0, 0, st);
     // The clear also resets the per-frame counters (pass results, culled
     // counts, atomics) and keeps the Hi-Z ticket and the mask iteration.
-    launch_clear(p->cur_fb, p->fb_words, bg, st, reinterpret_cast<uint32_t*>(p->ctr), 10, &p->ctr->atomics, 1);
+    launch_clear(p->cur_fb, p->fb_words, bg, st, reinterpret_cast<uint32_t*>(p->ctr), 10, &p->ctr->atomics,
+                 (uint32_t)((sizeof(FrameCounters) - offsetof(FrameCounters, atomics)) / 4));
     stage_mark(p, 0, 0, 1, st);
 
     Mat4 vpm;
@@ -741,6 +745,7 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         a.tile_cnt = p->tile_cnt;
         a.cull = p->cull;
         a.hiz = p->hiz;
+        a.rejects = p->count_rejects ? &p->ctr->rejects[0][0] : nullptr;
         stage_mark(p, 2, slot, 0, st);
         launch_preprocess(a, st);
         stage_mark(p, 2, slot, 1, st);
@@ -950,6 +955,7 @@ uint64_t gps_pass_debug_size(gps_pass* p, int32_t which) {
         case GPS_DBG_RECORDS: return 16ull * p->N;
         case GPS_DBG_HIZ: return p->hiz_size;
         case GPS_DBG_CULL_MASK: return p->cull.mask_words;
+        case GPS_DBG_REJECTS: return 18;
     }
     return 0;
 }
@@ -979,6 +985,7 @@ gps_status gps_pass_debug_copy(gps_pass* p, int32_t which, void* dst, uint64_t c
         case GPS_DBG_RECORDS: src = p->recs; break;
         case GPS_DBG_HIZ: src = p->hiz.mip; break;
         case GPS_DBG_CULL_MASK: src = p->cull_mask; break;
+        case GPS_DBG_REJECTS: src = &p->ctr->rejects[0][0]; break;
         case GPS_DBG_INDICES: {
             if (p->dbg_indices_cap < count) {
                 dfree(p->dbg_indices);

@@ -8,6 +8,9 @@
 // 12-56 the 45 fp8 codes (coefficient-major, rgb interleaved), 57-63 zero.
 #include <cmath>
 #include <cstring>
+#include <thread>
+#include <vector>
+#include <algorithm>
 
 #include "../../include/gps.h"
 
@@ -56,17 +59,30 @@ extern "C" gps_status gps_encode_half_fp8(const gps_scene_desc* d, uint16_t* rot
     if (!d || !rotations || !scales || !opacities || !sh_dc || !sh_rest || !sh_scale) return GPS_ERR_INVALID_ARG;
     if (d->sh_degree < 0 || d->sh_degree > 3) return GPS_ERR_INVALID_ARG;
     const int ncoef = (d->sh_degree + 1) * (d->sh_degree + 1);
-    uint8_t rec[64];
-    for (uint64_t g = 0; g < d->num_gaussians; ++g) {
-        for (int k = 0; k < 4; ++k) rotations[4 * g + k] = gps_float_to_half_rne(d->rotations[4 * g + k]);
-        for (int k = 0; k < 3; ++k) scales[3 * g + k] = gps_float_to_half_rne(d->scales[3 * g + k]);
-        opacities[g] = gps_float_to_half_rne(d->opacities[g]);
-        gps_encode_sh_record(d->shs + (uint64_t)3 * ncoef * g, ncoef, rec);
-        std::memcpy(sh_dc + 3 * g, rec, 6);
-        std::memcpy(sh_scale + g, rec + 8, 4);
-        std::memset(sh_rest + 45 * g, 0, 45);
-        std::memcpy(sh_rest + 45 * g, rec + 12, (size_t)3 * (ncoef - 1));
-    }
+    // Each Gaussian encodes independently: ranges on host threads.
+    auto encode_range = [&](uint64_t g0, uint64_t g1) {
+        uint8_t rec[64];
+        for (uint64_t g = g0; g < g1; ++g) {
+            for (int k = 0; k < 4; ++k) rotations[4 * g + k] = gps_float_to_half_rne(d->rotations[4 * g + k]);
+            for (int k = 0; k < 3; ++k) scales[3 * g + k] = gps_float_to_half_rne(d->scales[3 * g + k]);
+            opacities[g] = gps_float_to_half_rne(d->opacities[g]);
+            gps_encode_sh_record(d->shs + (uint64_t)3 * ncoef * g, ncoef, rec);
+            std::memcpy(sh_dc + 3 * g, rec, 6);
+            std::memcpy(sh_scale + g, rec + 8, 4);
+            std::memset(sh_rest + 45 * g, 0, 45);
+            std::memcpy(sh_rest + 45 * g, rec + 12, (size_t)3 * (ncoef - 1));
+        }
+    };
+    const uint64_t n = d->num_gaussians;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt == 0 ? 1 : (nt > 32 ? 32 : nt);
+    if (n < 65536) nt = 1;
+    const uint64_t chunk = (n + nt - 1) / nt;
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < nt && (uint64_t)t * chunk < n; ++t)
+        ts.emplace_back(encode_range, (uint64_t)t * chunk, std::min<uint64_t>(n, (uint64_t)(t + 1) * chunk));
+    encode_range(0, std::min<uint64_t>(n, chunk));
+    for (auto& t : ts) t.join();
     return GPS_OK;
 }
 

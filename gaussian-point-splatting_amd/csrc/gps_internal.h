@@ -103,7 +103,10 @@ struct FrameCounters {
     uint32_t iteration;   // valid-mask ping-pong index (flips when a pass yields points)
     uint32_t atomics;     // 64-bit atomicMin issued by the splat kernel
     uint32_t pad[3];
+    uint32_t rejects[2][9];  // per pass: preprocess reject reasons (diagnostic; see RejectReason)
+    uint32_t pad2[2];
 };
+static_assert(sizeof(FrameCounters) == 144, "counter words zeroed by the clear kernel");
 
 // Region geometry of the binned splat for one framebuffer.
 struct BinGeom {

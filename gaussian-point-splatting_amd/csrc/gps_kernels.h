@@ -17,6 +17,7 @@ struct PreprocessArgs {
     uint32_t* tile_cnt;
     CullView cull;
     HiZView hiz;
+    uint32_t* rejects = nullptr;  // 2 x 9 reject-reason counters (diagnostic), or null
 };
 
 struct DistributeArgs {

@@ -510,11 +510,41 @@ __device__ __forceinline__ void preprocess_epilogue(uint32_t n, uint32_t g, uint
     }
 }
 
-template <int FORMAT, int DEG>
+// Reject-reason histogram of a pass (diagnostic, STATS = true): why each
+// Gaussian of an unculled block did not become valid, in the cull order of
+// gaussian_point_splatting.cu:188-263.  One counter set per pass.
+enum RejectReason : uint32_t {
+    REJ_TESTED = 0,     // Gaussians of unculled blocks (every lane below counts here too)
+    REJ_PREV_VALID = 1, // second pass: valid in the first pass's mask
+    REJ_OPACITY = 2,    // opacity <= 0
+    REJ_PROJECT = 3,    // near plane, NDC z outside [-1, 1], det <= 0
+    REJ_BOUNDS = 4,     // footprint outside the screen
+    REJ_SMALL = 5,      // small-eigenvalue cull
+    REJ_HIZ = 6,        // occluded by the Hi-Z chain
+    REJ_ZERO = 7,       // point count 0
+    REJ_VALID = 8,      // valid (count > 0)
+    REJ_COUNT = 9
+};
+// Adds one per lane whose `reason` is r to counter r (r = REJ_TESTED adds every
+// lane with a reason); one LDS atomic per wave and reason, one global atomic
+// per workgroup and reason.
+__device__ __forceinline__ void count_reasons(uint32_t* s_rej, uint32_t reason, bool tested) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t nt = (uint32_t)__popcll(__ballot(tested));
+    if (lane == 0 && nt) atomicAdd(s_rej + REJ_TESTED, nt);
+#pragma unroll
+    for (uint32_t r = 1; r < REJ_COUNT; ++r) {
+        const uint32_t n = (uint32_t)__popcll(__ballot(reason == r));
+        if (lane == 0 && n) atomicAdd(s_rej + r, n);
+    }
+}
+
+template <int FORMAT, int DEG, bool STATS>
 __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
     const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
-    uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h) {
+    uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h,
+    uint32_t* __restrict__ rejects) {
     __shared__ PreItem s_item[GPS_PRE_THREADS];
     // list 2 reuses list 1's storage (read into registers before a barrier):
     // 16 KB of LDS per workgroup, eight workgroups per CU
@@ -536,34 +566,52 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
         if (tid < GPS_PRE_THREADS / 32 && g0 + 32u * (uint32_t)tid < s.N) mask_cur[(g0 >> 5) + (uint32_t)tid] = 0u;
         return;
     }
+    __shared__ uint32_t s_rej[STATS ? REJ_COUNT : 1];
     if (tid == 0) {
         s_n1 = 0u;
         s_n2 = 0u;
     }
+    if (STATS && tid < REJ_COUNT) s_rej[tid] = 0u;
     s_count[tid] = 0u;
     __syncthreads();
     // Phase A.  Gaussians of culled blocks leave their weight slot untouched:
     // the distributor reads the same cull mask and counts them as zero.
-    if (g < s.N && !(f.hier && lane_block_culled(cv, wb, (uint32_t)tid))) {
+    uint32_t reason = REJ_TESTED;  // STATS: this lane's reject reason (REJ_TESTED = none yet)
+    const bool tested = g < s.N && !(f.hier && lane_block_culled(cv, wb, (uint32_t)tid));
+    if (tested) {
         bool survives = false;
-        if (!(f.only_prev && mask_bit(mask_prev, g))) {
+        if (f.only_prev && mask_bit(mask_prev, g)) {
+            reason = REJ_PREV_VALID;
+        } else {
             const Geo ge = load_geo<FORMAT>(s, g);
             const float op = glm_min(ge.op, 1.0f);
             Projected pr;
-            if (op > 0.0f &&
-                project_gaussian(f, ge.mx, ge.my, ge.mz, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx, ge.sy, ge.sz, pr) &&
-                !(pr.minx >= f.up_w || pr.miny >= f.up_h || pr.maxx < 0 || pr.maxy < 0)) {
+            bool projected = false;
+            if (!(op > 0.0f)) {
+                reason = REJ_OPACITY;
+            } else if (!(projected = project_gaussian(f, ge.mx, ge.my, ge.mz, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx,
+                                                      ge.sy, ge.sz, pr))) {
+                reason = REJ_PROJECT;
+            } else if (pr.minx >= f.up_w || pr.miny >= f.up_h || pr.maxx < 0 || pr.maxy < 0) {
+                reason = REJ_BOUNDS;
+            }
+            if (projected && reason == REJ_TESTED) {
                 survives = true;
                 if (f.cull_small) {
                     const float tr = pr.ca + pr.cc;
                     const float t2 = 0.5f * tr;
                     const float term2 = det_sqrt(det_fmax(0.1f, t2 * t2 - pr.det));
                     const float e1 = t2 + term2, e2 = t2 - term2;
-                    if (e1 < 0.02f || e2 < 0.02f) survives = false;
+                    if (e1 < 0.02f || e2 < 0.02f) {
+                        survives = false;
+                        reason = REJ_SMALL;
+                    }
                 }
                 if (survives && f.occl &&
-                    hiz_occluded(h, pr.pz, div_ss(f, pr.minx), div_ss(f, pr.miny), div_ss(f, pr.maxx), div_ss(f, pr.maxy)))
+                    hiz_occluded(h, pr.pz, div_ss(f, pr.minx), div_ss(f, pr.miny), div_ss(f, pr.maxx), div_ss(f, pr.maxy))) {
                     survives = false;
+                    reason = REJ_HIZ;
+                }
                 if (survives) {
                     const uint32_t k = atomicAdd(&s_n1, 1u);
                     PreItem& q = s_item[k];
@@ -586,12 +634,14 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
         }
         if (!survives) weights[g] = 0u;
     }
+    if (STATS) count_reasons(s_rej, reason, tested);
     __syncthreads();
     // Phase B
     const bool in1 = (uint32_t)tid < s_n1;
     PreItem q;
     if (in1) q = s_item[tid];
     __syncthreads();  // list 1 is in registers before list 2 overwrites it
+    uint32_t cnt_b = 0u;  // this lane's count (STATS)
     if (in1) {
         u32x2 seed = make_seed(q.g, f.frame_seed);
         const float s_det = det_sqrt(q.det);
@@ -621,8 +671,18 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
             s_count[q.g - g0] = cnt;
         }
         weights[q.g] = cnt;
+        cnt_b = cnt;
+    }
+    if (STATS) {
+        const uint32_t nz = (uint32_t)__popcll(__ballot(in1 && cnt_b == 0u));
+        const uint32_t nv = (uint32_t)__popcll(__ballot(in1 && cnt_b != 0u));
+        if ((tid & 63) == 0) {
+            if (nz) atomicAdd(s_rej + REJ_ZERO, nz);
+            if (nv) atomicAdd(s_rej + REJ_VALID, nv);
+        }
     }
     __syncthreads();
+    if (STATS && tid < REJ_COUNT && s_rej[tid]) atomicAdd(rejects + (f.only_prev ? REJ_COUNT : 0) + tid, s_rej[tid]);
     // Phase C
     if ((uint32_t)tid < s_n2) {
         const ColourItem q = s_col[tid];
@@ -2398,10 +2458,14 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
 template <int FORMAT, int DEG>
 static void launch_pre(const PreprocessArgs& a, hipStream_t st) {
     const uint32_t blocks = (a.scene.N + GPS_PRE_THREADS - 1) / GPS_PRE_THREADS;
-    if (a.f.hier || a.f.occl)  // culled frames: work lists keep the later phases on full waves
-        hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st, a.scene,
-                           a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt, a.cull,
-                           a.hiz);
+    if (a.rejects)  // reject-reason counters: the work-list kernel (same outputs) with counting
+        hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG, true>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
+                           a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
+                           a.cull, a.hiz, a.rejects);
+    else if (a.f.hier || a.f.occl)  // culled frames: work lists keep the later phases on full waves
+        hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG, false>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
+                           a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
+                           a.cull, a.hiz, nullptr);
     else
         hipLaunchKernelGGL((preprocess_flat_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
                            a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,

@@ -53,6 +53,11 @@ DBG_FRAMEBUFFER = 6
 DBG_RECORDS = 7
 DBG_HIZ = 8
 DBG_CULL_MASK = 9
+DBG_REJECTS = 10
+COUNT_ATOMICS = 1
+COUNT_REJECTS = 2
+REJECT_REASONS = ("tested", "prev_valid", "opacity", "projection", "off_screen", "small", "hiz", "zero_count",
+                  "valid")
 _DBG_DTYPE = {DBG_FRAMEBUFFER: np.uint64, DBG_HIZ: np.float32}
 
 ACCUM_PROGRESSIVE = 0
@@ -575,8 +580,15 @@ class Pass:
         """SPLAT_AUTO, SPLAT_POINT_PARALLEL or SPLAT_BINNED."""
         check(_lib.gps_pass_set_splat_mode(self.handle, mode), "splat mode")
 
-    def set_counters(self, count_atomics: bool) -> None:
-        check(_lib.gps_pass_set_counters(self.handle, int(count_atomics)), "counters")
+    def set_counters(self, count_atomics: bool, count_rejects: bool = False) -> None:
+        flags = (COUNT_ATOMICS if count_atomics else 0) | (COUNT_REJECTS if count_rejects else 0)
+        check(_lib.gps_pass_set_counters(self.handle, flags), "counters")
+
+    def rejects(self, stream: Optional[Stream] = None) -> list:
+        """Per render pass of the last frame, {reason: Gaussians} (counters on
+        with set_counters(..., count_rejects=True))."""
+        r = self.debug(DBG_REJECTS, stream).reshape(2, len(REJECT_REASONS))
+        return [dict(zip(REJECT_REASONS, map(int, row))) for row in r]
 
     def set_timing(self, capacity: int) -> None:
         check(_lib.gps_pass_set_timing(self.handle, capacity), "set timing")

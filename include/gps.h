@@ -209,9 +209,16 @@ void gps_pass_destroy(gps_pass* pass);
 #define GPS_SPLAT_POINT_PARALLEL 1
 #define GPS_SPLAT_BINNED 2
 gps_status gps_pass_set_splat_mode(gps_pass* pass, int32_t mode);
-/* Count the splat's global atomics into gps_stats.atomics (off by default:
- * the count costs one atomic per workgroup on a single word). */
-gps_status gps_pass_set_counters(gps_pass* pass, int32_t count_atomics);
+/* Diagnostic counters, off by default.  GPS_COUNT_ATOMICS: the splat's
+ * global atomics into gps_stats.atomics (one atomic per workgroup on a single
+ * word).  GPS_COUNT_REJECTS: per render pass, why each Gaussian of an
+ * unculled block did not become valid, read with GPS_DBG_REJECTS (the
+ * preprocess then always runs its work-list form, same outputs).  No
+ * reference counterpart (the reference's statistics are the totals of
+ * gaussian_point_splatting.cu:675-685). */
+#define GPS_COUNT_ATOMICS 1
+#define GPS_COUNT_REJECTS 2
+gps_status gps_pass_set_counters(gps_pass* pass, int32_t flags);
 /* Device memory held by the pass (scene not included), and what
  * gps_pass_create would allocate for these settings. */
 uint64_t gps_pass_device_bytes(const gps_pass* pass);
@@ -241,7 +248,12 @@ typedef enum gps_debug_buffer {
     GPS_DBG_FRAMEBUFFER = 6,    /* uint64[F] framebuffer words of the current frame */
     GPS_DBG_RECORDS = 7,        /* 16 x uint32 per Gaussian: cached splat parameters */
     GPS_DBG_HIZ = 8,            /* float[mip chain] Hi-Z pyramid, coarsest level first */
-    GPS_DBG_CULL_MASK = 9       /* uint32[ceil(blocks/32)] culled-block mask of the last pass */
+    GPS_DBG_CULL_MASK = 9,      /* uint32[ceil(blocks/32)] culled-block mask of the last pass */
+    GPS_DBG_REJECTS = 10        /* uint32[2][9] per pass (GPS_COUNT_REJECTS): Gaussians of unculled
+                                   blocks tested; valid in the first pass (second pass only);
+                                   opacity <= 0; projection rejected (near plane, NDC z, det <= 0);
+                                   footprint off screen; small-eigenvalue cull; Hi-Z occluded;
+                                   point count 0; valid */
 } gps_debug_buffer;
 uint64_t gps_pass_debug_size(gps_pass* pass, int32_t which);
 gps_status gps_pass_debug_copy(gps_pass* pass, int32_t which, void* host_dst, uint64_t count, void* stream);

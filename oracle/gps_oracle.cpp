@@ -363,7 +363,13 @@ struct orc_pass {
         std::vector<uint32_t>& mcur = maskCurrent();
         const std::vector<uint32_t>& mprev = maskPrevious();
         const uint64_t nwarps = (N + 31) / 32;
+        const int slot = onlyPrev ? 1 : 0;
+        for (auto& r : rejects[slot]) r = 0;
         parallel_for(nwarps, nthreads, 1, [&](uint64_t wb, uint64_t we) {
+            // why each Gaussian of an unculled block is not valid (counted
+            // per range, then added): tested, valid in the first pass,
+            // opacity, projection, off screen, small, Hi-Z, count 0, valid
+            uint64_t rj[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (uint64_t w = wb; w < we; ++w) {
                 uint32_t ballot = 0;
                 for (uint32_t lane = 0; lane < 32; ++lane) {
@@ -372,48 +378,53 @@ struct orc_pass {
                     const uint32_t idx = (uint32_t)idx64;
                     uint2_ seed = makeSeed(idx, (uint32_t)frame_seed);
                     bool splats = false;
+                    int reason = 0;
                     do {
                         if (hier && is_culled(idx)) break;
-                        if (onlyPrev && is_valid(idx, mprev)) break;
+                        rj[0]++;
+                        if (onlyPrev && is_valid(idx, mprev)) { reason = 1; break; }
                         const float op = opacity(idx);
-                        if (op <= 0.0f) break;
+                        if (op <= 0.0f) { reason = 2; break; }
                         vec4<float> proj_mean;
                         vec2<float> pm;
                         vec3<float> cov2d;
                         ivec2_ pmin, pmax;
                         float det, view_z;
-                        if (!loadAndTransform(idx, f, proj_mean, pm, cov2d, pmin, pmax, det, view_z)) break;
-                        if (pmin.x >= f.up_w || pmin.y >= f.up_h || pmax.x < 0 || pmax.y < 0) break;
+                        if (!loadAndTransform(idx, f, proj_mean, pm, cov2d, pmin, pmax, det, view_z)) { reason = 3; break; }
+                        if (pmin.x >= f.up_w || pmin.y >= f.up_h || pmax.x < 0 || pmax.y < 0) { reason = 4; break; }
                         if (p->cull_small_gaussians) {
                             const float tr = cov2d.x + cov2d.z;
                             const float t2 = 0.5f * tr;
                             const float term2 = std::sqrt(gps::det_fmax(0.1f, t2 * t2 - det));
                             const float e1 = t2 + term2, e2 = t2 - term2;
-                            if (e1 < 0.02f || e2 < 0.02f) break;
+                            if (e1 < 0.02f || e2 < 0.02f) { reason = 5; break; }
                         }
                         const float TWO_PI = 2.0f * 3.14159265358979323846f;
                         const float s_det = std::sqrt(det);
                         const float importance = unbiased ? TWO_PI * s_det * dilog(op) : op * TWO_PI * s_det;
                         if (occl && hiz.is_occluded(proj_mean.z, {pmin.x / f.ss, pmin.y / f.ss},
-                                                    {pmax.x / f.ss, pmax.y / f.ss}))
-                            break;
+                                                    {pmax.x / f.ss, pmax.y / f.ss})) { reason = 6; break; }
                         float K = (float)(f.ss * f.ss * 1);
                         if (!p->reduce_point_count) K = 1.0f;
                         uint32_t n = unbiased ? poisson(seed, importance)
                                               : (uint32_t)stochastic_round(importance, pcg2d(seed).x);
                         if (K > 1) n = (uint32_t)stochastic_round(static_cast<float>(n) / K, pcg2d(seed).x);
-                        if (n <= 0) break;
+                        if (n <= 0) { reason = 7; break; }
                         n = std::min(n, gps::det_f2u_rz((float)(f.up_w * f.up_h) / (2 * K)));
                         colors[idx] = sh_color(mean(idx), f.campos, &sh48[48 * idx]);
                         weights[idx] = n;
                         splats = true;
+                        reason = 8;
                     } while (false);
+                    if (reason) rj[reason]++;
                     if (splats) ballot |= 1u << lane;
                 }
                 mcur[w] = ballot;
             }
+            for (int k = 0; k < 9; ++k) rejects[slot][k] += rj[k];
         });
     }
+    std::atomic<uint64_t> rejects[2][9];
 
     // workload_distributor.cu:62-145
     void build() {
@@ -563,6 +574,8 @@ struct orc_pass {
         stats.num_gaussians = N;
         biased_events.zero_colour = 0;
         biased_events.wrapped = 0;
+        for (auto& ps : rejects)
+            for (auto& r : ps) r = 0;
         int K = f.ss * f.ss;
         if (!p->reduce_point_count) K = 1;
         stats.K = K;
@@ -644,18 +657,21 @@ struct orc_pass {
             gpb = (uint32_t)gmax(4.0f, std::floor((float)N / (float)1e6));
         const uint64_t nb = (N + gpb - 1) / gpb;
         nodes.resize(nb);
-        for (uint64_t b = 0; b < nb; ++b) {
-            vec3<float> bmin = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
-            vec3<float> bmax = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
-            const uint64_t e = std::min<uint64_t>(N, (b + 1) * gpb);
-            for (uint64_t i = b * gpb; i < e; ++i) {
-                const mat3<float> cov = scale_rot_to_cov(scale(i), quat(i));
-                const vec3<float> ext = 3.0f * vec3<float>{std::sqrt(cov[0].x), std::sqrt(cov[1].y), std::sqrt(cov[2].z)};
-                bmin = vmin(bmin, mean(i) - ext);
-                bmax = vmax(bmax, mean(i) + ext);
+        parallel_for(nb, nthreads, 1, [&](uint64_t b0, uint64_t b1) {
+            for (uint64_t b = b0; b < b1; ++b) {
+                vec3<float> bmin = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+                vec3<float> bmax = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+                const uint64_t e = std::min<uint64_t>(N, (b + 1) * gpb);
+                for (uint64_t i = b * gpb; i < e; ++i) {
+                    const mat3<float> cov = scale_rot_to_cov(scale(i), quat(i));
+                    const vec3<float> ext =
+                        3.0f * vec3<float>{std::sqrt(cov[0].x), std::sqrt(cov[1].y), std::sqrt(cov[2].z)};
+                    bmin = vmin(bmin, mean(i) - ext);
+                    bmax = vmax(bmax, mean(i) + ext);
+                }
+                nodes[b] = {bmin, bmax, (uint32_t)(b * gpb), (uint32_t)(e - b * gpb)};
             }
-            nodes[b] = {bmin, bmax, (uint32_t)(b * gpb), (uint32_t)(e - b * gpb)};
-        }
+        });
         culled.assign((nb + 31) / 32, 0);
     }
 };
@@ -690,14 +706,16 @@ orc_pass* orc_pass_create_half_fp8(const gps_render_settings* s, const orc_half_
     p->opac.resize(N);
     p->sh48.assign(48 * N, 0.0f);
     const int nrest = (d->sh_degree + 1) * (d->sh_degree + 1) - 1;
-    for (uint64_t g = 0; g < N; ++g) {
-        for (int k = 0; k < 4; ++k) p->rots[4 * g + k] = half_to_float(d->rotations[4 * g + k]);
-        for (int k = 0; k < 3; ++k) p->scales[3 * g + k] = half_to_float(d->scales[3 * g + k]);
-        p->opac[g] = half_to_float(d->opacities[g]);
-        for (int k = 0; k < 3; ++k) p->sh48[48 * g + k] = half_to_float(d->sh_dc[3 * g + k]);
-        for (int k = 0; k < 3 * nrest; ++k)
-            p->sh48[48 * g + 3 + k] = fp8e4m3_to_float(d->sh_rest[45 * g + k]) * d->sh_scale[g];
-    }
+    parallel_for(N, std::max(1, nthreads), 1, [&](uint64_t g0, uint64_t g1) {
+        for (uint64_t g = g0; g < g1; ++g) {
+            for (int k = 0; k < 4; ++k) p->rots[4 * g + k] = half_to_float(d->rotations[4 * g + k]);
+            for (int k = 0; k < 3; ++k) p->scales[3 * g + k] = half_to_float(d->scales[3 * g + k]);
+            p->opac[g] = half_to_float(d->opacities[g]);
+            for (int k = 0; k < 3; ++k) p->sh48[48 * g + k] = half_to_float(d->sh_dc[3 * g + k]);
+            for (int k = 0; k < 3 * nrest; ++k)
+                p->sh48[48 * g + 3 + k] = fp8e4m3_to_float(d->sh_rest[45 * g + k]) * d->sh_scale[g];
+        }
+    });
     p->init_common(s, nthreads);
     return p;
 }
@@ -766,6 +784,11 @@ int32_t orc_pass_stats(orc_pass* p, gps_stats* out) {
 void orc_pass_biased_events(orc_pass* p, uint64_t out[2]) {
     out[0] = p->biased_events.zero_colour.load();
     out[1] = p->biased_events.wrapped.load();
+}
+
+void orc_pass_rejects(orc_pass* p, uint64_t out[18]) {
+    for (int s = 0; s < 2; ++s)
+        for (int k = 0; k < 9; ++k) out[9 * s + k] = p->rejects[s][k].load();
 }
 
 uint64_t orc_biased_sample(uint64_t* word, uint64_t value, int32_t precheck, int32_t add_if_equal) {
@@ -842,11 +865,44 @@ int32_t orc_pass_debug_copy(orc_pass* p, int32_t which, void* dst, uint64_t coun
     return GPS_ERR_INVALID_ARG;
 }
 
+int32_t orc_pass_set_state(orc_pass* p, const float* hiz, uint64_t hiz_count, const uint32_t* mask_other,
+                           uint64_t mask_words) {
+    if (!p || !hiz || !mask_other) return GPS_ERR_INVALID_ARG;
+    if (hiz_count != p->hiz.mip.size() || mask_words != p->maskPrevious().size()) return GPS_ERR_INVALID_ARG;
+    std::memcpy(p->hiz.mip.data(), hiz, hiz_count * 4);
+    std::memcpy(p->maskPrevious().data(), mask_other, mask_words * 4);
+    return GPS_OK;
+}
+
 int32_t orc_pass_cull(orc_pass* p, const gps_frame_params* fp) {
     if (!p || !fp || p->nodes.empty()) return GPS_ERR_INVALID_ARG;
     Frame f;
     p->make_frame(fp, f);
     p->compute_occlusion_mask(f, fp->enable_occlusion_culling != 0);
+    return GPS_OK;
+}
+
+int32_t orc_project_ids(orc_pass* p, const gps_frame_params* fp, const uint32_t* ids, uint64_t n, double* out) {
+    Frame f;
+    p->make_frame(fp, f);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t g = ids[i];
+        double* o = out + 16 * i;
+        for (int k = 0; k < 16; ++k) o[k] = 0.0;
+        if (g >= p->N) continue;
+        vec4<float> pmn;
+        vec2<float> pm;
+        vec3<float> cov;
+        ivec2_ a, bb;
+        float det, vz;
+        if (!p->loadAndTransform(g, f, pmn, pm, cov, a, bb, det, vz)) continue;
+        const vec3<float> c = p->sh_color(p->mean(g), f.campos, &p->sh48[48 * g]);
+        const float op = p->opacity(g);
+        const float imp = 2.0f * 3.14159265358979323846f * std::sqrt(det) * dilog(op);
+        const double vals[16] = {1, vz, pm.x, pm.y, cov.x, cov.y, cov.z, c.x, c.y, c.z, imp, pmn.z,
+                                 (double)a.x, (double)a.y, (double)bb.x, (double)bb.y};
+        for (int k = 0; k < 16; ++k) o[k] = vals[k];
+    }
     return GPS_OK;
 }
 

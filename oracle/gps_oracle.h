@@ -65,6 +65,11 @@ int32_t orc_pass_debug_copy(orc_pass* pass, int32_t which, void* dst, uint64_t c
  * equal-depth adds that carried out of the 36 colour bits.  When both are 0
  * every framebuffer word is independent of the order of its samples. */
 void orc_pass_biased_events(orc_pass* pass, uint64_t out[2]);
+/* Per render pass of the last frame (second pass zero without occlusion
+ * culling), why each Gaussian of an unculled block did not become valid, in
+ * the order of GPS_DBG_REJECTS: tested, valid in the first pass, opacity,
+ * projection, off screen, small, Hi-Z, count 0, valid. */
+void orc_pass_rejects(orc_pass* pass, uint64_t out[18]);
 /* One biased sample on one word: precheck != 0 applies the splat's pre-check
  * and atomicAddColor(add_if_equal = true) (gaussian_point_splatting.cu:130-138);
  * precheck == 0 applies atomicAddColor alone with the given add_if_equal and
@@ -80,10 +85,21 @@ void orc_pass_stage_ms(orc_pass* pass, double out[8]);
  * max x/y (FP32 only).  ok = 1 if the
  * Gaussian passes projection + bounds; colour is evaluated for every such
  * Gaussian. */
+/* Carried frame state, for rendering frame f without rendering frames
+ * 0 .. f-1 first: the Hi-Z chain (GPS_DBG_HIZ layout, coarsest level first)
+ * and the valid mask that the next frame's first pass does not overwrite
+ * (its second pass reads it as the previous mask when the first pass yields
+ * no points; workload_distributor.cu:32-40,111-115).  Nothing else survives
+ * a frame: the framebuffer is cleared, the weights and the first pass's mask
+ * are rewritten. */
+int32_t orc_pass_set_state(orc_pass* pass, const float* hiz, uint64_t hiz_count, const uint32_t* mask_other,
+                           uint64_t mask_words);
 /* Block culling alone (compute_occlusion_mask) against the Hi-Z chain the
  * pass currently holds; the mask is then read with GPS_DBG_CULL_MASK. */
 int32_t orc_pass_cull(orc_pass* pass, const gps_frame_params* frame);
 int32_t orc_project(orc_pass* pass, const gps_frame_params* frame, int32_t use_fp64, double* out);
+/* orc_project's FP32 values for the Gaussians `ids` only (n x 16 doubles). */
+int32_t orc_project_ids(orc_pass* pass, const gps_frame_params* frame, const uint32_t* ids, uint64_t n, double* out);
 
 /* Resolve kernels (post_processing_kernels.cuh). */
 void orc_resolve_taa(const float* cur_color, const float* cur_depth, const float* prev_color,

@@ -69,9 +69,12 @@ _sig("orc_pass_debug_size", C.c_uint64, _vp, C.c_int32)
 _sig("orc_pass_debug_copy", C.c_int32, _vp, C.c_int32, _vp, C.c_uint64)
 _sig("orc_pass_stage_ms", None, _vp, C.POINTER(C.c_double))
 _sig("orc_pass_biased_events", None, _vp, _vp)
+_sig("orc_pass_rejects", None, _vp, _vp)
 _sig("orc_biased_sample", C.c_uint64, _vp, C.c_uint64, C.c_int32, C.c_int32)
 _sig("orc_pass_cull", C.c_int32, _vp, C.POINTER(FrameParams))
+_sig("orc_pass_set_state", C.c_int32, _vp, _vp, C.c_uint64, _vp, C.c_uint64)
 _sig("orc_project", C.c_int32, _vp, C.POINTER(FrameParams), C.c_int32, _vp)
+_sig("orc_project_ids", C.c_int32, _vp, C.POINTER(FrameParams), _vp, C.c_uint64, _vp)
 _sig("orc_resolve_taa", None, _vp, _vp, _vp, _vp, C.POINTER(C.c_float), C.c_float, C.c_int32, C.c_int32, C.c_int32,
      _vp, _vp)
 _sig("orc_resolve_progressive", None, _vp, _vp, C.c_int32, C.c_int32, C.c_int32, _vp)
@@ -236,6 +239,29 @@ class OraclePass:
         self.width = settings.width
         self.height = settings.height
 
+    @classmethod
+    def from_half_fp8(cls, settings: "gps_amd.RenderSettings", means: np.ndarray, enc: dict, sh_degree: int,
+                      threads: int = 1) -> "OraclePass":
+        """A pass over a scene already encoded in the compact format (the
+        arrays gps_amd.encode_half_fp8 returns, concatenated over chunks), so
+        a large scene need not be held in fp32 on the host."""
+        self = cls.__new__(cls)
+        self._s = settings.c()
+        self._keep = (means, enc)
+        self._enc = enc
+        h = HalfFp8Scene()
+        h.num_gaussians = means.shape[0]
+        h.sh_degree = sh_degree
+        h.means = _ptr(means)
+        for k in ("rotations", "scales", "opacities", "sh_dc", "sh_rest", "sh_scale"):
+            setattr(h, k, _ptr(enc[k]))
+        self.handle = _lib.orc_pass_create_half_fp8(C.byref(self._s), C.byref(h), threads)
+        if not self.handle:
+            raise RuntimeError("oracle pass create failed")
+        self.width = settings.width
+        self.height = settings.height
+        return self
+
     def set_threads(self, n: int) -> None:
         _lib.orc_pass_set_threads(self.handle, n)
 
@@ -259,6 +285,13 @@ class OraclePass:
         _lib.orc_pass_biased_events(self.handle, _ptr(out))
         return {"zero_colour": int(out[0]), "wrapped": int(out[1])}
 
+    def rejects(self) -> list:
+        """Per render pass of the last frame, {reason: Gaussians} in the
+        reasons of gps_amd.REJECT_REASONS."""
+        out = np.zeros(18, np.uint64)
+        _lib.orc_pass_rejects(self.handle, _ptr(out))
+        return [dict(zip(gps_amd.REJECT_REASONS, map(int, out[9 * s:9 * s + 9]))) for s in range(2)]
+
     def stage_ms(self) -> list:
         out = (C.c_double * 8)()
         _lib.orc_pass_stage_ms(self.handle, out)
@@ -272,6 +305,14 @@ class OraclePass:
             raise RuntimeError(f"oracle debug copy {which} failed: {st}")
         return out
 
+    def set_state(self, hiz: np.ndarray, mask_other: np.ndarray) -> None:
+        """Carried frame state (Hi-Z chain, the valid mask the next first pass
+        leaves in place), as read from another pass after the frame before."""
+        hiz = np.ascontiguousarray(hiz, np.float32)
+        mask_other = np.ascontiguousarray(mask_other, np.uint32)
+        if _lib.orc_pass_set_state(self.handle, _ptr(hiz), hiz.size, _ptr(mask_other), mask_other.size) != 0:
+            raise RuntimeError("oracle set_state: sizes do not match the pass")
+
     def cull(self, frame: FrameParams) -> np.ndarray:
         """Block-cull mask for `frame` against the Hi-Z chain held now."""
         if _lib.orc_pass_cull(self.handle, C.byref(frame)) != 0:
@@ -282,6 +323,13 @@ class OraclePass:
         n = self._keep.n
         out = np.zeros((n, 16), np.float64)
         _lib.orc_project(self.handle, C.byref(frame), int(fp64), _ptr(out))
+        return out
+
+    def project_ids(self, frame: FrameParams, ids: np.ndarray) -> np.ndarray:
+        """project(frame, fp64=False) rows of the Gaussians `ids` only."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.zeros((ids.size, 16), np.float64)
+        _lib.orc_project_ids(self.handle, C.byref(frame), _ptr(ids), ids.size, _ptr(out))
         return out
 
     def close(self) -> None:

@@ -97,3 +97,15 @@ def assert_same(name: str, a: np.ndarray, b: np.ndarray) -> None:
         bad = np.flatnonzero(~eq.reshape(-1))
         raise AssertionError(f"{name}: {bad.size} of {eq.size} differ; first at {bad[:8]}: "
                              f"{a.reshape(-1)[bad[:4]]} vs {b.reshape(-1)[bad[:4]]}")
+
+
+def carried_state(p):
+    """(Hi-Z chain, valid mask the next first pass leaves in place) of a GPU
+    or oracle pass after a frame: the state a frame carries into the next.
+    The ping-pong flips when a pass yields points, so the untouched mask is
+    the last pass's own after a flip and the one before it otherwise."""
+    st = p.stats()
+    passes = st["passes"]
+    flipped = passes > 0 and st["num_points"][passes - 1] > 0
+    other = p.debug(G.DBG_MASK_CURRENT) if flipped else p.debug(G.DBG_MASK_PREVIOUS)
+    return p.debug(G.DBG_HIZ), other
