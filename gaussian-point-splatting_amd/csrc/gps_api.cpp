@@ -866,6 +866,104 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
     return GPS_OK;
 }
 
+}  // extern "C"
+
+extern "C++" {
+namespace gps {
+// gps_pass_render of a pass that exists only for this call: created as
+// gps_pass_create would (fully_disable_hierarchical_culling = 1, point budget
+// max_points, the frame's size and spp), with every buffer carved out of the
+// device workspace `ws` of `ws_bytes` instead of allocated.  What the kernels
+// need at zero (pass counters, tile sums, the empty cull mask) is cleared by
+// a fill kernel on `st`; the rest is written before it is read.  The point-
+// parallel splat is used (it gives the binned form's framebuffer bit for bit).
+gps_status render_in_workspace(gps_scene* scene, const gps_frame_params* fp, int64_t max_points, float* d_color,
+                               float* d_depth, hipStream_t st, uint8_t* ws, uint64_t ws_bytes, int cus) {
+    if (!scene || !fp || !d_color || !d_depth || !ws || max_points < 0) return GPS_ERR_INVALID_ARG;
+    if (fp->enable_occlusion_culling || fp->enable_hierarchical_culling) {
+        g_last_error = "gps_frame_render_noalloc renders frames without occlusion or block culling";
+        return GPS_ERR_UNSUPPORTED;
+    }
+    const int spp = fp->samples_per_pixel > 0 ? fp->samples_per_pixel : 1;
+    const int ss = fp->supersampling_factor;
+    if (fp->width < 1 || fp->height < 1 || ss < 1 || !(fp->far_plane > fp->near_plane)) return GPS_ERR_INVALID_ARG;
+    if ((int64_t)fp->width * ss >= 65536 || (int64_t)fp->height * ss >= 65536) return GPS_ERR_UNSUPPORTED;
+    const uint64_t N = scene->view.N;
+    const uint64_t words = fb_words(fp->width * ss, fp->height * ss, spp);
+    if (N > GPS_NOALLOC_MAX_GAUSSIANS) {
+        g_last_error = "gps_frame_render_noalloc: more Gaussians than GPS_NOALLOC_MAX_GAUSSIANS";
+        return GPS_ERR_UNSUPPORTED;
+    }
+    if (words > GPS_NOALLOC_MAX_FB_WORDS) {
+        g_last_error = "gps_frame_render_noalloc: framebuffer larger than GPS_NOALLOC_MAX_FB_WORDS";
+        return GPS_ERR_UNSUPPORTED;
+    }
+    if ((uint64_t)max_points > GPS_NOALLOC_MAX_POINTS) {
+        g_last_error = "gps_frame_render_noalloc: point budget above GPS_NOALLOC_MAX_POINTS";
+        return GPS_ERR_UNSUPPORTED;
+    }
+    gps_pass p;
+    p.scene = scene;
+    p.settings.width = fp->width;
+    p.settings.height = fp->height;
+    p.settings.render_pass_count = spp;
+    p.settings.supersampling_factor = ss;
+    p.settings.max_number_of_points = max_points;
+    p.settings.fully_disable_hierarchical_culling = 1;
+    p.N = (uint32_t)N;
+    p.budget = (uint32_t)max_points;
+    p.ntiles = (uint32_t)((N + GPS_TILE - 1) / GPS_TILE);
+    const uint64_t npad = (uint64_t)p.ntiles * GPS_TILE;
+    const uint64_t nwords = (N + 31) / 32 + 2;
+    // Layout: the words cleared per call first (one fill launch), then the rest.
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        uint8_t* q = ws + off;
+        off += round_up(bytes, 256);
+        return q;
+    };
+    uint8_t* zero_begin = ws;
+    p.ctr = reinterpret_cast<FrameCounters*>(take(sizeof(FrameCounters)));
+    p.cull_mask = reinterpret_cast<uint32_t*>(take(2 * 4));
+    p.tile_sum = reinterpret_cast<uint32_t*>(take(4ull * p.ntiles));
+    p.tile_cnt = reinterpret_cast<uint32_t*>(take(4ull * p.ntiles));
+    const uint64_t zero_words = off / 4;
+    p.tile_off = reinterpret_cast<uint32_t*>(take(4ull * p.ntiles));
+    p.tile_voff = reinterpret_cast<uint32_t*>(take(4ull * p.ntiles));
+    p.weights = reinterpret_cast<uint32_t*>(take(4 * npad));
+    p.maskA = reinterpret_cast<uint32_t*>(take(4 * nwords));
+    p.maskB = reinterpret_cast<uint32_t*>(take(4 * nwords));
+    p.recs = reinterpret_cast<SplatRecord*>(take(sizeof(SplatRecord) * std::max<uint64_t>(N, 1)));
+    p.cidx = reinterpret_cast<uint32_t*>(take(4 * std::max<uint64_t>(N, 1)));
+    p.coff = reinterpret_cast<uint32_t*>(take(4 * std::max<uint64_t>(N, 1)));
+    p.chunk_start = reinterpret_cast<uint32_t*>(take(4 * ((uint64_t)p.budget / GPS_SPLAT_CHUNK + 2)));
+    p.fbA = p.fbB = reinterpret_cast<uint64_t*>(take(8 * words));
+    if (off > ws_bytes) {
+        g_last_error = "gps_frame_render_noalloc: workspace too small for this frame";
+        return GPS_ERR_UNSUPPORTED;
+    }
+    p.fb_cap = words;
+    p.fb_words = words;
+    p.fb_ss = ss;
+    p.splat_mode = GPS_SPLAT_POINT_PARALLEL;
+    p.hier_available = false;
+    const uint32_t gpb = (uint32_t)std::max(4.0f, std::floor((float)N / (float)1e6));
+    p.cull.gpb = gpb;
+    p.cull.node_count = (uint32_t)((N + gpb - 1) / gpb);
+    p.cull.mask_words = (p.cull.node_count + 31) / 32;
+    p.cull.nodes = nullptr;
+    p.cull.mask = p.cull_mask;
+    p.xcds = (cus % 8 == 0) ? 8 : 1;
+    p.splat_grid = (uint32_t)cus * GPS_SPLAT_WGS_PER_CU;
+    p.cus = (uint32_t)cus;
+    launch_zero_u32(reinterpret_cast<uint32_t*>(zero_begin), (uint32_t)zero_words, st);
+    return gps_pass_render(&p, fp, d_color, d_depth, st);
+}
+}  // namespace gps
+}
+
+extern "C" {
+
 gps_status gps_pass_set_timing(gps_pass* p, int32_t capacity) {
     if (!p || capacity < 0) return GPS_ERR_INVALID_ARG;
     destroy_events(p);

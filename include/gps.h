@@ -269,6 +269,31 @@ gps_status gps_pass_set_timing(gps_pass* pass, int32_t capacity);
 gps_status gps_pass_timing_summary(gps_pass* pass, double out_ms[GPS_NUM_STAGES],
                                    uint64_t out_launches[GPS_NUM_STAGES], uint64_t* out_frames);
 
+/* One frame with no device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The frame
+ * is what gps_pass_render gives on a pass created with this frame's size and
+ * spp, fully_disable_hierarchical_culling = 1 and max_number_of_points =
+ * max_points, rendering this one frame (bit for bit the same colour and
+ * depth); every intermediate buffer (weights, valid masks, splat records,
+ * tile sums, compacted entries, chunk starts, counters, framebuffer) lives in
+ * a static device workspace of GPS_NOALLOC_SCRATCH_BYTES that the library
+ * holds on every device it runs on until the process ends.  Limits: at most
+ * GPS_NOALLOC_MAX_GAUSSIANS Gaussians, a framebuffer of at most
+ * GPS_NOALLOC_MAX_FB_WORDS words (64x64 tiles x spp; 1920x1080 at ss 1 and
+ * 1 spp fits), max_points <= GPS_NOALLOC_MAX_POINTS; frames with occlusion
+ * or block culling are refused (both need state kept between frames).
+ * Beyond a limit: GPS_ERR_UNSUPPORTED, nothing enqueued.  Every call on one
+ * device shares its workspace, which keeps the last call's contents (each
+ * call clears what must start at zero): calls that overlap on one device
+ * race, so enqueue them on one stream or order them with events. */
+#define GPS_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define GPS_NOALLOC_MAX_GAUSSIANS (1ull << 19)
+#define GPS_NOALLOC_MAX_FB_WORDS (1ull << 21)
+#define GPS_NOALLOC_MAX_POINTS (1ull << 26)
+gps_status gps_frame_render_noalloc(gps_scene* scene, const gps_frame_params* frame, int64_t max_points,
+                                    float* d_color, float* d_depth, void* stream);
+
 /* ---------------------------------------------------------------- resolve */
 typedef enum gps_accumulation_mode {
     GPS_ACCUM_PROGRESSIVE = 0,
