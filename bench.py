@@ -248,7 +248,8 @@ COPY_CAL_BYTES = 1 << 30  # the calibration copy of --profile-window: 1 GiB read
 
 
 def _kname(r):
-    return r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+    """Kernel name without arguments, return type and namespace."""
+    return r["Kernel_Name"].split("(")[0].replace("void ", "").strip().replace("gps::", "")
 
 
 def _window(trace_csv, gap_s):
@@ -310,6 +311,20 @@ def summarize_trace(traces, counters=(), gap_s=0.5, traffic_out=None, window_nam
         lines.append(f"{'kernel':50s} {'counter':24s} {'per dispatch':>16s} {'dispatches':>10s}")
         for (k, c), (v, n) in sorted(means.items(), key=lambda kv: (-per.get(kv[0][0], {"ns": 0})["ns"], kv[0][1])):
             lines.append(f"{k[:50]:50s} {c:24s} {v:16.1f} {n:10d}")
+    # VALU issue utilisation: a wave64 VALU instruction occupies a 16-lane
+    # SIMD for 4 cycles; GRBM_GUI_ACTIVE counts the busy cycles of every XCD
+    # (MI355X: 8 XCDs, 256 CUs, 4 SIMDs each).
+    valu = {}
+    for (k, c), (v, n) in means.items():
+        if c == "SQ_INSTS_VALU" and (k, "GRBM_GUI_ACTIVE") in means:
+            cycles = means[(k, "GRBM_GUI_ACTIVE")][0] / 8.0
+            if cycles > 0:
+                valu[k] = v * 4.0 / (256 * 4 * cycles)
+    if valu:
+        lines.append("")
+        lines.append("VALU issue utilisation (SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x GRBM_GUI_ACTIVE / 8 XCDs)): " +
+                     ", ".join(f"{k} {u:.2f}" for k, u in sorted(valu.items(), key=lambda kv: -per.get(kv[0], {"ns": 0})["ns"])
+                               if per.get(k, {"ns": 0})["ns"] > 0))
     cal = {}
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
         m = means.get(("selftest_copy_kernel", c))
@@ -336,6 +351,7 @@ def summarize_trace(traces, counters=(), gap_s=0.5, traffic_out=None, window_nam
                 st["kernels"].append(k)
             st["_calls"] = max(st.get("_calls", 0), calls)
         out = {"_meta": {"window": window_name, "calibration": {c: round(f, 4) for c, f in cal.items()},
+                         "valu_busy": {k: round(u, 3) for k, u in valu.items()},
                          "unit": "HBM bytes per stage launch = sum over the stage's kernels of "
                                  "(FETCH_SIZE KiB x 1024 x fetch factor + WRITE_SIZE KiB x 1024 x write factor) "
                                  "per dispatch, x their dispatches per stage launch"}}
@@ -451,8 +467,8 @@ def attribute_errors(G, W, w, perm, chunk=1 << 21):
     worst = {"rotation_abs": 0.0, "rotation_rel": 0.0, "scale_rel": 0.0, "opacity_abs": 0.0, "sh_dc_abs": 0.0,
              "sh_rest_abs": 0.0, "sh_rest_rel_to_max": 0.0}
     nc = (w.sh_degree + 1) ** 2
-    for b in range(0, w.n, chunk):
-        a = W._generate(w, perm[b:min(w.n, b + chunk)])
+    for b in range(0, len(perm), chunk):
+        a = W._generate(w, perm[b:b + chunk])
         e = G.encode_half_fp8(a)
         q = a.rotations.astype(np.float64)
         dq = e["rotations"].view(np.float16).astype(np.float64) - q

@@ -411,7 +411,7 @@ struct orc_pass {
                         if (K > 1) n = (uint32_t)stochastic_round(static_cast<float>(n) / K, pcg2d(seed).x);
                         if (n <= 0) { reason = 7; break; }
                         n = std::min(n, gps::det_f2u_rz((float)(f.up_w * f.up_h) / (2 * K)));
-                        colors[idx] = sh_color(mean(idx), f.campos, &sh48[48 * idx]);
+                        colors[idx] = sh_color(mean(idx), f.campos, &sh48[48 * (uint64_t)idx]);
                         weights[idx] = n;
                         splats = true;
                         reason = 8;
@@ -896,7 +896,7 @@ int32_t orc_project_ids(orc_pass* p, const gps_frame_params* fp, const uint32_t*
         ivec2_ a, bb;
         float det, vz;
         if (!p->loadAndTransform(g, f, pmn, pm, cov, a, bb, det, vz)) continue;
-        const vec3<float> c = p->sh_color(p->mean(g), f.campos, &p->sh48[48 * g]);
+        const vec3<float> c = p->sh_color(p->mean(g), f.campos, &p->sh48[48 * (uint64_t)g]);
         const float op = p->opacity(g);
         const float imp = 2.0f * 3.14159265358979323846f * std::sqrt(det) * dilog(op);
         const double vals[16] = {1, vz, pm.x, pm.y, cov.x, cov.y, cov.z, c.x, c.y, c.z, imp, pmn.z,
