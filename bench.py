@@ -47,10 +47,13 @@ def stage_bytes(w, st):
     """Algorithmic (compulsory) HBM bytes per frame of each stage, from the
     frame's own counters: what each kernel must read or write at least once.
       clear       F words x 8 B
-      cull        32 B per block node + 4 B per 32 nodes, per pass
-      preprocess  per pass: cull bit + mask bit per Gaussian; weight (4 B),
-                  position and geometry for Gaussians of unculled blocks;
-                  SH + a 64-B record per valid Gaussian
+      cull        32 B per block node + 4 B per 32 nodes, per pass; with
+                  occlusion, the group test: 48 B per group of unculled
+                  blocks + its mask bits
+      preprocess  per pass: cull bit + mask bit per Gaussian; weight (4 B)
+                  for Gaussians of unculled blocks, position and geometry for
+                  those not in groups the group test skipped; SH + a 64-B
+                  record per valid Gaussian
       distribute  weights (4 B) of Gaussians of unculled blocks, 16 B per
                   tile per pass, 8 B (index, offset) per valid Gaussian
       splat       the 64-B record per valid Gaussian + one 8-B FB word per
@@ -70,12 +73,14 @@ def stage_bytes(w, st):
     P = sum(st["num_points"][:passes])
     unculled = sum(max(0, N - st["culled_blocks"][i] * gpb) for i in range(passes)) if w.hierarchical \
         else passes * N
+    skipped = min(unculled, 64 * sum(st.get("culled_groups", (0, 0))[:passes]))
+    group_test = (unculled / 64 * 48 + passes * math.ceil(N / 64) / 8) if w.occlusion else 0
     ntiles = math.ceil(N / 4096)
     combines = passes
     return {
         "clear": F * 8,
-        "cull": passes * (nblocks * 32 + math.ceil(nblocks / 32) * 4),
-        "preprocess": passes * N * 2 / 8 + unculled * (4 + pos_geo) + V * (sh_bytes + 64),
+        "cull": passes * (nblocks * 32 + math.ceil(nblocks / 32) * 4) + group_test,
+        "preprocess": passes * N * 2 / 8 + unculled * 4 + (unculled - skipped) * pos_geo + V * (sh_bytes + 64),
         "distribute": unculled * 4 + passes * ntiles * 16 + V * 8,
         "splat": V * 64 + P * 8,
         "combine": combines * (F * 8 + W * H * 16),
@@ -651,6 +656,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=0)
     ap.add_argument("--no-sort", action="store_true",
                     help="keep generation order instead of the load-time Morton order")
+    ap.add_argument("--no-group-cull", action="store_true",
+                    help="time the path without the group occlusion test (same outputs)")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serving measurement (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-sustained-seconds", type=float, default=30.0,
@@ -736,6 +743,7 @@ def main() -> int:
 
     stream = G.Stream()
     loop = W.FrameLoop(w, scene, stream)
+    loop.pass_.set_group_culling(not args.no_group_cull)
     log(f"[bench] warmup {Wm} frames, then {K} timed frames (camera positions {F0 + Wm}..{F0 + Wm + K - 1} mod "
         f"{w.path_frames})")
     run(loop, range(F0, F0 + Wm))
@@ -771,9 +779,11 @@ def main() -> int:
     # Work counters: replay the same deterministic frame sequence on a fresh
     # pass (same initial state), reading the counters after every frame.
     loop = W.FrameLoop(w, scene, stream)
+    loop.pass_.set_group_culling(not args.no_group_cull)
     loop.pass_.set_counters(True, count_rejects=True)
     per_frame = []
     rejects = [dict.fromkeys(G.REJECT_REASONS, 0) for _ in range(2)]
+    group_violations = 0
     run(loop, range(F0, F0 + Wm))
     for i in range(F0 + Wm, F0 + Wm + K):
         loop.step(W.frame(w, i % w.path_frames, i + 1))
@@ -781,6 +791,7 @@ def main() -> int:
         for acc, r in zip(rejects, loop.pass_.rejects(loop.stream)):
             for key, v in r.items():
                 acc[key] += v
+        group_violations += sum(loop.pass_.group_violations(loop.stream))
     loop.close()
     reject_report = [{key: round(v / K) for key, v in acc.items()} for acc in rejects]
     # The same frames with each splat form forced (binned vs point-parallel
@@ -892,7 +903,9 @@ def main() -> int:
         "two_pass_frames": sum(1 for p in passes if p == 2),
         "valid_gaussians_per_frame": round(sum(sum(st["valid_gaussians"][:st["passes"]]) for st in per_frame) / K),
         "culled_blocks_per_frame": round(sum(sum(st["culled_blocks"][:st["passes"]]) for st in per_frame) / K),
-        "preprocess_rejects_per_frame": {"first_pass": reject_report[0], "second_pass": reject_report[1]},
+        "skipped_groups_per_frame": round(sum(sum(st.get("culled_groups", (0, 0))[:st["passes"]]) for st in per_frame) / K),
+        "preprocess_rejects_per_frame": {"first_pass": reject_report[0], "second_pass": reject_report[1],
+                                         "skipped_group_survivors": group_violations},
         "splat_forms": splat_forms,
         "roofline": roofline,
         "cpu_baseline": cpu,

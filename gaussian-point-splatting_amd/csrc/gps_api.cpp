@@ -112,6 +112,11 @@ struct gps_pass {
     uint32_t* cull_mask = nullptr;
     CullView cull{};
     bool hier_available = false;
+    GroupBox* gboxes = nullptr;  // group occlusion test (null: off)
+    uint8_t* gocc = nullptr;     // occluded-group bytes (padded to whole words)
+    uint32_t* live = nullptr;    // unculled blocks of the pass, listed by the block cull
+    GroupView group{};
+    int32_t group_culling = 1;
     // splat grid
     uint32_t splat_grid = 0;
     int32_t xcds = 1;
@@ -129,6 +134,13 @@ struct gps_pass {
 };
 
 namespace {
+
+bool camera_finite(const FrameConsts& f) {
+    for (int i = 0; i < 16; ++i)
+        if (!std::isfinite(f.V[i]) || !std::isfinite(f.VP[i])) return false;
+    return std::isfinite(f.focal[0]) && std::isfinite(f.focal[1]) && std::isfinite(f.tan_fov[0]) &&
+           std::isfinite(f.tan_fov[1]) && std::isfinite(f.dilation);
+}
 
 uint64_t mip_total(HiZView& h, int w, int hgt) {
     h.width = w;
@@ -514,6 +526,16 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     }
     p->cull.nodes = p->nodes;
     p->cull.mask = p->cull_mask;
+    // Group bounds for the group occlusion test (0.75 byte per Gaussian).
+    p->group.count = (uint32_t)((N + GPS_GROUP - 1) / GPS_GROUP);
+    if (N > 0) {
+        chk(dalloc(&p->gboxes, p->group.count, false));
+        chk(dalloc(&p->gocc, round_up(p->group.count, 4) + 4, true));
+        if (p->hier_available) chk(dalloc(&p->live, nb, false));
+        if (s == GPS_OK) launch_group_boxes(scene->view, p->gboxes, p->group.count, nullptr);
+    }
+    p->group.boxes = p->gboxes;
+    p->group.occluded = p->gocc;
     if (s == GPS_OK)
         s = ensure_fb(p, st->supersampling_factor,
                       fb_words(st->width * st->supersampling_factor, st->height * st->supersampling_factor,
@@ -548,6 +570,12 @@ gps_status gps_pass_set_splat_mode(gps_pass* p, int32_t mode) {
     return GPS_OK;
 }
 
+gps_status gps_pass_set_group_culling(gps_pass* p, int32_t enable) {
+    if (!p || (enable != 0 && enable != 1)) return GPS_ERR_INVALID_ARG;
+    p->group_culling = enable;
+    return GPS_OK;
+}
+
 gps_status gps_pass_set_counters(gps_pass* p, int32_t flags) {
     if (!p || (flags & ~(GPS_COUNT_ATOMICS | GPS_COUNT_REJECTS))) return GPS_ERR_INVALID_ARG;
     p->count_atomics = (flags & GPS_COUNT_ATOMICS) ? 1 : 0;
@@ -577,6 +605,13 @@ uint64_t gps_pass_estimate_bytes(const gps_render_settings* st, const gps_scene*
         const uint64_t nb = (N + gpb - 1) / gpb;
         b += nb * sizeof(CullNode) + ((nb + 31) / 32 + 2) * 4;
     }
+    const uint64_t groups = (N + GPS_GROUP - 1) / GPS_GROUP;
+    if (N > 0) b += groups * sizeof(GroupBox) + round_up(groups, 4) + 4;
+    if (N > 0 && !st->fully_disable_hierarchical_culling) {
+        const uint64_t gpb = st->gaussians_per_block > 0 ? (uint64_t)st->gaussians_per_block
+                                                         : (uint64_t)std::max(4.0f, std::floor((float)N / 1e6f));
+        b += (N + gpb - 1) / gpb * 4;
+    }
     return b;
 }
 
@@ -587,6 +622,8 @@ uint64_t gps_pass_device_bytes(const gps_pass* p) {
                  4ull * 4 * p->ntiles + ((uint64_t)p->budget / GPS_SPLAT_CHUNK + 2) * 4 + sizeof(FrameCounters) +
                  (uint64_t)p->hiz_size * 4 + 2 * p->fb_cap * 8 + p->dbg_indices_cap * 4;
     if (p->hier_available) b += (uint64_t)p->cull.node_count * sizeof(CullNode) + (p->cull.mask_words + 2) * 4ull;
+    if (p->gboxes) b += (uint64_t)p->group.count * sizeof(GroupBox) + round_up(p->group.count, 4) + 4;
+    if (p->live) b += (uint64_t)p->cull.node_count * 4;
     b += std::max<uint64_t>(N, 1) * 16 + 3ull * p->list_cap * 4 + 2ull * (p->list_cap / GPS_TILE + 1) * 4 +
          8ull * 4 * GPS_BIN_HIST + 2ull * 4 * p->job_cap + 32;
     return b;
@@ -613,6 +650,9 @@ void gps_pass_destroy(gps_pass* p) {
     dfree(p->hiz.mip);
     dfree(p->nodes);
     dfree(p->cull_mask);
+    dfree(p->gboxes);
+    dfree(p->gocc);
+    dfree(p->live);
     dfree(p->erect);
     dfree(p->ew);
     dfree(p->big);
@@ -664,6 +704,8 @@ void make_frame_consts(const gps_frame_params* fp, int spp, FrameConsts* out) {
     f.reduce = fp->reduce_point_count ? 1 : 0;
     f.frame_seed = 0;
     f.only_prev = 0;
+    f.vn2 = (float)(view_norm2_bound(f.V) * (1.0 + 1e-6));
+    f.hiz_max = fp->occlusion_culling_min_reduce_count == 0 ? 1 : 0;
 }
 }  // namespace gps
 }
@@ -723,12 +765,24 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
 
     Mat4 vpm;
     std::memcpy(vpm.m, VP.m, 64);
+    // Group occlusion test: needs a max-only Hi-Z chain (no min-reduced levels).
+    const bool group_test = f.occl && p->gboxes && p->group_culling && fp->occlusion_culling_min_reduce_count == 0 &&
+                            camera_finite(f);
     auto render_pass = [&](bool first) {
         const int slot = first ? 0 : 1;
         const uint32_t frame_seed = (uint32_t)(4 * fp->frame + (first ? 0 : 2));
-        if (hier) {
+        if (hier || group_test) {
             stage_mark(p, 1, slot, 0, st);
-            launch_cull(p->cull, fp->frustum_planes, vpm, p->hiz, f.occl, &p->ctr->culled_blocks[slot], st);
+            if (hier)
+                launch_cull(p->cull, fp->frustum_planes, vpm, p->hiz, f.occl, &p->ctr->culled_blocks[slot],
+                            group_test ? p->live : nullptr, &p->ctr->live_blocks[slot], st);
+            // skipped groups are counted with the diagnostic counters only
+            uint32_t* gcount = (p->count_rejects || p->count_atomics) ? &p->ctr->culled_groups[slot] : nullptr;
+            if (group_test && hier)  // the groups of the blocks the block cull kept
+                launch_group_cull_live(p->group, p->cull, p->live, &p->ctr->live_blocks[slot], p->N, f, p->hiz,
+                                       p->cus * 8, gcount, st);
+            else if (group_test)
+                launch_group_cull(p->group, f, p->hiz, gcount, st);
             stage_mark(p, 1, slot, 1, st);
         }
         PreprocessArgs a;
@@ -745,6 +799,7 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         a.tile_cnt = p->tile_cnt;
         a.cull = p->cull;
         a.hiz = p->hiz;
+        a.gocc = group_test ? p->gocc : nullptr;
         a.rejects = p->count_rejects ? &p->ctr->rejects[0][0] : nullptr;
         stage_mark(p, 2, slot, 0, st);
         launch_preprocess(a, st);
@@ -1016,6 +1071,7 @@ gps_status gps_pass_stats(gps_pass* p, gps_stats* out, void* stream) {
         out->total_weight[i] = c.pass[i].total;
         out->num_points[i] = c.pass[i].points;
         out->culled_blocks[i] = c.culled_blocks[i];
+        out->culled_groups[i] = c.culled_groups[i];
         out->samples += (uint64_t)c.pass[i].points * (uint64_t)p->last_K * (uint64_t)p->last_spp;
         if (c.pass[i].total > c.pass[i].points) out->budget_truncated = 1;
     }
@@ -1053,7 +1109,7 @@ uint64_t gps_pass_debug_size(gps_pass* p, int32_t which) {
         case GPS_DBG_RECORDS: return 16ull * p->N;
         case GPS_DBG_HIZ: return p->hiz_size;
         case GPS_DBG_CULL_MASK: return p->cull.mask_words;
-        case GPS_DBG_REJECTS: return 18;
+        case GPS_DBG_REJECTS: return 20;
     }
     return 0;
 }

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "../../include/gps.h"
+#include "gps_hiz.h"
 #include "gps_project.h"
 
 namespace gps {
@@ -64,15 +65,6 @@ struct SplatRecord {
 static_assert(sizeof(SplatRecord) == 64, "record must stay 64 bytes");
 static_assert(sizeof(gps_quantized_bounds) == 102 * sizeof(float), "scene box is read as 102 floats");
 
-// Hi-Z pyramid (coarsest level 0 first), src/core/rendering/occlusion/depth_mip_chain.cuh:10-71.
-#define GPS_HIZ_MAX_LEVELS 16
-struct HiZView {
-    float* mip;
-    int32_t width, height, levels;
-    int32_t lw[GPS_HIZ_MAX_LEVELS];
-    int32_t lh[GPS_HIZ_MAX_LEVELS];
-    uint32_t off[GPS_HIZ_MAX_LEVELS];
-};
 
 // Flat culling blocks (gaussian_bvh.cuh:28-53).
 struct CullNode {
@@ -86,6 +78,16 @@ struct CullView {
     uint32_t node_count;
     uint32_t mask_words;
     uint32_t gpb;   // Gaussians per block
+};
+
+// Group occlusion test (gps_hiz.h): group bounds built once per pass, one
+// byte per group and pass.  Only groups with a member in an unculled block
+// are written in a pass; the preprocess reads no other (their members are
+// not tested).
+struct GroupView {
+    const GroupBox* boxes;
+    uint8_t* occluded;    // 1: every member is rejected by the Hi-Z test (this pass)
+    uint32_t count;       // ceil(N / GPS_GROUP); the byte array is padded to a multiple of 4
 };
 
 // Device-side counters of one render pass.
@@ -102,11 +104,12 @@ struct FrameCounters {
     uint32_t hiz_ticket;
     uint32_t iteration;   // valid-mask ping-pong index (flips when a pass yields points)
     uint32_t atomics;     // 64-bit atomicMin issued by the splat kernel
-    uint32_t pad[3];
+    uint32_t culled_groups[2];  // per pass: groups whose members all skip the per-Gaussian steps
+    uint32_t live_blocks[2];    // per pass: unculled blocks listed by the block cull for the group test
     uint32_t rejects[2][9];  // per pass: preprocess reject reasons (diagnostic; see RejectReason)
-    uint32_t pad2[2];
+    uint32_t group_violations[2];  // per pass, with the reasons: members of culled groups that survived (must be 0)
 };
-static_assert(sizeof(FrameCounters) == 144, "counter words zeroed by the clear kernel");
+static_assert(sizeof(FrameCounters) == 148, "counter words zeroed by the clear kernel");
 
 // Region geometry of the binned splat for one framebuffer.
 struct BinGeom {

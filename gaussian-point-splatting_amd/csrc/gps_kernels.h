@@ -17,7 +17,8 @@ struct PreprocessArgs {
     uint32_t* tile_cnt;
     CullView cull;
     HiZView hiz;
-    uint32_t* rejects = nullptr;  // 2 x 9 reject-reason counters (diagnostic), or null
+    const uint8_t* gocc = nullptr;  // occluded-group bytes of this pass (group cull), or null
+    uint32_t* rejects = nullptr;  // 2 x 9 reject-reason counters + 2 group checks (diagnostic), or null
 };
 
 struct DistributeArgs {
@@ -142,8 +143,10 @@ void launch_batch_bin_big(const BatchArgs& a, hipStream_t st);
 // Fills the framebuffer with `value`; also zeroes n0 words at z0 and n1 at z1.
 void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st, uint32_t* z0 = nullptr,
                   uint32_t n0 = 0, uint32_t* z1 = nullptr, uint32_t n1 = 0);
+// `live` (or null): the unculled blocks are listed there for the group
+// occlusion test, their count in *live_count (zeroed by the frame's clear).
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
-                 uint32_t* culled_count, hipStream_t st);
+                 uint32_t* culled_count, uint32_t* live, uint32_t* live_count, hipStream_t st);
 void launch_preprocess(const PreprocessArgs& a, hipStream_t st);
 void launch_distribute(const DistributeArgs& a, hipStream_t st);
 void launch_splat(const SplatArgs& a, hipStream_t st);
@@ -153,6 +156,14 @@ void launch_hiz_build(const HiZView& h, const float* depth, int kmin, uint32_t* 
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st);
 void launch_fill_f32(float* p, uint32_t n, float v, hipStream_t st);
 void launch_build_nodes(const DevScene& s, uint32_t gpb, uint32_t nb, CullNode* nodes, hipStream_t st);
+// Bounds of every group of GPS_GROUP Gaussians (once per pass), and the
+// per-pass conservative group occlusion test against the pass's Hi-Z chain.
+void launch_group_boxes(const DevScene& s, GroupBox* boxes, uint32_t count, hipStream_t st);
+void launch_group_cull(const GroupView& gv, const FrameConsts& f, const HiZView& h, uint32_t* culled_count,
+                       hipStream_t st);
+void launch_group_cull_live(const GroupView& gv, const CullView& cv, const uint32_t* live, const uint32_t* live_count,
+                            uint32_t N, const FrameConsts& f, const HiZView& h, uint32_t grid,
+                            uint32_t* culled_count, hipStream_t st);
 void launch_taa(const float* cur, const float* cur_depth, const float* prev, const Mat4& M, float blend, int W, int H,
                 int frame_number, float* out, uint8_t* rgb8, hipStream_t st);
 void launch_progressive(float* color, float* accum, int W, int H, int n, uint8_t* rgb8, hipStream_t st);

@@ -60,14 +60,20 @@ __device__ __forceinline__ bool mask_bit(const uint32_t* m, uint32_t i) { return
 // more than half an ulp) or, for blocks of 2^23 or more, 0 or 1.
 struct WgBlocks {
     uint32_t b0, r0;
+    bool some_culled;  // a block of the workgroup is culled (else no lane needs its own lookup)
 };
 __device__ __forceinline__ bool wg_all_culled(const CullView& c, uint32_t g0, uint32_t n_in_wg, WgBlocks& wb) {
     wb.b0 = g0 / c.gpb;
     wb.r0 = g0 - wb.b0 * c.gpb;
     const uint32_t b1 = (g0 + n_in_wg - 1u) / c.gpb;
-    for (uint32_t b = wb.b0; b <= b1; ++b)
-        if (!mask_bit(c.mask, b)) return false;
-    return true;
+    bool all = true, some = false;
+    for (uint32_t b = wb.b0; b <= b1; ++b) {
+        const bool cb = mask_bit(c.mask, b);
+        all = all && cb;
+        some = some || cb;
+    }
+    wb.some_culled = some;
+    return all;
 }
 __device__ __forceinline__ bool lane_block_culled(const CullView& c, const WgBlocks& wb, uint32_t lane_off) {
     const uint32_t o = wb.r0 + lane_off;
@@ -82,48 +88,6 @@ __device__ __forceinline__ bool block_culled(const CullView& c, uint32_t g) {
     const uint32_t blk = (g % span) / c.gpb;
     return ((c.mask[tree] >> blk) & 1u) != 0u;
 }
-
-// ceil(log2(d / 2)) for integer d >= 1, exactly as the float expression of
-// depth_mip_chain.cuh:186 evaluates it (log2 of a power of two is exact; any
-// other d < 2^24 is far from an integer).
-__device__ __forceinline__ int ceil_log2_half(int d) {
-    if (d <= 1) return -1;
-    const int cl = 32 - __clz((uint32_t)(d - 1));  // ceil(log2(d))
-    return cl - 1;
-}
-
-// depth_mip_chain.cuh:172-215
-__device__ __forceinline__ bool hiz_occluded(const HiZView& h, float depth, int minx, int miny, int maxx, int maxy) {
-    minx = max(0, minx);
-    miny = max(0, miny);
-    maxx = min(h.width - 1, maxx);
-    maxy = min(h.height - 1, maxy);
-    const int w = maxx - minx + 1;
-    const int hh = maxy - miny + 1;
-    const int d = max(w, hh);
-    int lod = ceil_log2_half(d);
-    lod = min(max(lod, 0), h.levels - 1);
-    const int l = (h.levels - 1) - lod;
-    const int wl = h.lw[l], hl = h.lh[l];
-    // The tile is max(1, min(size, 2^k)) texels with k = levels - l - 1 >= 0:
-    // a power of two (a shift), or the whole image, which the coordinates
-    // (clamped to [0, size - 1] above) divide to 0.
-    const int k = h.levels - l - 1;
-    const bool kx = (1 << k) <= h.width, ky = (1 << k) <= h.height;
-    const int tx0 = max(0, min(wl - 1, kx ? minx >> k : 0));
-    const int ty0 = max(0, min(hl - 1, ky ? miny >> k : 0)) * wl;
-    const int tx1 = max(0, min(wl - 1, kx ? maxx >> k : 0));
-    const int ty1 = max(0, min(hl - 1, ky ? maxy >> k : 0)) * wl;
-    const float* m = h.mip + h.off[l];
-    const float zmax = det_fmax(m[tx0 + ty0], det_fmax(m[tx1 + ty0], det_fmax(m[tx0 + ty1], m[tx1 + ty1])));
-    return zmax < depth;
-}
-
-// A supersampled pixel bound in Hi-Z pixels: x / ss, as a shift when ss is a
-// power of two.  For negative x the shift rounds down where the division
-// truncates; hiz_occluded clamps both to 0, and only the minima can be
-// negative (a Gaussian with a negative maximum was rejected before).
-__device__ __forceinline__ int div_ss(const FrameConsts& f, int x) { return f.ss_shift >= 0 ? x >> f.ss_shift : x / f.ss; }
 
 // ------------------------------------------------------------------ scene access
 __device__ __forceinline__ float plane(const DevScene& s, int p, uint32_t g) {
@@ -340,8 +304,32 @@ __global__ __launch_bounds__(256) void clear_kernel(uint4* __restrict__ fb, uint
 struct Planes {
     float p[24];
 };
+// Appends the lanes with `pred` to a list with one atomic per workgroup on
+// the list's counter; returns each such lane's slot.
+__device__ __forceinline__ uint32_t wg_append(uint32_t* counter, bool pred) {
+    __shared__ uint32_t s_wave[4], s_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t b = __ballot(pred);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            const uint32_t c = s_wave[w];
+            s_wave[w] = t;
+            t += c;
+        }
+        s_base = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    return s_base + s_wave[wave] + __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+}
+
+// `live` (or null): the unculled blocks are listed there, their count in
+// *live_count (the group occlusion test runs over their groups only).
 __global__ __launch_bounds__(256) void cull_kernel(CullView cv, Planes pl, Mat4 vp, HiZView h, int occl,
-                                                   uint32_t* __restrict__ culled_count) {
+                                                   uint32_t* __restrict__ culled_count, uint32_t* __restrict__ live,
+                                                   uint32_t* __restrict__ live_count) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     bool culled = false;
     if (idx < cv.node_count) {
@@ -400,6 +388,96 @@ __global__ __launch_bounds__(256) void cull_kernel(CullView cv, Planes pl, Mat4 
         if (w0 + 1 < cv.mask_words) cv.mask[w0 + 1] = (uint32_t)(b >> 32);
     }
     add_count(culled_count, culled ? 1u : 0u);  // one atomic per workgroup on the single counter
+    if (live) {
+        const bool keep = idx < cv.node_count && !culled;
+        const uint32_t k = wg_append(live_count, keep);
+        if (keep) live[k] = idx;
+    }
+}
+
+// ------------------------------------------------------------------ group occlusion test
+// group_occluded (gps_hiz.h) per group of GPS_GROUP Gaussians, before the
+// preprocess of each pass; the preprocess skips the marked groups.
+
+// One wave per group: the box of the centres and the covariance bounds of the
+// members, decoded exactly as the preprocess decodes them.
+template <int FORMAT>
+__global__ __launch_bounds__(256) void group_box_kernel(DevScene s, GroupBox* __restrict__ boxes, uint32_t count) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gi = t / GPS_GROUP;
+    if (gi >= count) return;  // whole waves (count * 64 >= N)
+    float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    float lam = 0.0f, sig[3] = {0.0f, 0.0f, 0.0f};
+    bool ok = true;
+    if (t < s.N) {
+        const Geo ge = load_geo<FORMAT>(s, t);
+        const float a[10] = {ge.mx, ge.my, ge.mz, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx, ge.sy, ge.sz};
+        ok = group_member_bounds(a, &lam, sig);
+        lo[0] = hi[0] = ge.mx;
+        lo[1] = hi[1] = ge.my;
+        lo[2] = hi[2] = ge.mz;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = fminf(lo[i], __shfl_xor(lo[i], o, 64));
+            hi[i] = fmaxf(hi[i], __shfl_xor(hi[i], o, 64));
+            sig[i] = fmaxf(sig[i], __shfl_xor(sig[i], o, 64));
+        }
+        lam = fmaxf(lam, __shfl_xor(lam, o, 64));
+    }
+    ok = __ballot(!ok) == 0ull;
+    if ((threadIdx.x & 63) == 0) {
+        GroupBox b;
+        for (int i = 0; i < 3; ++i) {
+            b.lo[i] = lo[i];
+            b.hi[i] = hi[i];
+            b.sig[i] = sig[i];
+        }
+        b.lam = lam;
+        b.ok = ok ? 1u : 0u;
+        b.pad = 0.0f;
+        boxes[gi] = b;
+    }
+}
+
+// Without block culling: one thread per group.
+__global__ __launch_bounds__(256) void group_cull_kernel(GroupView gv, FrameConsts f, HiZView h,
+                                                         uint32_t* __restrict__ culled_count) {
+    const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    bool occ = false;
+    if (gi < gv.count) {
+        occ = group_occluded(gv.boxes[gi], f, h);
+        gv.occluded[gi] = occ ? 1u : 0u;
+    }
+    if (culled_count) add_count(culled_count, occ ? 1u : 0u);
+}
+
+// With block culling: the groups of the unculled blocks the block cull
+// listed (grid-stride over list entries x groups per block).  A group that
+// straddles two unculled blocks is written twice with the same value; it is
+// counted with the block holding its first Gaussian.  The count (diagnostic,
+// `culled_count` null when off) costs one atomic on one word per workgroup.
+__global__ __launch_bounds__(256) void group_cull_live_kernel(GroupView gv, CullView cv, const uint32_t* __restrict__ live,
+                                                              const uint32_t* __restrict__ live_count, uint32_t per_block,
+                                                              uint32_t N, FrameConsts f, HiZView h,
+                                                              uint32_t* __restrict__ culled_count) {
+    const uint64_t items = (uint64_t)(*live_count) * per_block;
+    uint32_t counted = 0;
+    for (uint64_t it = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; it < items;
+         it += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = live[it / per_block];
+        const uint32_t first = b * cv.gpb;                                // block's first Gaussian
+        const uint32_t last = min(N, first + cv.gpb) - 1u;                // and last
+        const uint32_t gi = first / GPS_GROUP + (uint32_t)(it % per_block);
+        if (gi > last / GPS_GROUP) continue;
+        const bool occ = group_occluded(gv.boxes[gi], f, h);
+        gv.occluded[gi] = occ ? 1u : 0u;
+        counted += (occ && gi * GPS_GROUP >= first) ? 1u : 0u;
+    }
+    if (culled_count) add_count(culled_count, counted);
 }
 
 // ------------------------------------------------------------------ preprocess
@@ -476,6 +554,24 @@ struct ColourItem {
     uint32_t g, depth, bmax;
     float mx, my, mz;
 };
+__device__ __forceinline__ void push_item(PreItem* s_item, uint32_t* s_n, uint32_t g, float op, const Projected& pr,
+                                          float mx, float my, float mz) {
+    PreItem& q = s_item[atomicAdd(s_n, 1u)];
+    q.g = g;
+    q.op = op;
+    q.view_z = pr.view_z;
+    q.pmx = pr.pmx;
+    q.pmy = pr.pmy;
+    q.ca = pr.ca;
+    q.cb = pr.cb;
+    q.cc = pr.cc;
+    q.det = pr.det;
+    q.bmin = (uint32_t)pr.minx | ((uint32_t)pr.miny << 16);
+    q.bmax = (uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16);
+    q.mx = mx;
+    q.my = my;
+    q.mz = mz;
+}
 
 // Two 32-bit valid-mask words per wave and the workgroup's (points, valid)
 // sums added to its 4096-Gaussian tile; n = this lane's point count.
@@ -544,7 +640,7 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
     const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
     uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h,
-    uint32_t* __restrict__ rejects) {
+    const uint8_t* __restrict__ gocc_bytes, uint32_t* __restrict__ rejects) {
     __shared__ PreItem s_item[GPS_PRE_THREADS];
     // list 2 reuses list 1's storage (read into registers before a barrier):
     // 16 KB of LDS per workgroup, eight workgroups per CU
@@ -559,25 +655,44 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     const uint32_t it = *iteration;
     uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
     const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
-    WgBlocks wb{0u, 0u};
+    WgBlocks wb{0u, 0u, false};
     if (f.hier && wg_all_culled(cv, g0, min((uint32_t)GPS_PRE_THREADS, s.N - g0), wb)) {
         // Every Gaussian here is culled: only the valid-mask words (zero)
         // are written; weights stay untouched, the tile sums get nothing.
         if (tid < GPS_PRE_THREADS / 32 && g0 + 32u * (uint32_t)tid < s.N) mask_cur[(g0 >> 5) + (uint32_t)tid] = 0u;
         return;
     }
-    __shared__ uint32_t s_rej[STATS ? REJ_COUNT : 1];
+    // Occluded groups (group_cull_kernel; one group per wave): their members
+    // are rejected by the Hi-Z test or earlier, so they take weight 0 here.
+    // The counting form runs every step and checks that claim instead.
+    static_assert(GPS_PRE_THREADS / GPS_GROUP == 4 && GPS_GROUP == 64, "a workgroup's four group bytes in one word");
+    constexpr uint32_t kAllGroups = 0xFu;
+    uint32_t gbits = 0u;
+    if (gocc_bytes) {
+        const uint32_t w = reinterpret_cast<const uint32_t*>(gocc_bytes)[g0 / GPS_PRE_THREADS];
+        gbits = (w & 1u) | ((w >> 7) & 2u) | ((w >> 14) & 4u) | ((w >> 21) & 8u);
+    }
+    const bool lane_blk = f.hier && wb.some_culled && lane_block_culled(cv, wb, (uint32_t)tid);
+    const bool gocc = ((gbits >> (tid / GPS_GROUP)) & 1u) != 0u;
+    if (!STATS && gbits == kAllGroups) {
+        if (g < s.N && !lane_blk) weights[g] = 0u;
+        if (tid < GPS_PRE_THREADS / 32 && g0 + 32u * (uint32_t)tid < s.N) mask_cur[(g0 >> 5) + (uint32_t)tid] = 0u;
+        return;
+    }
+    __shared__ uint32_t s_rej[STATS ? REJ_COUNT + 1 : 1];
     if (tid == 0) {
         s_n1 = 0u;
         s_n2 = 0u;
     }
-    if (STATS && tid < REJ_COUNT) s_rej[tid] = 0u;
+    if (STATS && tid <= REJ_COUNT) s_rej[tid] = 0u;
     s_count[tid] = 0u;
     __syncthreads();
     // Phase A.  Gaussians of culled blocks leave their weight slot untouched:
     // the distributor reads the same cull mask and counts them as zero.
     uint32_t reason = REJ_TESTED;  // STATS: this lane's reject reason (REJ_TESTED = none yet)
-    const bool tested = g < s.N && !(f.hier && lane_block_culled(cv, wb, (uint32_t)tid));
+    const bool tested = g < s.N && !lane_blk && (STATS || !gocc);
+    if (!STATS && gocc && g < s.N && !lane_blk) weights[g] = 0u;
+    if constexpr (STATS) {
     if (tested) {
         bool survives = false;
         if (f.only_prev && mask_bit(mask_prev, g)) {
@@ -612,29 +727,48 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
                     survives = false;
                     reason = REJ_HIZ;
                 }
-                if (survives) {
-                    const uint32_t k = atomicAdd(&s_n1, 1u);
-                    PreItem& q = s_item[k];
-                    q.g = g;
-                    q.op = op;
-                    q.view_z = pr.view_z;
-                    q.pmx = pr.pmx;
-                    q.pmy = pr.pmy;
-                    q.ca = pr.ca;
-                    q.cb = pr.cb;
-                    q.cc = pr.cc;
-                    q.det = pr.det;
-                    q.bmin = (uint32_t)pr.minx | ((uint32_t)pr.miny << 16);
-                    q.bmax = (uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16);
-                    q.mx = ge.mx;
-                    q.my = ge.my;
-                    q.mz = ge.mz;
-                }
+                if (survives) push_item(s_item, &s_n1, g, op, pr, ge.mx, ge.my, ge.mz);
             }
         }
         if (!survives) weights[g] = 0u;
     }
-    if (STATS) count_reasons(s_rej, reason, tested);
+    } else {
+    // The same steps without the reasons.  The previous-mask word, the
+    // attributes and the projection's centre are all needed before the first
+    // branch, so their loads go out together (one memory round trip).
+    if (tested) {
+        bool survives = false;
+        const uint32_t prev_word = mask_prev[g >> 5];  // read in both passes: no branch before the other loads
+        const Geo ge = load_geo<FORMAT>(s, g);
+        const bool prev = f.only_prev && ((prev_word >> (g & 31u)) & 1u) != 0u;
+        const float op = glm_min(ge.op, 1.0f);
+        Center ce;
+        const bool front = project_center(f, ge.mx, ge.my, ge.mz, ce);
+        Projected pr;
+        if (!prev && op > 0.0f && front)
+            survives = project_footprint(f, ce, ge.qw, ge.qx, ge.qy, ge.qz, ge.sx, ge.sy, ge.sz, pr) &&
+                       !(pr.minx >= f.up_w || pr.miny >= f.up_h || pr.maxx < 0 || pr.maxy < 0);
+        if (survives && f.cull_small) {
+            const float tr = pr.ca + pr.cc;
+            const float t2 = 0.5f * tr;
+            const float term2 = det_sqrt(det_fmax(0.1f, t2 * t2 - pr.det));
+            const float e1 = t2 + term2, e2 = t2 - term2;
+            survives = !(e1 < 0.02f || e2 < 0.02f);
+        }
+        if (survives && f.occl)
+            survives = !hiz_occluded(h, pr.pz, div_ss(f, pr.minx), div_ss(f, pr.miny), div_ss(f, pr.maxx),
+                                     div_ss(f, pr.maxy));
+        if (survives)
+            push_item(s_item, &s_n1, g, op, pr, ge.mx, ge.my, ge.mz);
+        else
+            weights[g] = 0u;
+    }
+    }
+    if (STATS) {
+        count_reasons(s_rej, reason, tested);
+        const uint32_t nv = (uint32_t)__popcll(__ballot(tested && gocc && reason == REJ_TESTED));
+        if ((tid & 63) == 0 && nv) atomicAdd(s_rej + REJ_COUNT, nv);  // survivors of occluded groups
+    }
     __syncthreads();
     // Phase B
     const bool in1 = (uint32_t)tid < s_n1;
@@ -683,6 +817,7 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     }
     __syncthreads();
     if (STATS && tid < REJ_COUNT && s_rej[tid]) atomicAdd(rejects + (f.only_prev ? REJ_COUNT : 0) + tid, s_rej[tid]);
+    if (STATS && tid == REJ_COUNT && s_rej[tid]) atomicAdd(rejects + 2 * REJ_COUNT + (f.only_prev ? 1 : 0), s_rej[tid]);
     // Phase C
     if ((uint32_t)tid < s_n2) {
         const ColourItem q = s_col[tid];
@@ -2461,11 +2596,11 @@ static void launch_pre(const PreprocessArgs& a, hipStream_t st) {
     if (a.rejects)  // reject-reason counters: the work-list kernel (same outputs) with counting
         hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG, true>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
                            a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
-                           a.cull, a.hiz, a.rejects);
+                           a.cull, a.hiz, a.gocc, a.rejects);
     else if (a.f.hier || a.f.occl)  // culled frames: work lists keep the later phases on full waves
         hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG, false>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
                            a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
-                           a.cull, a.hiz, nullptr);
+                           a.cull, a.hiz, a.gocc, nullptr);
     else
         hipLaunchKernelGGL((preprocess_flat_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
                            a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
@@ -2507,12 +2642,13 @@ void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st, 
 }
 
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
-                 uint32_t* culled_count, hipStream_t st) {
+                 uint32_t* culled_count, uint32_t* live, uint32_t* live_count, hipStream_t st) {
     if (cv.node_count == 0) return;
     Planes pl;
     for (int i = 0; i < 24; ++i) pl.p[i] = planes[i];
     const uint32_t blocks = (cv.node_count + 255) / 256;
-    hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, st, cv, pl, vp, h, occl, culled_count);
+    hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, st, cv, pl, vp, h, occl, culled_count, live,
+                       live_count);
 }
 
 // One tile-scan workgroup per 8192 tiles, at least one (it publishes the
@@ -2583,6 +2719,33 @@ void launch_fill_f32(float* p, uint32_t n, float v, hipStream_t st) {
 
 void launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t st) {
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, p, n);
+}
+
+void launch_group_boxes(const DevScene& s, GroupBox* boxes, uint32_t count, hipStream_t st) {
+    if (count == 0) return;
+    const uint32_t blocks = (count + 3) / 4;  // four groups (waves) per workgroup
+    if (s.format == GPS_FORMAT_FP32)
+        hipLaunchKernelGGL(group_box_kernel<GPS_FORMAT_FP32>, dim3(blocks), dim3(256), 0, st, s, boxes, count);
+    else if (s.format == GPS_FORMAT_QUANTIZED)
+        hipLaunchKernelGGL(group_box_kernel<GPS_FORMAT_QUANTIZED>, dim3(blocks), dim3(256), 0, st, s, boxes, count);
+    else
+        hipLaunchKernelGGL(group_box_kernel<GPS_FORMAT_HALF_FP8>, dim3(blocks), dim3(256), 0, st, s, boxes, count);
+}
+
+void launch_group_cull(const GroupView& gv, const FrameConsts& f, const HiZView& h, uint32_t* culled_count,
+                       hipStream_t st) {
+    if (gv.count == 0) return;
+    hipLaunchKernelGGL(group_cull_kernel, dim3((gv.count + 255) / 256), dim3(256), 0, st, gv, f, h, culled_count);
+}
+
+void launch_group_cull_live(const GroupView& gv, const CullView& cv, const uint32_t* live, const uint32_t* live_count,
+                            uint32_t N, const FrameConsts& f, const HiZView& h, uint32_t grid,
+                            uint32_t* culled_count, hipStream_t st) {
+    if (gv.count == 0 || cv.node_count == 0) return;
+    // groups a block meets: gpb / 64 when blocks start on group boundaries, else one more
+    const uint32_t per_block = (cv.gpb % GPS_GROUP == 0) ? cv.gpb / GPS_GROUP : (cv.gpb + GPS_GROUP - 1) / GPS_GROUP + 1;
+    hipLaunchKernelGGL(group_cull_live_kernel, dim3(grid), dim3(256), 0, st, gv, cv, live, live_count, per_block, N, f,
+                       h, culled_count);
 }
 
 void launch_build_nodes(const DevScene& s, uint32_t gpb, uint32_t nb, CullNode* nodes, hipStream_t st) {

@@ -29,6 +29,8 @@ struct FrameConsts {
     uint32_t frame_seed;
     int32_t occl, hier, cull_small, unbiased, reduce, only_prev;
     int32_t ss_shift;   // log2(ss) when ss is a power of two, else -1
+    float vn2;          // >= |mat3(V)|_2^2 (footprint bounds of the occlusion tests)
+    int32_t hiz_max;    // the Hi-Z chain is max-only (no min-reduced levels): the bounds apply
 };
 
 struct Projected {
@@ -41,8 +43,16 @@ struct Projected {
 };
 
 // linalg.cuh:319-342: R = mat3_cast(quat(w,x,y,z)), M = R*S, cov = M*M^T.
-// Returns the 6 unique entries c[col][row] as cov00,cov01,cov02,cov11,cov12,cov22
-// plus the full matrix in `C` (column-major 3x3).
+// Returns the full matrix in `C` (column-major 3x3).
+//
+// S is diagonal, so the reference's sums in M = R*S each hold one product and
+// two products with an exact zero; they are left out here.  For a finite
+// rotation (quaternion components below 2^63) a product with zero is +-0,
+// and x + (+-0) is x for x != 0, so only the sign of a zero entry can differ.
+// Signs of zeros never reach a result: cov enters the 2D covariance only
+// through sums that end in "+ dilation" (> 0) or "+ 0.0f" (both zeros give
+// +0), and the diagonal only through squares.  The same holds for the zero
+// entries of the Jacobian in project_gaussian.
 GPS_HD void scale_rot_to_cov(float sx, float sy, float sz, float qw, float qx, float qy, float qz, float C[9]) {
     const float xx = qx * qx, yy = qy * qy, zz = qz * qz;
     const float xz = qx * qz, xy = qx * qy, yz = qy * qz;
@@ -57,12 +67,12 @@ GPS_HD void scale_rot_to_cov(float sx, float sy, float sz, float qw, float qx, f
     R[6] = 2.0f * (xz + wy);
     R[7] = 2.0f * (yz - wx);
     R[8] = 1.0f - 2.0f * (xx + yy);
-    const float S[9] = {sx, 0.0f, 0.0f, 0.0f, sy, 0.0f, 0.0f, 0.0f, sz};
-    float M[9];
+    const float s3[3] = {sx, sy, sz};
+    float M[9];  // column j of R times s_j
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) M[3 * j + r] = (R[r] * S[3 * j] + R[3 + r] * S[3 * j + 1]) + R[6 + r] * S[3 * j + 2];
+        for (int r = 0; r < 3; ++r) M[3 * j + r] = R[3 * j + r] * s3[j];
     // M * transpose(M): column j of M^T is (M[0][j], M[1][j], M[2][j]).
 #pragma unroll
     for (int j = 0; j < 3; ++j)
@@ -71,27 +81,43 @@ GPS_HD void scale_rot_to_cov(float sx, float sy, float sz, float qw, float qx, f
             C[3 * j + r] = (M[r] * M[j] + M[3 + r] * M[3 + j]) + M[6 + r] * M[6 + j];
 }
 
-// loadAndTransformGaussian (gaussian_point_splatting.cu:17-65).  Returns false
-// when the Gaussian is rejected.
-GPS_HD bool project_gaussian(const FrameConsts& f, float mx, float my, float mz, float qw, float qx, float qy, float qz,
-                             float sx, float sy, float sz, Projected& o) {
+// The centre part of loadAndTransformGaussian (gaussian_point_splatting.cu:
+// 17-36): view-space position, near-plane test, NDC position and its test.
+struct Center {
+    float vx, vy, vz;  // view space
+    float px, py, pz;  // NDC
+};
+GPS_HD bool project_center(const FrameConsts& f, float mx, float my, float mz, Center& c) {
     const float* V = f.V;
     // transformPoint4x3 (linalg.cuh:211-215)
-    const float vx = ((V[0] * mx + V[4] * my) + V[8] * mz) + V[12];
-    const float vy = ((V[1] * mx + V[5] * my) + V[9] * mz) + V[13];
-    const float vz = ((V[2] * mx + V[6] * my) + V[10] * mz) + V[14];
-    o.view_z = vz;
-    if (-vz <= 0.2f) return false;
+    c.vx = ((V[0] * mx + V[4] * my) + V[8] * mz) + V[12];
+    c.vy = ((V[1] * mx + V[5] * my) + V[9] * mz) + V[13];
+    c.vz = ((V[2] * mx + V[6] * my) + V[10] * mz) + V[14];
+    if (-c.vz <= 0.2f) return false;
     const float* P = f.VP;
     const float hx = (P[0] * mx + P[4] * my) + (P[8] * mz + P[12] * 1.0f);
     const float hy = (P[1] * mx + P[5] * my) + (P[9] * mz + P[13] * 1.0f);
     const float hz = (P[2] * mx + P[6] * my) + (P[10] * mz + P[14] * 1.0f);
     const float hw = (P[3] * mx + P[7] * my) + (P[11] * mz + P[15] * 1.0f);
     const float iw = 1.0f / hw;
-    o.px = hx * iw;
-    o.py = hy * iw;
-    o.pz = hz * iw;
-    if (o.pz < -1.0f || o.pz > 1.0f) return false;
+    c.px = hx * iw;
+    c.py = hy * iw;
+    c.pz = hz * iw;
+    return !(c.pz < -1.0f || c.pz > 1.0f);
+}
+// Pixel-space mean (upsampled pixels).
+GPS_HD float pixel_mean(float p, int32_t size) { return (p * 0.5f + 0.5f) * (float)size; }
+
+// The rest of loadAndTransformGaussian after project_center (its :37-65):
+// 3D and 2D covariance, determinant test, pixel mean, tile bounds.
+GPS_HD bool project_footprint(const FrameConsts& f, const Center& ce, float qw, float qx, float qy, float qz, float sx,
+                              float sy, float sz, Projected& o) {
+    const float* V = f.V;
+    o.view_z = ce.vz;
+    o.px = ce.px;
+    o.py = ce.py;
+    o.pz = ce.pz;
+    const float vx = ce.vx, vy = ce.vy, vz = ce.vz;
 
     float C3[9];
     scale_rot_to_cov(sx, sy, sz, qw, qx, qy, qz, C3);
@@ -103,16 +129,18 @@ GPS_HD bool project_gaussian(const FrameConsts& f, float mx, float my, float mz,
     const float tytz = vy / vz;
     const float tx = det_fmin(limx, det_fmax(-limx, txtz)) * vz;
     const float ty = det_fmin(limy, det_fmax(-limy, tytz)) * vz;
-    const float J[9] = {f.focal[0] / vz, 0.0f, -(f.focal[0] * tx) / (vz * vz),
-                        0.0f, f.focal[1] / vz, -(f.focal[1] * ty) / (vz * vz),
-                        0.0f, 0.0f, 0.0f};
-    // Wt = transpose(mat3(V)): Wt[c][r] = V[4r + c]
-    float T[6];  // columns 0 and 1 of T = Wt * J
+    // J = mat3(f0 / vz, 0, -(f0 tx) / vz^2,  0, f1 / vz, -(f1 ty) / vz^2,  0, 0, 0)
+    const float j00 = f.focal[0] / vz, j02 = -(f.focal[0] * tx) / (vz * vz);
+    const float j11 = f.focal[1] / vz, j12 = -(f.focal[1] * ty) / (vz * vz);
+    // Wt = transpose(mat3(V)): Wt[c][r] = V[4r + c].  Columns 0 and 1 of
+    // T = Wt * J without the products with J's zero entries (see
+    // scale_rot_to_cov: only signs of zeros can differ).
+    float T[6];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-            T[3 * j + r] = (V[4 * r + 0] * J[3 * j] + V[4 * r + 1] * J[3 * j + 1]) + V[4 * r + 2] * J[3 * j + 2];
+    for (int r = 0; r < 3; ++r) {
+        T[r] = V[4 * r + 0] * j00 + V[4 * r + 2] * j02;
+        T[3 + r] = V[4 * r + 1] * j11 + V[4 * r + 2] * j12;
+    }
     // A = transpose(T) * C3, rows 0..1: A[j][r] = (T[r][0]*C[j].x + T[r][1]*C[j].y) + T[r][2]*C[j].z
     float A[9];
 #pragma unroll
@@ -130,8 +158,8 @@ GPS_HD bool project_gaussian(const FrameConsts& f, float mx, float my, float mz,
     o.cc = c11 + f.dilation;
     o.det = o.ca * o.cc - o.cb * o.cb;
     if (o.det <= 0.0f) return false;
-    o.pmx = (o.px * 0.5f + 0.5f) * (float)f.up_w;
-    o.pmy = (o.py * 0.5f + 0.5f) * (float)f.up_h;
+    o.pmx = pixel_mean(o.px, f.up_w);
+    o.pmy = pixel_mean(o.py, f.up_h);
 
     // compute2DGaussianBounds3DGS (linalg.cuh:251-296)
     const int32_t ex = det_f2i_rz(det_ceil(3.0f * det_sqrt(o.ca)));
@@ -154,6 +182,16 @@ GPS_HD bool project_gaussian(const FrameConsts& f, float mx, float my, float mz,
     o.maxx = t1x * 16;
     o.maxy = t1y * 16;
     return true;
+}
+
+// loadAndTransformGaussian (gaussian_point_splatting.cu:17-65).  Returns false
+// when the Gaussian is rejected.
+GPS_HD bool project_gaussian(const FrameConsts& f, float mx, float my, float mz, float qw, float qx, float qy, float qz,
+                             float sx, float sy, float sz, Projected& o) {
+    Center ce;
+    const bool front = project_center(f, mx, my, mz, ce);
+    o.view_z = ce.vz;
+    return front && project_footprint(f, ce, qw, qx, qy, qz, sx, sy, sz, o);
 }
 
 // SH basis constants (sh.cuh:137-169).

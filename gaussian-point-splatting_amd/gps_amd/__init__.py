@@ -111,7 +111,7 @@ class Stats(C.Structure):
                 ("K", C.c_int32), ("gaussians_per_block", C.c_int32), ("atomics", C.c_uint64),
                 ("num_blocks", C.c_uint64), ("splat_binned", C.c_int32), ("splat_fallback", C.c_int32),
                 ("bin_entries", C.c_uint32), ("bin_jobs", C.c_uint32), ("bin_big", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("culled_groups", C.c_uint64 * 2)]
 
     def as_dict(self) -> dict:
         return {"num_gaussians": int(self.num_gaussians), "valid_gaussians": list(self.valid_gaussians),
@@ -121,7 +121,8 @@ class Stats(C.Structure):
                 "gpb": int(self.gaussians_per_block), "num_blocks": int(self.num_blocks),
                 "atomics": int(self.atomics), "splat_binned": int(self.splat_binned),
                 "splat_fallback": int(self.splat_fallback), "bin_entries": int(self.bin_entries),
-                "bin_jobs": int(self.bin_jobs), "bin_big": int(self.bin_big)}
+                "bin_jobs": int(self.bin_jobs), "bin_big": int(self.bin_big),
+                "culled_groups": list(self.culled_groups)}
 
 
 class SynthParams(C.Structure):
@@ -196,6 +197,7 @@ _sig("gps_event_elapsed_ms", C.c_int32, _vp, _vp, C.POINTER(C.c_float))
 _sig("gps_pass_set_timing", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_set_splat_mode", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_set_counters", C.c_int32, _vp, C.c_int32)
+_sig("gps_pass_set_group_culling", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_timing_summary", C.c_int32, _vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
      C.POINTER(C.c_uint64))
 _sig("gps_selftest_math", C.c_int32, C.c_int32, _vp, _vp, C.c_uint64, _vp)
@@ -587,8 +589,18 @@ class Pass:
     def rejects(self, stream: Optional[Stream] = None) -> list:
         """Per render pass of the last frame, {reason: Gaussians} (counters on
         with set_counters(..., count_rejects=True))."""
-        r = self.debug(DBG_REJECTS, stream).reshape(2, len(REJECT_REASONS))
+        r = self.debug(DBG_REJECTS, stream)[:2 * len(REJECT_REASONS)].reshape(2, len(REJECT_REASONS))
         return [dict(zip(REJECT_REASONS, map(int, row))) for row in r]
+
+    def group_violations(self, stream: Optional[Stream] = None) -> list:
+        """Per render pass of the last frame (counters on with count_rejects):
+        members of groups the group occlusion test skipped that survived every
+        per-Gaussian test.  0 when the test is sound."""
+        return [int(v) for v in self.debug(DBG_REJECTS, stream)[2 * len(REJECT_REASONS):]]
+
+    def set_group_culling(self, enable: bool) -> None:
+        """The group occlusion test (on by default; outputs are the same bits either way)."""
+        check(_lib.gps_pass_set_group_culling(self.handle, 1 if enable else 0), "group culling")
 
     def set_timing(self, capacity: int) -> None:
         check(_lib.gps_pass_set_timing(self.handle, capacity), "set timing")

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GPS_ABI_VERSION 1
+#define GPS_ABI_VERSION 2
 
 typedef enum gps_status {
     GPS_OK = 0,
@@ -144,6 +144,8 @@ typedef struct gps_stats {
     uint32_t bin_jobs;             /* region jobs of the last pass */
     uint32_t bin_big;              /* Gaussians splatted with global atomics (too many regions) */
     uint32_t reserved;
+    uint64_t culled_groups[2];     /* per pass: 64-Gaussian groups the group occlusion test skipped
+                                      (counted with gps_pass_set_counters flags on, else 0) */
 } gps_stats;
 
 /* -------------------------------------------------------------- versioning */
@@ -219,6 +221,14 @@ gps_status gps_pass_set_splat_mode(gps_pass* pass, int32_t mode);
 #define GPS_COUNT_ATOMICS 1
 #define GPS_COUNT_REJECTS 2
 gps_status gps_pass_set_counters(gps_pass* pass, int32_t flags);
+/* Group occlusion test, on by default (enable = 1): in frames with occlusion
+ * culling and a max-only Hi-Z chain (occlusion_culling_min_reduce_count 0), a
+ * conservative test per group of 64 consecutive Gaussians against the pass's
+ * Hi-Z chain lets the preprocess skip groups whose every member the
+ * per-Gaussian Hi-Z test (gaussian_point_splatting.cu:226-242) would reject.
+ * Outputs are the same bits with it on or off.  No reference counterpart (the
+ * reference tests every Gaussian of an unculled block). */
+gps_status gps_pass_set_group_culling(gps_pass* pass, int32_t enable);
 /* Device memory held by the pass (scene not included), and what
  * gps_pass_create would allocate for these settings. */
 uint64_t gps_pass_device_bytes(const gps_pass* pass);
@@ -253,7 +263,9 @@ typedef enum gps_debug_buffer {
                                    blocks tested; valid in the first pass (second pass only);
                                    opacity <= 0; projection rejected (near plane, NDC z, det <= 0);
                                    footprint off screen; small-eigenvalue cull; Hi-Z occluded;
-                                   point count 0; valid */
+                                   point count 0; valid; then uint32[2]: per pass, members of
+                                   groups the group occlusion test marked that survived the
+                                   per-Gaussian tests (0 when the test is sound) */
 } gps_debug_buffer;
 uint64_t gps_pass_debug_size(gps_pass* pass, int32_t which);
 gps_status gps_pass_debug_copy(gps_pass* pass, int32_t which, void* host_dst, uint64_t count, void* stream);
