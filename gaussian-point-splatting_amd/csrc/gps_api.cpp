@@ -115,6 +115,7 @@ struct gps_pass {
     GroupBox* gboxes = nullptr;  // group occlusion test (null: off)
     uint8_t* gocc = nullptr;     // occluded-group bytes (padded to whole words)
     uint32_t* live = nullptr;    // unculled blocks of the pass, listed by the block cull
+    uint8_t* wg_live = nullptr;  // per preprocess workgroup: its block is unculled (this pass)
     GroupView group{};
     int32_t group_culling = 1;
     // splat grid
@@ -531,7 +532,10 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     if (N > 0) {
         chk(dalloc(&p->gboxes, p->group.count, false));
         chk(dalloc(&p->gocc, round_up(p->group.count, 4) + 4, true));
-        if (p->hier_available) chk(dalloc(&p->live, nb, false));
+        if (p->hier_available) {
+            chk(dalloc(&p->live, nb, false));
+            chk(dalloc(&p->wg_live, (N + GPS_PRE_THREADS - 1) / GPS_PRE_THREADS, false));
+        }
         if (s == GPS_OK) launch_group_boxes(scene->view, p->gboxes, p->group.count, nullptr);
     }
     p->group.boxes = p->gboxes;
@@ -610,7 +614,7 @@ uint64_t gps_pass_estimate_bytes(const gps_render_settings* st, const gps_scene*
     if (N > 0 && !st->fully_disable_hierarchical_culling) {
         const uint64_t gpb = st->gaussians_per_block > 0 ? (uint64_t)st->gaussians_per_block
                                                          : (uint64_t)std::max(4.0f, std::floor((float)N / 1e6f));
-        b += (N + gpb - 1) / gpb * 4;
+        b += (N + gpb - 1) / gpb * 4 + (N + GPS_PRE_THREADS - 1) / GPS_PRE_THREADS;
     }
     return b;
 }
@@ -623,7 +627,7 @@ uint64_t gps_pass_device_bytes(const gps_pass* p) {
                  (uint64_t)p->hiz_size * 4 + 2 * p->fb_cap * 8 + p->dbg_indices_cap * 4;
     if (p->hier_available) b += (uint64_t)p->cull.node_count * sizeof(CullNode) + (p->cull.mask_words + 2) * 4ull;
     if (p->gboxes) b += (uint64_t)p->group.count * sizeof(GroupBox) + round_up(p->group.count, 4) + 4;
-    if (p->live) b += (uint64_t)p->cull.node_count * 4;
+    if (p->live) b += (uint64_t)p->cull.node_count * 4 + (N + GPS_PRE_THREADS - 1) / GPS_PRE_THREADS;
     b += std::max<uint64_t>(N, 1) * 16 + 3ull * p->list_cap * 4 + 2ull * (p->list_cap / GPS_TILE + 1) * 4 +
          8ull * 4 * GPS_BIN_HIST + 2ull * 4 * p->job_cap + 32;
     return b;
@@ -653,6 +657,7 @@ void gps_pass_destroy(gps_pass* p) {
     dfree(p->gboxes);
     dfree(p->gocc);
     dfree(p->live);
+    dfree(p->wg_live);
     dfree(p->erect);
     dfree(p->ew);
     dfree(p->big);
@@ -768,6 +773,9 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
     // Group occlusion test: needs a max-only Hi-Z chain (no min-reduced levels).
     const bool group_test = f.occl && p->gboxes && p->group_culling && fp->occlusion_culling_min_reduce_count == 0 &&
                             camera_finite(f);
+    // Blocks that are whole preprocess workgroups: the block cull marks each
+    // workgroup live or not and zeroes the culled blocks' mask words.
+    const bool live_mode = hier && p->wg_live && p->cull.gpb % GPS_PRE_THREADS == 0;
     auto render_pass = [&](bool first) {
         const int slot = first ? 0 : 1;
         const uint32_t frame_seed = (uint32_t)(4 * fp->frame + (first ? 0 : 2));
@@ -775,7 +783,8 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
             stage_mark(p, 1, slot, 0, st);
             if (hier)
                 launch_cull(p->cull, fp->frustum_planes, vpm, p->hiz, f.occl, &p->ctr->culled_blocks[slot],
-                            group_test ? p->live : nullptr, &p->ctr->live_blocks[slot], st);
+                            group_test ? p->live : nullptr, &p->ctr->live_blocks[slot], p->maskA, p->maskB,
+                            &p->ctr->iteration, live_mode ? p->wg_live : nullptr, p->N, st);
             // skipped groups are counted with the diagnostic counters only
             uint32_t* gcount = (p->count_rejects || p->count_atomics) ? &p->ctr->culled_groups[slot] : nullptr;
             if (group_test && hier)  // the groups of the blocks the block cull kept
@@ -800,6 +809,7 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         a.cull = p->cull;
         a.hiz = p->hiz;
         a.gocc = group_test ? p->gocc : nullptr;
+        if (live_mode) a.wg_live = p->wg_live;
         a.rejects = p->count_rejects ? &p->ctr->rejects[0][0] : nullptr;
         stage_mark(p, 2, slot, 0, st);
         launch_preprocess(a, st);

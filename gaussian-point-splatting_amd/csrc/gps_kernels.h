@@ -18,6 +18,9 @@ struct PreprocessArgs {
     CullView cull;
     HiZView hiz;
     const uint8_t* gocc = nullptr;  // occluded-group bytes of this pass (group cull), or null
+    // one byte per workgroup, 0 when the block cull culled its block (and
+    // zeroed its mask words), or null; needs gaussians_per_block a multiple of 256
+    const uint8_t* wg_live = nullptr;
     uint32_t* rejects = nullptr;  // 2 x 9 reject-reason counters + 2 group checks (diagnostic), or null
 };
 
@@ -143,10 +146,14 @@ void launch_batch_bin_big(const BatchArgs& a, hipStream_t st);
 // Fills the framebuffer with `value`; also zeroes n0 words at z0 and n1 at z1.
 void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st, uint32_t* z0 = nullptr,
                   uint32_t n0 = 0, uint32_t* z1 = nullptr, uint32_t n1 = 0);
-// `live` (or null): the unculled blocks are listed there for the group
-// occlusion test, their count in *live_count (zeroed by the frame's clear).
+// `live` (or null): the unculled blocks are listed there, their count in
+// *live_count (zeroed by the frame's clear).  `wg_live` (or null): one byte
+// per 256-Gaussian preprocess workgroup, 0 when its block is culled (blocks
+// must be whole workgroups), and the culled blocks' valid-mask words zeroed
+// in the preprocess's current mask (maskA / maskB by *iteration).
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
-                 uint32_t* culled_count, uint32_t* live, uint32_t* live_count, hipStream_t st);
+                 uint32_t* culled_count, uint32_t* live, uint32_t* live_count, uint32_t* maskA, uint32_t* maskB,
+                 const uint32_t* iteration, uint8_t* wg_live, uint32_t N, hipStream_t st);
 void launch_preprocess(const PreprocessArgs& a, hipStream_t st);
 void launch_distribute(const DistributeArgs& a, hipStream_t st);
 void launch_splat(const SplatArgs& a, hipStream_t st);

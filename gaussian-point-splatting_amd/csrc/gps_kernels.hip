@@ -327,9 +327,21 @@ __device__ __forceinline__ uint32_t wg_append(uint32_t* counter, bool pred) {
 
 // `live` (or null): the unculled blocks are listed there, their count in
 // *live_count (the group occlusion test runs over their groups only).
+// `mz.wg_live` (or null): for a preprocess whose workgroups lie in single
+// blocks (gpb a multiple of its 256 Gaussians), the cull writes each
+// workgroup's live byte (0: its block is culled) and zeroes the valid-mask
+// words of the culled blocks in the preprocess's current mask (maskA / maskB
+// by the ping-pong index).
+struct MaskZero {
+    uint32_t* maskA;
+    uint32_t* maskB;
+    const uint32_t* iteration;
+    uint8_t* wg_live;
+    uint32_t N;
+};
 __global__ __launch_bounds__(256) void cull_kernel(CullView cv, Planes pl, Mat4 vp, HiZView h, int occl,
                                                    uint32_t* __restrict__ culled_count, uint32_t* __restrict__ live,
-                                                   uint32_t* __restrict__ live_count) {
+                                                   uint32_t* __restrict__ live_count, MaskZero mz) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     bool culled = false;
     if (idx < cv.node_count) {
@@ -392,6 +404,27 @@ __global__ __launch_bounds__(256) void cull_kernel(CullView cv, Planes pl, Mat4 
         const bool keep = idx < cv.node_count && !culled;
         const uint32_t k = wg_append(live_count, keep);
         if (keep) live[k] = idx;
+    }
+    if (mz.wg_live) {
+        if (idx < cv.node_count) {
+            const uint32_t wpb = cv.gpb / 256u, wg0 = idx * wpb;  // the block's preprocess workgroups
+            const uint32_t nwg = min(wg0 + wpb, (mz.N + 255u) / 256u) - wg0;
+            for (uint32_t w = 0; w < nwg; ++w) mz.wg_live[wg0 + w] = culled ? 0u : 1u;
+        }
+        // The wave's 64 blocks own consecutive mask words (gpb / 32 each): the
+        // lanes zero the culled blocks' words in consecutive 16-byte pieces.
+        uint32_t* m = (*mz.iteration & 1u) ? mz.maskB : mz.maskA;
+        const uint32_t q = cv.gpb >> 7;  // 16-byte pieces per block
+        const uint32_t blk0 = idx - (uint32_t)lane, words = (mz.N + 31u) >> 5;
+        for (uint32_t c = (uint32_t)lane; c < 64u * q; c += 64u) {
+            const uint32_t bl = c / q, w = (blk0 + bl) * (cv.gpb >> 5) + (c - bl * q) * 4u;
+            if (((b >> bl) & 1ull) && blk0 + bl < cv.node_count) {
+                if (w + 4u <= words)
+                    *reinterpret_cast<uint4*>(m + w) = make_uint4(0u, 0u, 0u, 0u);
+                else
+                    for (uint32_t j = w; j < words; ++j) m[j] = 0u;
+            }
+        }
     }
 }
 
@@ -635,12 +668,15 @@ __device__ __forceinline__ void count_reasons(uint32_t* s_rej, uint32_t reason, 
     }
 }
 
+// The preprocess of the workgroup's Gaussians [g0, g0 + GPS_PRE_THREADS).
+// `live`: they lie in a block the block cull kept (no block lookup).
 template <int FORMAT, int DEG, bool STATS>
-__global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
-    DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
-    const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
-    uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h,
-    const uint8_t* __restrict__ gocc_bytes, uint32_t* __restrict__ rejects) {
+__device__ __forceinline__ void preprocess_wg(const DevScene& s, const FrameConsts& f, uint32_t* __restrict__ mask_cur,
+                                              const uint32_t* __restrict__ mask_prev, uint32_t* __restrict__ weights,
+                                              SplatRecord* __restrict__ recs, uint32_t* __restrict__ tile_sum,
+                                              uint32_t* __restrict__ tile_cnt, const CullView& cv, const HiZView& h,
+                                              const uint8_t* __restrict__ gocc_bytes, uint32_t* __restrict__ rejects,
+                                              uint32_t g0, bool live) {
     __shared__ PreItem s_item[GPS_PRE_THREADS];
     // list 2 reuses list 1's storage (read into registers before a barrier):
     // 16 KB of LDS per workgroup, eight workgroups per CU
@@ -649,14 +685,9 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
     __shared__ uint32_t s_count[GPS_PRE_THREADS];
     __shared__ uint32_t s_n1, s_n2;
     const int tid = threadIdx.x;
-    const uint32_t g0 = blockIdx.x * GPS_PRE_THREADS;
     const uint32_t g = g0 + (uint32_t)tid;
-    // workload_distributor.cu:32-40: current = A on even iterations.
-    const uint32_t it = *iteration;
-    uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
-    const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
     WgBlocks wb{0u, 0u, false};
-    if (f.hier && wg_all_culled(cv, g0, min((uint32_t)GPS_PRE_THREADS, s.N - g0), wb)) {
+    if (!live && f.hier && wg_all_culled(cv, g0, min((uint32_t)GPS_PRE_THREADS, s.N - g0), wb)) {
         // Every Gaussian here is culled: only the valid-mask words (zero)
         // are written; weights stay untouched, the tile sums get nothing.
         if (tid < GPS_PRE_THREADS / 32 && g0 + 32u * (uint32_t)tid < s.N) mask_cur[(g0 >> 5) + (uint32_t)tid] = 0u;
@@ -829,6 +860,25 @@ __global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
         reinterpret_cast<uint4*>(recs + q.g)[3] = make_uint4(q.bmax, (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
     }
     preprocess_epilogue(s_count[tid], g, g0, s.N, mask_cur, tile_sum, tile_cnt);
+}
+
+// One workgroup per GPS_PRE_THREADS Gaussians.  `wg_live` (or null): one
+// byte per workgroup written by the block cull, 0 when its block is culled;
+// the cull also zeroed such a workgroup's valid-mask words, so it returns
+// after that one load (most of the grid in culled frames).
+template <int FORMAT, int DEG, bool STATS>
+__global__ __launch_bounds__(GPS_PRE_THREADS) void preprocess_kernel(
+    DevScene s, FrameConsts f, uint32_t* __restrict__ maskA, uint32_t* __restrict__ maskB,
+    const uint32_t* __restrict__ iteration, uint32_t* __restrict__ weights, SplatRecord* __restrict__ recs,
+    uint32_t* __restrict__ tile_sum, uint32_t* __restrict__ tile_cnt, CullView cv, HiZView h,
+    const uint8_t* __restrict__ gocc_bytes, uint32_t* __restrict__ rejects, const uint8_t* __restrict__ wg_live) {
+    if (wg_live && wg_live[blockIdx.x] == 0u) return;
+    // workload_distributor.cu:32-40: current = A on even iterations.
+    const uint32_t it = *iteration;
+    uint32_t* mask_cur = (it & 1u) ? maskB : maskA;
+    const uint32_t* mask_prev = (it & 1u) ? maskA : maskB;
+    preprocess_wg<FORMAT, DEG, STATS>(s, f, mask_cur, mask_prev, weights, recs, tile_sum, tile_cnt, cv, h, gocc_bytes,
+                                      rejects, blockIdx.x * GPS_PRE_THREADS, wg_live != nullptr);
 }
 
 // The same outputs with preprocess_one per lane and no work lists: cheaper
@@ -2596,11 +2646,11 @@ static void launch_pre(const PreprocessArgs& a, hipStream_t st) {
     if (a.rejects)  // reject-reason counters: the work-list kernel (same outputs) with counting
         hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG, true>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
                            a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
-                           a.cull, a.hiz, a.gocc, a.rejects);
+                           a.cull, a.hiz, a.gocc, a.rejects, a.wg_live);
     else if (a.f.hier || a.f.occl)  // culled frames: work lists keep the later phases on full waves
         hipLaunchKernelGGL((preprocess_kernel<FORMAT, DEG, false>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
                            a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
-                           a.cull, a.hiz, a.gocc, nullptr);
+                           a.cull, a.hiz, a.gocc, nullptr, a.wg_live);
     else
         hipLaunchKernelGGL((preprocess_flat_kernel<FORMAT, DEG>), dim3(blocks), dim3(GPS_PRE_THREADS), 0, st,
                            a.scene, a.f, a.maskA, a.maskB, a.iteration, a.weights, a.recs, a.tile_sum, a.tile_cnt,
@@ -2642,13 +2692,15 @@ void launch_clear(uint64_t* fb, uint64_t words, uint64_t value, hipStream_t st, 
 }
 
 void launch_cull(const CullView& cv, const float planes[24], const Mat4& vp, const HiZView& h, int occl,
-                 uint32_t* culled_count, uint32_t* live, uint32_t* live_count, hipStream_t st) {
+                 uint32_t* culled_count, uint32_t* live, uint32_t* live_count, uint32_t* maskA, uint32_t* maskB,
+                 const uint32_t* iteration, uint8_t* wg_live, uint32_t N, hipStream_t st) {
     if (cv.node_count == 0) return;
     Planes pl;
     for (int i = 0; i < 24; ++i) pl.p[i] = planes[i];
     const uint32_t blocks = (cv.node_count + 255) / 256;
+    const MaskZero mz{maskA, maskB, iteration, wg_live, N};
     hipLaunchKernelGGL(cull_kernel, dim3(blocks), dim3(256), 0, st, cv, pl, vp, h, occl, culled_count, live,
-                       live_count);
+                       live_count, mz);
 }
 
 // One tile-scan workgroup per 8192 tiles, at least one (it publishes the
