@@ -342,6 +342,7 @@ def summarize_trace(traces, counters=(), gap_s=0.5, traffic_out=None, window_nam
         stage_of = {"preprocess_kernel": "preprocess", "splat_kernel": "splat", "distribute_kernel": "distribute",
                     "tile_scan_kernel": "distribute", "combine_staged_kernel": "combine", "combine_kernel": "combine",
                     "hiz_build_kernel": "hiz", "hiz_top_kernel": "hiz", "cull_kernel": "cull", "clear_kernel": "clear",
+                    "group_cull_live_kernel": "cull", "group_cull_kernel": "cull",
                     "taa_kernel": "taa", "progressive_kernel": "resolve"}
         traffic = {}
         for (k, c), (v, n) in means.items():
