@@ -208,8 +208,11 @@ __device__ __forceinline__ void sh_sloan_l3(float x, float y, float z, float o[1
 }
 
 // SH colour of Gaussian g.
+// Compact formats (kRecord) read one 64-byte SH record: load() requests it,
+// eval_words() evaluates it, so a caller can issue the load early.
 template <int FORMAT, int DEG> struct ShEval;
 template <int DEG> struct ShEval<GPS_FORMAT_FP32, DEG> {
+    static constexpr bool kRecord = false;
     __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch)
@@ -217,10 +220,10 @@ template <int DEG> struct ShEval<GPS_FORMAT_FP32, DEG> {
     }
 };
 template <int DEG> struct ShEval<GPS_FORMAT_HALF_FP8, DEG> {
-    __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
+    static constexpr bool kRecord = true;
+    __device__ __forceinline__ static void load(const DevScene& s, uint32_t g, uint32_t words[16]) {
         const uint4* r = s.shrec + 4 * (uint64_t)g;
         const uint4 w0 = r[0];
-        uint32_t words[16];
         words[0] = w0.x; words[1] = w0.y; words[2] = w0.z; words[3] = w0.w;
         if (DEG > 0) {
             const uint4 w1 = r[1];
@@ -234,6 +237,14 @@ template <int DEG> struct ShEval<GPS_FORMAT_HALF_FP8, DEG> {
             const uint4 w3 = r[3];
             words[12] = w3.x; words[13] = w3.y; words[14] = w3.z; words[15] = w3.w;
         }
+    }
+    __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
+        uint32_t words[16];
+        load(s, g, words);
+        eval_words(s, words, x, y, z, rgb);
+    }
+    __device__ __forceinline__ static void eval_words(const DevScene&, const uint32_t words[16], float x, float y, float z,
+                                                      float rgb[3]) {
         const float scale = __builtin_bit_cast(float, words[2]);
         const float dc[3] = {half_bits_to_float(words[0] & 0xFFFFu), half_bits_to_float(words[0] >> 16),
                              half_bits_to_float(words[1] & 0xFFFFu)};
@@ -253,9 +264,9 @@ template <int DEG> struct ShEval<GPS_FORMAT_HALF_FP8, DEG> {
 // the scene's degree, summed in order from 0; coefficient i > 0 of channel c
 // is mix(min_sh, max_sh, byte * (1/255)) (CompactSH::get, sh.cuh:41-47).
 template <int DEG> struct ShEval<GPS_FORMAT_QUANTIZED, DEG> {
-    __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
+    static constexpr bool kRecord = true;
+    __device__ __forceinline__ static void load(const DevScene& s, uint32_t g, uint32_t words[16]) {
         const uint4* r = s.shrec + 4 * (uint64_t)g;
-        uint32_t words[16];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint4 v = r[q];
@@ -264,6 +275,14 @@ template <int DEG> struct ShEval<GPS_FORMAT_QUANTIZED, DEG> {
             words[4 * q + 2] = v.z;
             words[4 * q + 3] = v.w;
         }
+    }
+    __device__ __forceinline__ static void eval(const DevScene& s, uint32_t g, float x, float y, float z, float rgb[3]) {
+        uint32_t words[16];
+        load(s, g, words);
+        eval_words(s, words, x, y, z, rgb);
+    }
+    __device__ __forceinline__ static void eval_words(const DevScene& s, const uint32_t words[16], float x, float y,
+                                                      float z, float rgb[3]) {
         float e[16];
         sh_sloan_l3(x, y, z, e);
         const float* mn = s.qbounds + 12;
@@ -567,25 +586,20 @@ __device__ __forceinline__ uint32_t preprocess_one(const DevScene& s, const Fram
     return cnt;
 }
 
-// The single-view preprocess runs the steps of preprocess_one in three
-// phases with the surviving Gaussians repacked onto the first lanes of the
-// workgroup between phases (LDS work lists), so the counting and the colour
-// steps run on full waves instead of the few lanes that need them:
+// The single-view preprocess runs the steps of preprocess_one in two phases
+// with the Gaussians that survive the first repacked onto the first lanes of
+// the workgroup (an LDS work list), so the counting and the colour steps run
+// on full waves instead of the few lanes that need them:
 //   A  every lane: block cull, previous-mask skip, opacity, projection,
 //      bounds, small-eigenvalue cull, Hi-Z test -> list 1
-//   B  list 1: importance, Poisson count, / K, cap; weight; record geometry
-//      for counts > 0 -> list 2
-//   C  list 2: SH colour, packed depth | colour word of the record.
+//   B  list 1: importance, Poisson count, / K, cap; weight; for counts > 0
+//      the record and its packed depth | colour word (SH colour).
 // Per Gaussian the arithmetic is exactly preprocess_one's.
 struct PreItem {
     uint32_t g;
     float op, view_z, pmx, pmy, ca, cb, cc, det;
     uint32_t bmin, bmax;
     float mx, my, mz;  // world mean (the colour's view direction)
-};
-struct ColourItem {
-    uint32_t g, depth, bmax;
-    float mx, my, mz;
 };
 __device__ __forceinline__ void push_item(PreItem* s_item, uint32_t* s_n, uint32_t g, float op, const Projected& pr,
                                           float mx, float my, float mz) {
@@ -678,12 +692,9 @@ __device__ __forceinline__ void preprocess_wg(const DevScene& s, const FrameCons
                                               const uint8_t* __restrict__ gocc_bytes, uint32_t* __restrict__ rejects,
                                               uint32_t g0, bool live) {
     __shared__ PreItem s_item[GPS_PRE_THREADS];
-    // list 2 reuses list 1's storage (read into registers before a barrier):
-    // 16 KB of LDS per workgroup, eight workgroups per CU
-    static_assert(sizeof(ColourItem) <= sizeof(PreItem), "list 2 fits in list 1");
-    ColourItem* const s_col = reinterpret_cast<ColourItem*>(s_item);
+    // 15 KB of LDS per workgroup
     __shared__ uint32_t s_count[GPS_PRE_THREADS];
-    __shared__ uint32_t s_n1, s_n2;
+    __shared__ uint32_t s_n1;
     const int tid = threadIdx.x;
     const uint32_t g = g0 + (uint32_t)tid;
     WgBlocks wb{0u, 0u, false};
@@ -713,7 +724,6 @@ __device__ __forceinline__ void preprocess_wg(const DevScene& s, const FrameCons
     __shared__ uint32_t s_rej[STATS ? REJ_COUNT + 1 : 1];
     if (tid == 0) {
         s_n1 = 0u;
-        s_n2 = 0u;
     }
     if (STATS && tid <= REJ_COUNT) s_rej[tid] = 0u;
     s_count[tid] = 0u;
@@ -801,13 +811,16 @@ __device__ __forceinline__ void preprocess_wg(const DevScene& s, const FrameCons
         if ((tid & 63) == 0 && nv) atomicAdd(s_rej + REJ_COUNT, nv);  // survivors of occluded groups
     }
     __syncthreads();
-    // Phase B
+    // Phase B: importance, count, record and, for counts > 0, colour.  The
+    // compact formats' 64-byte SH record is requested before the count, so
+    // its latency overlaps the count's arithmetic.
     const bool in1 = (uint32_t)tid < s_n1;
     PreItem q;
     if (in1) q = s_item[tid];
-    __syncthreads();  // list 1 is in registers before list 2 overwrites it
     uint32_t cnt_b = 0u;  // this lane's count (STATS)
     if (in1) {
+        uint32_t shw[16];
+        if constexpr (ShEval<FORMAT, DEG>::kRecord) ShEval<FORMAT, DEG>::load(s, q.g, shw);
         u32x2 seed = make_seed(q.g, f.frame_seed);
         const float s_det = det_sqrt(q.det);
         const float dl = dilog(q.op);
@@ -826,13 +839,16 @@ __device__ __forceinline__ void preprocess_wg(const DevScene& s, const FrameCons
                               __float_as_uint(q.ca * inv_det));
             r[2] = make_uint4(__float_as_uint(q.op), __float_as_uint(corrected_inv_alpha(q.op)), __float_as_uint(dl),
                               q.bmin);
-            const uint32_t k = atomicAdd(&s_n2, 1u);
-            s_col[k].g = q.g;
-            s_col[k].depth = (uint32_t)depth_quantize(q.view_z, f.near_plane, f.far_plane);
-            s_col[k].bmax = q.bmax;
-            s_col[k].mx = q.mx;
-            s_col[k].my = q.my;
-            s_col[k].mz = q.mz;
+            float dx, dy, dz;
+            view_dir(f, q.mx, q.my, q.mz, dx, dy, dz);
+            float rgb[3];
+            if constexpr (ShEval<FORMAT, DEG>::kRecord)
+                ShEval<FORMAT, DEG>::eval_words(s, shw, dx, dy, dz, rgb);
+            else
+                ShEval<FORMAT, DEG>::eval(s, q.g, dx, dy, dz, rgb);
+            const uint64_t packed = ((uint64_t)(uint32_t)depth_quantize(q.view_z, f.near_plane, f.far_plane)
+                                     << GPS_DEPTH_SHIFT) | pack_color36(rgb[0], rgb[1], rgb[2]);
+            r[3] = make_uint4(q.bmax, (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
             s_count[q.g - g0] = cnt;
         }
         weights[q.g] = cnt;
@@ -849,16 +865,6 @@ __device__ __forceinline__ void preprocess_wg(const DevScene& s, const FrameCons
     __syncthreads();
     if (STATS && tid < REJ_COUNT && s_rej[tid]) atomicAdd(rejects + (f.only_prev ? REJ_COUNT : 0) + tid, s_rej[tid]);
     if (STATS && tid == REJ_COUNT && s_rej[tid]) atomicAdd(rejects + 2 * REJ_COUNT + (f.only_prev ? 1 : 0), s_rej[tid]);
-    // Phase C
-    if ((uint32_t)tid < s_n2) {
-        const ColourItem q = s_col[tid];
-        float dx, dy, dz;
-        view_dir(f, q.mx, q.my, q.mz, dx, dy, dz);
-        float rgb[3];
-        ShEval<FORMAT, DEG>::eval(s, q.g, dx, dy, dz, rgb);
-        const uint64_t packed = ((uint64_t)q.depth << GPS_DEPTH_SHIFT) | pack_color36(rgb[0], rgb[1], rgb[2]);
-        reinterpret_cast<uint4*>(recs + q.g)[3] = make_uint4(q.bmax, (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
-    }
     preprocess_epilogue(s_count[tid], g, g0, s.N, mask_cur, tile_sum, tile_cnt);
 }
 
