@@ -894,9 +894,12 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
             ba.job_cap = p->job_cap;
             ba.job_points = GPS_BIN_JOB_POINTS;
             ba.cus = p->cus;
+            // the automatic choice also tests the pass's density on the device
+            ba.dense_points =
+                p->splat_mode == GPS_SPLAT_BINNED ? 0xFFFFFFFFu : binned_dense_points(f.up_w, f.up_h, f.Ki);
             ba.atomics_count = p->count_atomics ? &p->ctr->atomics : nullptr;
             launch_binned_splat(ba, st);
-            sa.only_if = p->bin_state + 3;  // overflow: the point-parallel splat takes the pass
+            sa.only_if = p->bin_state + 3;  // stood down: the point-parallel splat takes the pass
         } else {
             sa.only_if = nullptr;
         }
@@ -1090,7 +1093,7 @@ gps_status gps_pass_stats(gps_pass* p, gps_stats* out, void* stream) {
         uint32_t bs[8];
         GPS_HIP(hipMemcpy(bs, p->bin_state, sizeof(bs), hipMemcpyDeviceToHost));
         out->splat_binned = 1;
-        out->splat_fallback = bs[3] ? 1 : 0;
+        out->splat_fallback = (int32_t)bs[3];
         out->bin_entries = bs[0];
         out->bin_jobs = bs[1];
         out->bin_big = bs[2];

@@ -348,6 +348,8 @@ gps_status gps_batch_render(gps_batch* b, int32_t nviews, const gps_frame_params
             regions + g.nregions <= b->bin_regions_cap) {
             bv.bg = g;
             bv.rbase = regions;
+            bv.dense_points =
+                b->splat_mode == GPS_SPLAT_BINNED ? 0xFFFFFFFFu : binned_dense_points(bv.f.up_w, bv.f.up_h, bv.f.Ki);
             regions += g.nregions;
             max_view_regions = std::max(max_view_regions, g.nregions);
         } else {
@@ -474,8 +476,9 @@ gps_status gps_batch_view_stats(gps_batch* b, int32_t view, gps_stats* out, void
     if (bv.bg.nregions) {
         uint32_t st[8];
         if (hipMemcpy(st, b->bin.state, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return GPS_ERR_HIP;
-        out->splat_binned = st[3] ? 0 : 1;
-        out->splat_fallback = st[3] ? 1 : 0;
+        const bool dense = bv.points > bv.dense_points;
+        out->splat_binned = st[3] || dense ? 0 : 1;
+        out->splat_fallback = st[3] ? 1 : dense ? 2 : 0;
         out->bin_entries = st[0];  // whole batch
         out->bin_jobs = st[1];
         out->bin_big = st[2];

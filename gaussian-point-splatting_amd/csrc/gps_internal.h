@@ -132,6 +132,7 @@ struct BatchView {
     uint32_t budget;        // point budget of the view
     BinGeom bg;             // binned splat regions (bg.nregions == 0: point-parallel splat)
     uint32_t rbase;         // first region of the view in the batch's region numbering
+    uint32_t dense_points;  // more points than this: the view is splatted point-parallel
     // written on the device
     uint32_t base;          // first point of the view in the batch's point list
     uint32_t points;        // points splatted (min(total, budget))
@@ -200,6 +201,14 @@ static_assert((GPS_SPLAT_SLOTS & (GPS_SPLAT_SLOTS - 1)) == 0, "slots: a power of
 // they save (over the whole 100M fly-through at 1 sample per point: 2.42 ms
 // binned against 1.40 ms point-parallel per frame).
 #define GPS_BIN_AUTO_SAMPLES 8
+// ... and while the pass has at most GPS_BIN_DENSE_SAMPLES samples per
+// framebuffer word (points * K <= 8 * up_w * up_h, tested on the device once
+// the point count is known).  Denser frames reject most samples at the
+// point-parallel splat's pre-read, which the binned form cannot skip:
+// 1920x1080 ss 2, 4 spp, K 4 requests measured 0.5-1.05 ms binned vs
+// 0.6-1.9 ms point-parallel at 2.4-4.8 samples per word, 2.44 vs 2.70 at
+// 7.5, 2.17 vs 1.6 at 15, 14.6 vs 9.1 at 250.
+#define GPS_BIN_DENSE_SAMPLES 8u
 
 // Distributor tile: 256 threads x 16 Gaussians.
 #define GPS_TILE_SHIFT 12

@@ -80,12 +80,17 @@ struct BinnedArgs {
     uint32_t *list, *lw, *lpre;
     uint32_t *tsum, *toff;
     uint32_t* big;
-    uint32_t* state;      // 8 words: list length, jobs, big count, overflow, job cursor, big cursor
+    uint32_t* state;      // 8 words: list length, jobs, big count, stand-down, job cursor, big cursor
     uint32_t list_cap, job_cap, job_points;
     uint32_t cus;
+    uint32_t dense_points;  // more points than this: the binned kernels stand down (state[3] = 2)
     uint32_t* atomics_count;
 };
 bool binned_geometry(int up_w, int up_h, int spp, BinGeom* g);
+// Points above which a frame of up_w x up_h pixels has more than
+// GPS_BIN_DENSE_SAMPLES samples per framebuffer word (a point writes K * spp
+// samples into up_w * up_h * spp words).
+uint32_t binned_dense_points(int up_w, int up_h, int K);
 void launch_binned_splat(const BinnedArgs& a, hipStream_t st);
 
 // Binned splat buffers of a batch: entries are the batch's compacted entries,

@@ -1422,8 +1422,10 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
 // global atomicMin.  Gaussians overlapping more than GPS_BIN_MAX_REGIONS
 // regions are splatted afterwards with global atomics (no replay).  Since
 // min is order-independent the framebuffer is bit-identical to the
-// point-parallel splat.  If the region lists overflow their allocation the
-// binned kernels stand down and the point-parallel splat runs instead.
+// point-parallel splat.  If the region lists overflow their allocation
+// (state[3] = 1), or the pass has more points than dense_points (state[3] = 2,
+// the automatic choice's density test), the binned kernels stand down and the
+// point-parallel splat runs instead.
 
 __device__ __forceinline__ uint32_t bin_region_words(const BinGeom& g) {
     return (uint32_t)(g.rtx * g.rty) * GPS_TILE_WORDS * (uint32_t)g.spp;
@@ -1492,9 +1494,11 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const SplatRecord* __res
                                                         const PassCounters* __restrict__ ctr, BinGeom g,
                                                         uint2* __restrict__ erect, uint32_t* __restrict__ ew,
                                                         uint32_t* __restrict__ reg_cnt, uint32_t* __restrict__ reg_pts,
-                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ state) {
+                                                        uint32_t* __restrict__ big, uint32_t* __restrict__ state,
+                                                        uint32_t dense_points) {
     extern __shared__ uint32_t s_dyn[];  // 2 * nregions words
     const uint32_t V = ctr->valid, P = ctr->points, T = ctr->total;
+    if (P > dense_points) return;  // too dense to bin: bin_scan_kernel stands the binned kernels down
     const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = min(V, blockIdx.x * per), r1 = min(V, r0 + per);
     bin_count_range(recs, cidx, coff, g, r0, r1, V, T, P, 0u, erect, ew, reg_cnt, reg_pts, big, state, s_dyn,
@@ -1513,7 +1517,8 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinGeom g, uint32_t* __r
                                                         uint32_t* __restrict__ reg_jobs,
                                                         uint32_t* __restrict__ job_reg, uint32_t* __restrict__ job_p0,
                                                         uint32_t* __restrict__ state, uint32_t list_cap,
-                                                        uint32_t job_points, uint32_t job_cap) {
+                                                        uint32_t job_points, uint32_t job_cap,
+                                                        const PassCounters* __restrict__ ctr, uint32_t dense_points) {
     __shared__ uint32_t s_a[16], s_b[16], s_c[16];
     __shared__ uint32_t s_ca, s_cb, s_cc;
     __shared__ unsigned long long s_wide_e, s_wide_p;  // 64-bit totals: list and point overflow
@@ -1575,7 +1580,8 @@ __global__ __launch_bounds__(1024) void bin_scan_kernel(BinGeom g, uint32_t* __r
     if (tid == 0) {
         state[0] = s_ca;
         state[1] = min(s_cc, job_cap);
-        state[3] = (s_wide_e > list_cap || s_cc > job_cap || s_wide_p > 0xFFFFFFFFull) ? 1u : 0u;
+        const bool dense = ctr && ctr->points > dense_points;
+        state[3] = dense ? 2u : (s_wide_e > list_cap || s_cc > job_cap || s_wide_p > 0xFFFFFFFFull) ? 1u : 0u;
         state[4] = 0u;
         state[5] = 0u;
     }
@@ -1937,8 +1943,9 @@ __global__ __launch_bounds__(256) void bin_big_kernel(const SplatRecord* __restr
 // list scans and the job queue are the single-frame kernels over the whole
 // batch; counting and scattering take a grid row per view (grid.y = view)
 // with the view's own histogram in LDS, and a job finds its view by its
-// region.  Views whose bg.nregions is 0 are left to the point-parallel batch
-// splat.
+// region.  Views whose bg.nregions is 0, or with more points than their
+// dense_points, are left to the point-parallel batch splat (their regions stay
+// empty).
 
 __global__ __launch_bounds__(256) void batch_bin_count_kernel(const SplatRecord* __restrict__ recs,
                                                               const uint32_t* __restrict__ cidx,
@@ -1951,7 +1958,7 @@ __global__ __launch_bounds__(256) void batch_bin_count_kernel(const SplatRecord*
     extern __shared__ uint32_t s_dyn[];  // 2 * the largest view's regions
     const BatchView& bv = views[blockIdx.y];
     const BinGeom g = bv.bg;
-    if (g.nregions == 0u) return;
+    if (g.nregions == 0u || bv.points > bv.dense_points) return;
     const uint32_t V = bv.valid, eb = bv.ebase;
     const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = eb + min(V, blockIdx.x * per), r1 = eb + min(V, blockIdx.x * per + per);
@@ -1969,7 +1976,7 @@ __global__ __launch_bounds__(256) void batch_bin_scatter_kernel(const BatchView*
     if (state[3]) return;
     const BatchView& bv = views[blockIdx.y];
     const BinGeom g = bv.bg;
-    if (g.nregions == 0u) return;
+    if (g.nregions == 0u || bv.points > bv.dense_points) return;
     const uint32_t V = bv.valid, eb = bv.ebase;
     const uint32_t per = (V + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = eb + min(V, blockIdx.x * per), r1 = eb + min(V, blockIdx.x * per + per);
@@ -2536,7 +2543,8 @@ __global__ __launch_bounds__(256) void batch_views_kernel(BatchView* __restrict_
 // Persistent splat over the points of all views (same schedule as
 // splat_kernel, THREADS x PPT points per chunk).  Point p of view v is point
 // p - base(v) of that view's own frame.  Views with binned regions are
-// skipped unless the binned kernels stood down (bin_state[3], list overflow).
+// skipped unless the binned kernels stood down (bin_state[3], list overflow)
+// or the view is too dense to bin (points > dense_points).
 template <int THREADS, int PPT>
 __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord* __restrict__ recs,
                                                               const uint32_t* __restrict__ cidx,
@@ -2625,7 +2633,7 @@ __global__ __launch_bounds__(THREADS) void batch_splat_kernel(const SplatRecord*
                         vup_w = bv.f.up_w;
                         vunb = bv.f.unbiased;
                         voff = bv.fb_offset;
-                        vskip = bin_ran && bv.bg.nregions != 0u;
+                        vskip = bin_ran && bv.bg.nregions != 0u && bv.points <= bv.dense_points;
                     }
                     if (!vskip) rec = load_record(recs, g);
                     last = -1;
@@ -2928,7 +2936,7 @@ void launch_batch_binned(const BatchArgs& a, hipStream_t st) {
     all.nregions = b.nregions;
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, all, b.reg_cnt, b.reg_pts, b.reg_ecnt, b.reg_ptot,
                        b.reg_eoff, b.reg_poff, b.reg_cursor, b.reg_jobs, b.job_reg, b.job_p0, b.state, b.list_cap,
-                       b.job_points, b.job_cap);
+                       b.job_points, b.job_cap, (const PassCounters*)nullptr, 0xFFFFFFFFu);
     hipLaunchKernelGGL(batch_bin_scatter_kernel, dim3(gx, a.nviews), dim3(256), hist, st, (const BatchView*)a.views,
                        (const uint2*)b.erect, (const uint32_t*)b.ew, b.reg_cursor, b.list, b.lw,
                        (const uint32_t*)b.state);
@@ -2962,10 +2970,10 @@ void launch_binned_splat(const BinnedArgs& a, hipStream_t st) {
     const uint32_t grid = a.cus * 4u;
     const size_t hist = 2u * a.g.nregions * sizeof(uint32_t);
     hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), hist, st, a.recs, a.cidx, a.coff, a.ctr, a.g, a.erect,
-                       a.ew, a.reg_cnt, a.reg_pts, a.big, a.state);
+                       a.ew, a.reg_cnt, a.reg_pts, a.big, a.state, a.dense_points);
     hipLaunchKernelGGL(bin_scan_kernel, dim3(1), dim3(1024), 0, st, a.g, a.reg_cnt, a.reg_pts, a.reg_ecnt, a.reg_ptot,
                        a.reg_eoff, a.reg_poff, a.reg_cursor, a.reg_jobs, a.job_reg, a.job_p0, a.state, a.list_cap,
-                       a.job_points, a.job_cap);
+                       a.job_points, a.job_cap, a.ctr, a.dense_points);
     hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), hist, st, a.ctr, a.g, (const uint2*)a.erect,
                        (const uint32_t*)a.ew, a.reg_cursor, a.list, a.lw, (const uint32_t*)a.state);
     hipLaunchKernelGGL(bin_tile_sum_kernel, dim3(grid), dim3(256), 0, st, (const uint32_t*)a.lw,
@@ -2988,6 +2996,11 @@ void launch_binned_splat(const BinnedArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(bin_big_kernel, dim3(grid), dim3(256), 0, st, a.recs, a.cidx, a.coff, (const uint32_t*)a.ew,
                        (const uint32_t*)a.big, a.state, reinterpret_cast<unsigned long long*>(a.fb), a.sc,
                        a.atomics_count);
+}
+
+uint32_t binned_dense_points(int up_w, int up_h, int K) {
+    const uint64_t p = (uint64_t)GPS_BIN_DENSE_SAMPLES * (uint64_t)up_w * (uint64_t)up_h / (uint64_t)(K > 0 ? K : 1);
+    return p < 0xFFFFFFFFull ? (uint32_t)p : 0xFFFFFFFFu;
 }
 
 bool binned_geometry(int up_w, int up_h, int spp, BinGeom* g) {

@@ -139,7 +139,8 @@ typedef struct gps_stats {
     uint64_t atomics;              /* framebuffer atomicMin operations issued (diagnostic) */
     uint64_t num_blocks;           /* culling blocks (0 when block culling is off) */
     int32_t splat_binned;          /* 1: the last pass used the binned splat */
-    int32_t splat_fallback;        /* 1: its region lists overflowed, the point-parallel splat ran */
+    int32_t splat_fallback;        /* the point-parallel splat ran instead: 1 region lists overflowed,
+                                      2 more than 8 samples per framebuffer word (GPS_SPLAT_AUTO) */
     uint32_t bin_entries;          /* (Gaussian, region) pairs of the last pass */
     uint32_t bin_jobs;             /* region jobs of the last pass */
     uint32_t bin_big;              /* Gaussians splatted with global atomics (too many regions) */
@@ -204,7 +205,10 @@ void gps_pass_destroy(gps_pass* pass);
  * no per-sample global atomics (spp <= 4, at most 8192 regions per frame,
  * else point-parallel).  GPS_SPLAT_POINT_PARALLEL: one 64-bit global
  * atomicMin per accepted sample.  GPS_SPLAT_AUTO (default): binned when a
- * point has at least 8 samples (K * spp).  All give the same framebuffer bit
+ * point has at least 8 samples (K * spp) and the pass has at most 8 samples
+ * per framebuffer word (points * K <= 8 * upscaled width * height, tested on
+ * the device after the point count; denser passes splat point-parallel and
+ * report gps_stats.splat_fallback = 2).  All give the same framebuffer bit
  * for bit.  No reference counterpart (the reference has only the
  * point-parallel form, gaussian_point_splatting.cu:383-432). */
 #define GPS_SPLAT_AUTO 0

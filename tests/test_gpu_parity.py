@@ -568,8 +568,8 @@ def test_batch_binned_views_identical(gpu, case):
     """Binned batch views (the regions of all binned views in one job queue,
     counting and scattering per view, jobs that find their view by region)
     give every view's colour and depth bit for bit as a pass rendering the
-    frame alone, with the binned form forced, automatic (K * spp >= 8) and
-    off; with Gaussians too large to bin and with a point budget that cuts
+    frame alone, with the binned form forced, automatic (K * spp >= 8, at
+    most 8 samples per framebuffer word) and off; with Gaussians too large to bin and with a point budget that cuts
     views short."""
     cfg = dict(H.C1, half_extent=6.0, log_scale_mu=-2.0) if case != "closeup_big" else dict(H.C1, log_scale_mu=-0.7)
     arrays = H.scene(0x5EED0701 + len(case), 30_000, cfg).morton_sorted()
@@ -606,8 +606,10 @@ def test_batch_binned_views_identical(gpu, case):
             assert_same(f"{case} {name} view {v} depth", deps[v].download(), ref[v][1])
             st = batch.view_stats(v, stream)
             assert st["num_points"][0] == ref[v][2]["num_points"][0], (case, name, v)
-            binned = {"binned": True, "auto": ss * ss * spp >= 8, "point-parallel": False}[name]
-            assert st["splat_binned"] == int(binned) and st["splat_fallback"] == 0, (case, name, v, st)
+            stood_down = name == "auto" and ss * ss * spp >= 8 and _dense(st, w, h, ss)
+            binned = {"binned": True, "auto": ss * ss * spp >= 8 and not stood_down, "point-parallel": False}[name]
+            assert st["splat_binned"] == int(binned), (case, name, v, st)
+            assert st["splat_fallback"] == (2 if stood_down else 0), (case, name, v, st)
         if name == "binned":
             st = batch.view_stats(0, stream)
             flush(f"{case}: entries {st['bin_entries']} jobs {st['bin_jobs']} big {st['bin_big']}")
@@ -658,6 +660,78 @@ def test_binned_splat_equals_point_parallel(gpu, case):
         assert s["bin_big"] > 0
     if case == "spp1_ss1":
         assert s["bin_jobs"] > 6  # 3 x 2 regions of 128 px: some are split into several jobs
+
+
+def _dense(st, w, h, ss):
+    """More than 8 samples per framebuffer word (points * K > 8 * up_w * up_h)."""
+    return st["num_points"][0] * st["K"] > 8 * (w * ss) * (h * ss)
+
+
+def test_binned_density_gate(gpu):
+    """The automatic splat choice bins a 4-sample frame only while it has at
+    most 8 samples per framebuffer word: a dense frame (many points on few
+    pixels) stands the binned kernels down on the device
+    (splat_fallback = 2) and splats point-parallel, a sparse one stays
+    binned, the forced binned form never stands down, and every form gives
+    the same framebuffer bit for bit."""
+    arrays = H.scene(0x5EED0611, 30_000, dict(H.C1, half_extent=6.0, log_scale_mu=-2.0)).morton_sorted()
+    seen = set()
+    for w, h in ((48, 32), (320, 200)):
+        st = gpu.RenderSettings(w, h, 4, 2, gaussians_per_block=64)
+        f = gpu.frame_params((0.0, 3.0, -25.0), (0, 0, 0), (0, 1, 0), 45.0, w, h, ss=2, frame=5, spp=4)
+        runs = {}
+        for mode, name in ((gpu.Pass.SPLAT_AUTO, "auto"), (gpu.Pass.SPLAT_BINNED, "binned"),
+                           (gpu.Pass.SPLAT_POINT_PARALLEL, "point-parallel")):
+            flush(f"density gate {w}x{h}: {name}")
+            r = H.GpuRunner(arrays, st)
+            r.p.set_splat_mode(mode)
+            runs[name] = r.render(f, [gpu.DBG_FRAMEBUFFER])
+        dense = _dense(runs["auto"].stats, w, h, 2)
+        seen.add(dense)
+        a, b, p = runs["auto"], runs["binned"], runs["point-parallel"]
+        assert a.stats["splat_binned"] == 1 and a.stats["splat_fallback"] == (2 if dense else 0), (w, h, a.stats)
+        assert b.stats["splat_binned"] == 1 and b.stats["splat_fallback"] == 0, (w, h, b.stats)
+        assert p.stats["splat_binned"] == 0
+        for name in ("auto", "binned"):
+            assert_same(f"{w}x{h} {name} fb", runs[name].debug[gpu.DBG_FRAMEBUFFER], p.debug[gpu.DBG_FRAMEBUFFER])
+            assert_same(f"{w}x{h} {name} colour", runs[name].color, p.color)
+    assert seen == {True, False}
+
+
+def test_batch_binned_density_gate(gpu):
+    """In a batch the density test is per view: a dense view is splatted
+    point-parallel (splat_fallback = 2) while the sparse views of the same
+    batch stay binned, and every view equals a pass rendering it alone."""
+    arrays = H.scene(0x5EED0612, 30_000, dict(H.C1, half_extent=6.0, log_scale_mu=-2.0)).morton_sorted()
+    scene = gpu.Scene(arrays, gpu.FORMAT_FP32)
+    stream = gpu.Stream()
+    batch = gpu.Batch(scene, 3, 320, 200, max_spp=4, max_supersampling=2, gaussians_per_block=64)
+    single = gpu.Pass(gpu.RenderSettings(320, 200, 4, 2, flexible_frame_size=True, gaussians_per_block=64), scene)
+    shapes = [(48, 32), (320, 200), (64, 40)]
+    frames = [gpu.frame_params((0.5 * v, 3.0, -25.0), (0, 0, 0), (0, 1, 0), 45.0, w, h, ss=2, frame=v + 7, spp=4)
+              for v, (w, h) in enumerate(shapes)]
+    ref = []
+    for f in frames:
+        c = gpu.DeviceArray((f.height, f.width, 3), np.float32)
+        d = gpu.DeviceArray((f.height, f.width), np.float32)
+        single.set_splat_mode(gpu.Pass.SPLAT_POINT_PARALLEL)
+        single.render(f, c, d, stream)
+        stream.synchronize()
+        ref.append((c.download(), d.download()))
+    cols = [gpu.DeviceArray((f.height, f.width, 3), np.float32, zero=True) for f in frames]
+    deps = [gpu.DeviceArray((f.height, f.width), np.float32, zero=True) for f in frames]
+    flush("batch density gate: auto")
+    batch.render(frames, cols, deps, stream)
+    stream.synchronize()
+    kinds = []
+    for v, (w, h) in enumerate(shapes):
+        assert_same(f"view {v} colour", cols[v].download(), ref[v][0])
+        assert_same(f"view {v} depth", deps[v].download(), ref[v][1])
+        st = batch.view_stats(v, stream)
+        dense = _dense(st, w, h, 2)
+        kinds.append(dense)
+        assert st["splat_binned"] == (0 if dense else 1) and st["splat_fallback"] == (2 if dense else 0), (v, st)
+    assert kinds[0] and not kinds[1], kinds
 
 
 def test_device_fp8_decode_matches_oracle(gpu):
