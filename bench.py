@@ -659,6 +659,8 @@ def main() -> int:
                     help="keep generation order instead of the load-time Morton order")
     ap.add_argument("--no-group-cull", action="store_true",
                     help="time the path without the group occlusion test (same outputs)")
+    ap.add_argument("--no-ordered", action="store_true",
+                    help="time the path with the ordered splat off (chunk order; same outputs)")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serving measurement (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-sustained-seconds", type=float, default=30.0,
@@ -745,6 +747,7 @@ def main() -> int:
     stream = G.Stream()
     loop = W.FrameLoop(w, scene, stream)
     loop.pass_.set_group_culling(not args.no_group_cull)
+    loop.pass_.set_ordered_splat(G.Pass.ORDERED_OFF if args.no_ordered else G.Pass.ORDERED_AUTO)
     log(f"[bench] warmup {Wm} frames, then {K} timed frames (camera positions {F0 + Wm}..{F0 + Wm + K - 1} mod "
         f"{w.path_frames})")
     run(loop, range(F0, F0 + Wm))
@@ -781,10 +784,12 @@ def main() -> int:
     # pass (same initial state), reading the counters after every frame.
     loop = W.FrameLoop(w, scene, stream)
     loop.pass_.set_group_culling(not args.no_group_cull)
+    loop.pass_.set_ordered_splat(G.Pass.ORDERED_OFF if args.no_ordered else G.Pass.ORDERED_AUTO)
     loop.pass_.set_counters(True, count_rejects=True)
     per_frame = []
     rejects = [dict.fromkeys(G.REJECT_REASONS, 0) for _ in range(2)]
     group_violations = 0
+    ordered_skipped = 0
     run(loop, range(F0, F0 + Wm))
     for i in range(F0 + Wm, F0 + Wm + K):
         loop.step(W.frame(w, i % w.path_frames, i + 1))
@@ -793,6 +798,7 @@ def main() -> int:
             for key, v in r.items():
                 acc[key] += v
         group_violations += sum(loop.pass_.group_violations(loop.stream))
+        ordered_skipped += sum(loop.pass_.ordered_skipped(loop.stream))
     loop.close()
     reject_report = [{key: round(v / K) for key, v in acc.items()} for acc in rejects]
     # The same frames with each splat form forced (binned vs point-parallel
@@ -904,6 +910,7 @@ def main() -> int:
         "two_pass_frames": sum(1 for p in passes if p == 2),
         "valid_gaussians_per_frame": round(sum(sum(st["valid_gaussians"][:st["passes"]]) for st in per_frame) / K),
         "culled_blocks_per_frame": round(sum(sum(st["culled_blocks"][:st["passes"]]) for st in per_frame) / K),
+        "ordered_skipped_points_per_frame": round(ordered_skipped / K),
         "skipped_groups_per_frame": round(sum(sum(st.get("culled_groups", (0, 0))[:st["passes"]]) for st in per_frame) / K),
         "preprocess_rejects_per_frame": {"first_pass": reject_report[0], "second_pass": reject_report[1],
                                          "skipped_group_survivors": group_violations},

@@ -121,6 +121,12 @@ struct gps_pass {
     // splat grid
     uint32_t splat_grid = 0;
     int32_t xcds = 1;
+    // ordered point-parallel splat (dense passes)
+    int32_t ordered_mode = GPS_ORDERED_AUTO;
+    uint16_t* ord_bucket = nullptr;  // per chunk
+    uint32_t* ord_hist = nullptr;    // GPS_ORD_BINS (zero between passes)
+    uint32_t* ord_lim = nullptr;     // GPS_ORD_PHASES
+    uint32_t* ord_tmax = nullptr;    // per 16x16 tile of the framebuffer allocation
     // stage timing: ring of frames x stages x 2 occurrences x (start, stop)
     std::vector<hipEvent_t> ev;
     std::vector<uint8_t> ev_used;
@@ -175,9 +181,13 @@ gps_status ensure_fb(gps_pass* p, int ss, uint64_t frame_words) {
     }
     dfree(p->fbA);
     dfree(p->fbB);
+    dfree(p->ord_tmax);
     gps_status s = dalloc(&p->fbA, words, false);
     if (s != GPS_OK) return s;
     s = dalloc(&p->fbB, words, false);
+    if (s != GPS_OK) return s;
+    // one word per 16x16 tile of a 1-layer framebuffer of the allocation's size
+    s = dalloc(&p->ord_tmax, words / (256ull * (uint64_t)p->settings.render_pass_count), false);
     if (s != GPS_OK) return s;
     p->fb_cap = words;
     p->fb_words = frame_words;
@@ -484,6 +494,9 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     chk(dalloc(&p->cidx, N, false));
     chk(dalloc(&p->coff, N, false));
     chk(dalloc(&p->chunk_start, (uint64_t)p->budget / GPS_SPLAT_CHUNK + 2, true));
+    chk(dalloc(&p->ord_bucket, (uint64_t)p->budget / GPS_SPLAT_CHUNK + 2, false));
+    chk(dalloc(&p->ord_hist, GPS_ORD_BINS, true));
+    chk(dalloc(&p->ord_lim, GPS_ORD_PHASES, true));
     // Binned splat lists: up to 2 regions per valid Gaussian on average
     // (overflow falls back to the point-parallel splat on the device).
     {
@@ -567,6 +580,13 @@ gps_status gps_pass_create(const gps_render_settings* st, gps_scene* scene, gps_
     return GPS_OK;
 }
 
+gps_status gps_pass_set_ordered_splat(gps_pass* p, int32_t mode) {
+    if (!p || (mode != GPS_ORDERED_AUTO && mode != GPS_ORDERED_OFF && mode != GPS_ORDERED_ALWAYS))
+        return GPS_ERR_INVALID_ARG;
+    p->ordered_mode = mode;
+    return GPS_OK;
+}
+
 gps_status gps_pass_set_splat_mode(gps_pass* p, int32_t mode) {
     if (!p || (mode != GPS_SPLAT_AUTO && mode != GPS_SPLAT_POINT_PARALLEL && mode != GPS_SPLAT_BINNED))
         return GPS_ERR_INVALID_ARG;
@@ -598,6 +618,8 @@ uint64_t gps_pass_estimate_bytes(const gps_render_settings* st, const gps_scene*
                                  st->render_pass_count);
     uint64_t b = npad * 4 + 2 * nwords * 4 + std::max<uint64_t>(N, 1) * (sizeof(SplatRecord) + 8) +
                  16 * ntiles + (budget / GPS_SPLAT_CHUNK + 2) * 4 + sizeof(FrameCounters) + hiz * 4 + 2 * fb * 8;
+    b += (budget / GPS_SPLAT_CHUNK + 2) * 2 + 4ull * (GPS_ORD_BINS + GPS_ORD_PHASES) +
+         fb / (256ull * (uint64_t)st->render_pass_count) * 4;  // ordered splat
     {
         const uint64_t cap = round_up(std::min<uint64_t>(2 * std::max<uint64_t>(N, 1) + 65536, 0xFFFFF000ull), GPS_TILE);
         b += std::max<uint64_t>(N, 1) * 16 + 3 * cap * 4 + 2 * (cap / GPS_TILE + 1) * 4 + 8ull * 4 * GPS_BIN_HIST +
@@ -626,6 +648,9 @@ uint64_t gps_pass_device_bytes(const gps_pass* p) {
                  4ull * 4 * p->ntiles + ((uint64_t)p->budget / GPS_SPLAT_CHUNK + 2) * 4 + sizeof(FrameCounters) +
                  (uint64_t)p->hiz_size * 4 + 2 * p->fb_cap * 8 + p->dbg_indices_cap * 4;
     if (p->hier_available) b += (uint64_t)p->cull.node_count * sizeof(CullNode) + (p->cull.mask_words + 2) * 4ull;
+    if (p->ord_hist)
+        b += ((uint64_t)p->budget / GPS_SPLAT_CHUNK + 2) * 2 + 4ull * (GPS_ORD_BINS + GPS_ORD_PHASES) +
+             p->fb_cap / (256ull * (uint64_t)p->settings.render_pass_count) * 4;
     if (p->gboxes) b += (uint64_t)p->group.count * sizeof(GroupBox) + round_up(p->group.count, 4) + 4;
     if (p->live) b += (uint64_t)p->cull.node_count * 4 + (N + GPS_PRE_THREADS - 1) / GPS_PRE_THREADS;
     b += std::max<uint64_t>(N, 1) * 16 + 3ull * p->list_cap * 4 + 2ull * (p->list_cap / GPS_TILE + 1) * 4 +
@@ -647,6 +672,10 @@ void gps_pass_destroy(gps_pass* p) {
     dfree(p->cidx);
     dfree(p->coff);
     dfree(p->chunk_start);
+    dfree(p->ord_bucket);
+    dfree(p->ord_hist);
+    dfree(p->ord_lim);
+    dfree(p->ord_tmax);
     dfree(p->ctr);
     dfree(p->dbg_indices);
     dfree(p->fbA);
@@ -903,7 +932,34 @@ gps_status gps_pass_render(gps_pass* p, const gps_frame_params* fp, float* d_col
         } else {
             sa.only_if = nullptr;
         }
-        launch_splat(sa, st);
+        // Dense passes of many-sample points splat point-parallel nearest
+        // first and skip Gaussians behind their tiles (the density is decided
+        // on the device from the point count; launched when the scene could
+        // make the pass dense).
+        const bool ordered =
+            f.unbiased && p->ord_hist && p->ordered_mode != GPS_ORDERED_OFF &&
+            (p->ordered_mode == GPS_ORDERED_ALWAYS ||
+             (f.Ki * f.spp >= GPS_BIN_AUTO_SAMPLES &&
+              2ull * p->N * (uint64_t)f.Ki > (uint64_t)GPS_ORD_DENSE_SAMPLES * (uint64_t)f.up_w * (uint64_t)f.up_h));
+        if (ordered) {
+            OrderedArgs oa;
+            oa.s = sa;
+            oa.up_w = f.up_w;
+            oa.up_h = f.up_h;
+            oa.forced = p->ordered_mode == GPS_ORDERED_ALWAYS ? 1 : 0;
+            oa.pre_test = first ? 0 : 1;
+            oa.bucket = p->ord_bucket;
+            oa.hist = p->ord_hist;
+            oa.lim = p->ord_lim;
+            oa.tmax = p->ord_tmax;
+            oa.tw64 = fb_tiles_per_row(f.up_w);
+            oa.ntiles = (uint32_t)(p->fb_words / (256ull * (uint64_t)f.spp));
+            oa.cus = p->cus;
+            oa.skipped_count = p->count_rejects ? &p->ctr->ordered_skipped[slot] : nullptr;
+            launch_ordered_splat(oa, st);
+        } else {
+            launch_splat(sa, st);
+        }
         stage_mark(p, 4, slot, 1, st);
     };
     render_pass(true);
@@ -1122,7 +1178,7 @@ uint64_t gps_pass_debug_size(gps_pass* p, int32_t which) {
         case GPS_DBG_RECORDS: return 16ull * p->N;
         case GPS_DBG_HIZ: return p->hiz_size;
         case GPS_DBG_CULL_MASK: return p->cull.mask_words;
-        case GPS_DBG_REJECTS: return 20;
+        case GPS_DBG_REJECTS: return 22;
     }
     return 0;
 }

@@ -57,6 +57,24 @@ struct SplatArgs {
     const uint32_t* only_if;  // when set, the splat runs only if *only_if != 0
 };
 
+// Ordered point-parallel splat (dense passes, GPS_ORD_BINS): the splat
+// arguments plus the ordering buffers.
+struct OrderedArgs {
+    SplatArgs s;
+    int32_t up_w, up_h;       // framebuffer pixels of this frame
+    int32_t forced;           // order whatever the density
+    int32_t pre_test;         // the framebuffer holds an earlier pass: phase 0 tests too
+    uint16_t* bucket;         // per chunk
+    uint32_t* hist;           // GPS_ORD_BINS, zero between passes
+    uint32_t* lim;            // GPS_ORD_PHASES
+    uint32_t* tmax;           // per 16x16 tile of the framebuffer
+    uint32_t ntiles;          // 16x16 tiles of this frame's framebuffer
+    int32_t tw64;             // 64x64 tiles per framebuffer row
+    uint32_t cus;
+    uint32_t* skipped_count;  // points skipped (diagnostic; may be null)
+};
+void launch_ordered_splat(const OrderedArgs& a, hipStream_t st);
+
 // Splat constants shared by the point-parallel and the binned splat.
 struct SplatConsts {
     int32_t spp, K, up_w, unbiased;

@@ -1329,6 +1329,97 @@ __device__ __forceinline__ uint32_t flush_word_table(uint32_t* s_key, unsigned l
 }
 #endif
 
+// A Gaussian whose bounds (16-pixel aligned) cover at most
+// GPS_ORD_MAX_TILES 16x16 tiles whose maximum depth (tmax, the tiles in
+// framebuffer order: 64x64 tiles of tw64 per row, 16 Morton-ordered 16x16
+// tiles in each) is below its own: each of its samples lands inside the
+// bounds on a word that is already nearer, so none can lower a word.
+__device__ __forceinline__ bool tile_behind(const SplatRecord& rec, const uint32_t* __restrict__ tmax, int tw64) {
+    const uint32_t gd = rec.packed_hi >> 4;  // depth28 of the packed word
+    const int tx0 = (int)(rec.bmin & 0xFFFFu) >> 4, ty0 = (int)(rec.bmin >> 20);
+    const int tx1 = (int)(rec.bmax & 0xFFFFu) >> 4, ty1 = (int)(rec.bmax >> 20);
+    if ((tx1 - tx0) * (ty1 - ty0) > GPS_ORD_MAX_TILES) return false;
+    for (int ty = ty0; ty < ty1; ++ty)
+        for (int tx = tx0; tx < tx1; ++tx) {
+            const uint32_t t = ((uint32_t)((ty >> 2) * tw64 + (tx >> 2)) << 4) |
+                               (uint32_t)((tx & 1) | ((ty & 1) << 1) | ((tx & 2) << 1) | ((ty & 2) << 2));
+            if (tmax[t] >= gd) return false;
+        }
+    return true;
+}
+
+// One chunk of the point-parallel splat: the chunk's entry offsets into LDS,
+// PPT consecutive points per thread (a binary search for the first one),
+// then the word table's flush.  With TEST and tmax set, the points of a
+// Gaussian that tile_behind() clears are skipped and counted in `skipped`.
+template <bool TEST>
+__device__ __forceinline__ uint32_t splat_chunk(uint32_t c, const SplatRecord* __restrict__ recs,
+                                                const uint32_t* __restrict__ cidx, const uint32_t* __restrict__ coff,
+                                                const uint32_t* __restrict__ chunk_start, uint32_t P, uint32_t V,
+                                                unsigned long long* __restrict__ fb, const SplatConsts& sc,
+                                                uint32_t* s_off, uint32_t* s_key, unsigned long long* s_val,
+                                                const uint32_t* __restrict__ tmax, int tw64, uint32_t& skipped) {
+    const int tid = threadIdx.x;
+    uint32_t issued = 0;
+    const uint32_t r_lo = chunk_start[c];
+    for (int j = tid; j <= GPS_SPLAT_CHUNK; j += GPS_SPLAT_THREADS) {
+        const uint32_t r = r_lo + (uint32_t)j;
+        s_off[j] = r < V ? coff[r] : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint32_t p0 = c * GPS_SPLAT_CHUNK + (uint32_t)tid * GPS_SPLAT_PPT;
+    if (p0 < P) {
+        // last j with s_off[j] <= p0 (s_off[0] <= c*CHUNK <= p0)
+        int lo = 0, hi = GPS_SPLAT_CHUNK;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= p0) lo = mid;
+            else hi = mid - 1;
+        }
+        int j = lo;
+        uint32_t cur_g = 0xFFFFFFFFu;
+        SplatRecord rec;
+        int last = -1;
+        bool skip = false;
+        for (int k = 0; k < GPS_SPLAT_PPT; ++k) {
+            const uint32_t p = p0 + (uint32_t)k;
+            if (p >= P) break;
+            while (j < GPS_SPLAT_CHUNK - 1 && s_off[j + 1] <= p) ++j;
+            if (r_lo + (uint32_t)j >= V) break;
+            const uint32_t g = cidx[r_lo + (uint32_t)j];
+            if (g != cur_g) {
+                cur_g = g;
+                last = -1;
+                if (TEST && tmax) {
+                    // the test's words first (bounds, depth: the record's
+                    // second half); the rest only for a Gaussian that stays
+                    const uint4* rp = reinterpret_cast<const uint4*>(recs + g);
+                    const uint4 cc = rp[2], d = rp[3];
+                    rec.bmin = cc.w;
+                    rec.bmax = d.x;
+                    rec.packed_hi = d.z;
+                    skip = tile_behind(rec, tmax, tw64);
+                    if (!skip) rec = load_record(recs, g);
+                } else {
+                    rec = load_record(recs, g);
+                }
+            }
+            if (TEST && skip) {
+                ++skipped;
+                continue;
+            }
+            issued += splat_point(rec, p, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb, last, s_key, s_val);
+        }
+    }
+    __syncthreads();
+#if GPS_SPLAT_SLOTS
+    // flush the chunk's merged words and clear the table; the next chunk
+    // inserts only after the barrier that follows its offsets
+    issued += flush_word_table<GPS_SPLAT_THREADS>(s_key, s_val, fb);
+#endif
+    return issued;
+}
+
 __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRecord* __restrict__ recs,
                                                                   const uint32_t* __restrict__ cidx,
                                                                   const uint32_t* __restrict__ coff,
@@ -1350,7 +1441,7 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
     const uint32_t V = ctr->valid;
     const uint32_t nchunks = ctr->chunks;
     const int tid = threadIdx.x;
-    uint32_t issued = 0;
+    uint32_t issued = 0, skipped = 0;
 #if GPS_SPLAT_SLOTS
     for (int i = tid; i < GPS_SPLAT_SLOTS; i += GPS_SPLAT_THREADS) {
         s_key[i] = 0xFFFFFFFFu;
@@ -1365,49 +1456,182 @@ __global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_kernel(const SplatRec
     const uint32_t per_xcd = gridDim.x / xcds;
     const uint32_t seg_lo = (uint32_t)(((uint64_t)nchunks * xcd) / xcds);
     const uint32_t seg_hi = (uint32_t)(((uint64_t)nchunks * (xcd + 1)) / xcds);
-    for (uint32_t c = seg_lo + blockIdx.x / xcds; c < seg_hi; c += per_xcd) {
-        const uint32_t r_lo = chunk_start[c];
-        for (int j = tid; j <= GPS_SPLAT_CHUNK; j += GPS_SPLAT_THREADS) {
-            const uint32_t r = r_lo + (uint32_t)j;
-            s_off[j] = r < V ? coff[r] : 0xFFFFFFFFu;
-        }
-        __syncthreads();
-        const uint32_t p0 = c * GPS_SPLAT_CHUNK + (uint32_t)tid * GPS_SPLAT_PPT;
-        if (p0 < P) {
-            // last j with s_off[j] <= p0 (s_off[0] <= c*CHUNK <= p0)
-            int lo = 0, hi = GPS_SPLAT_CHUNK;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= p0) lo = mid;
-                else hi = mid - 1;
-            }
-            int j = lo;
-            uint32_t cur_g = 0xFFFFFFFFu;
-            SplatRecord rec;
-            int last = -1;
-            for (int k = 0; k < GPS_SPLAT_PPT; ++k) {
-                const uint32_t p = p0 + (uint32_t)k;
-                if (p >= P) break;
-                while (j < GPS_SPLAT_CHUNK - 1 && s_off[j + 1] <= p) ++j;
-                if (r_lo + (uint32_t)j >= V) break;
-                const uint32_t g = cidx[r_lo + (uint32_t)j];
-                if (g != cur_g) {
-                    cur_g = g;
-                    rec = load_record(recs, g);
-                    last = -1;
-                }
-                issued += splat_point(rec, p, sc.frame_seed, sc.K, sc.spp, sc.up_w, sc.unbiased, fb, last, s_key,
-                                      s_val);
-            }
-        }
-        __syncthreads();
+    for (uint32_t c = seg_lo + blockIdx.x / xcds; c < seg_hi; c += per_xcd)
+        issued += splat_chunk<false>(c, recs, cidx, coff, chunk_start, P, V, fb, sc, s_off, s_key, s_val, nullptr, 0,
+                                     skipped);
+    if (atomics_count) add_count(atomics_count, issued);
+}
+
+// ------------------------------------------------------------------ ordered splat
+// The ordered form of the point-parallel splat for dense passes (see
+// GPS_ORD_BINS): ord_key_kernel buckets the chunks by depth, ord_scan_kernel
+// turns the bucket histogram into the phases' bucket limits, and every phase
+// splats its chunks after ord_tile_max_kernel has taken the tiles' maximum
+// depth.  Every kernel first evaluates ord_active on the pass's counters (the
+// same decision everywhere); an inactive pass is splatted by phase 0 in the
+// plain chunk order, the other launches return at once.
+struct OrdGate {
+    uint32_t K, up_w, up_h;
+    int32_t forced;  // order whatever the density (tests)
+};
+
+__device__ __forceinline__ bool ord_active(const PassCounters* __restrict__ ctr, const OrdGate& g) {
+    const uint32_t n = ctr->chunks;
+    if (n == 0u) return false;
+    if (g.forced) return true;
+    return n >= GPS_ORD_MIN_CHUNKS &&
+           (uint64_t)ctr->points * g.K > (uint64_t)GPS_ORD_DENSE_SAMPLES * g.up_w * (uint64_t)g.up_h;
+}
+
+// Log-depth bucket: depths below 128 one bucket each, then 128 buckets per
+// power of two (monotone; 2816 buckets for 28-bit depths).
+__device__ __forceinline__ uint32_t ord_bucket(uint32_t d) {
+    if (d < 128u) return d;
+    const uint32_t e = 31u - (uint32_t)__builtin_clz(d);  // 7..27
+    return (e - 6u) * 128u + ((d >> (e - 7u)) & 127u);
+}
+
+__global__ __launch_bounds__(256) void ord_key_kernel(const SplatRecord* __restrict__ recs,
+                                                      const uint32_t* __restrict__ cidx,
+                                                      const uint32_t* __restrict__ chunk_start,
+                                                      const PassCounters* __restrict__ ctr, OrdGate gate,
+                                                      const uint32_t* __restrict__ only_if,
+                                                      uint16_t* __restrict__ bucket, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_h[GPS_ORD_BINS];
+    if (only_if && *only_if == 0u) return;
+    if (!ord_active(ctr, gate)) return;
+    const uint32_t n = ctr->chunks;
+    for (uint32_t i = threadIdx.x; i < GPS_ORD_BINS; i += 256) s_h[i] = 0u;
+    __syncthreads();
+    for (uint32_t c = blockIdx.x * 256u + threadIdx.x; c < n; c += gridDim.x * 256u) {
+        const uint32_t g = cidx[chunk_start[c]];
+        const uint32_t d = reinterpret_cast<const uint32_t*>(recs + g)[14] >> 4;  // packed_hi: depth28
+        const uint32_t b = ord_bucket(d);
+        bucket[c] = (uint16_t)b;
+        atomicAdd(&s_h[b], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < GPS_ORD_BINS; i += 256)
+        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// One workgroup: lim[k] = one past the bucket in which the running count of
+// chunks reaches n / 2^(GPS_ORD_PHASES - 1 - k) (n/8, n/4, n/2, n for four
+// phases); clears the histogram.
+__global__ __launch_bounds__(1024) void ord_scan_kernel(const PassCounters* __restrict__ ctr, OrdGate gate,
+                                                       const uint32_t* __restrict__ only_if,
+                                                       uint32_t* __restrict__ hist, uint32_t* __restrict__ lim) {
+    static_assert(GPS_ORD_PHASES >= 2 && GPS_ORD_PHASES <= 8 && 3u * 1024u >= GPS_ORD_BINS,
+                  "three buckets per thread, phases of n / 2^k");
+    __shared__ uint32_t s_w[16];
+    if (only_if && *only_if == 0u) return;
+    if (!ord_active(ctr, gate)) return;
+    const uint32_t n = ctr->chunks;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t h[3], sum = 0u;
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t b = (uint32_t)tid * 3u + (uint32_t)k;
+        h[k] = b < GPS_ORD_BINS ? hist[b] : 0u;
+        sum += h[k];
+    }
+    const uint32_t incl = wave_scan_incl(sum);
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (int w = 0; w < wave; ++w) run += s_w[w];
+    uint32_t t[GPS_ORD_PHASES];
+    for (int ph = 0; ph < GPS_ORD_PHASES - 1; ++ph) t[ph] = max(1u, n >> (GPS_ORD_PHASES - 1 - ph));
+    t[GPS_ORD_PHASES - 1] = n;
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t b = (uint32_t)tid * 3u + (uint32_t)k;
+        if (b >= GPS_ORD_BINS) break;
+        const uint32_t excl = run;
+        run += h[k];
+        for (int ph = 0; ph < GPS_ORD_PHASES; ++ph)
+            if (excl < t[ph] && run >= t[ph]) lim[ph] = b + 1u;
+        hist[b] = 0u;
+    }
+}
+
+// Maximum depth28 of every 16x16-pixel tile (256 * spp consecutive words in
+// the framebuffer's Morton layout), one wave per tile.
+__global__ __launch_bounds__(256) void ord_tile_max_kernel(const unsigned long long* __restrict__ fb,
+                                                           uint32_t* __restrict__ tmax, uint32_t ntiles, int spp,
+                                                           const PassCounters* __restrict__ ctr, OrdGate gate,
+                                                           const uint32_t* __restrict__ only_if) {
+    if (only_if && *only_if == 0u) return;
+    if (!ord_active(ctr, gate)) return;
+    const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= ntiles) return;
+    const uint4* w = reinterpret_cast<const uint4*>(fb + (uint64_t)t * 256u * (uint32_t)spp);
+    uint32_t m = 0u;
+    for (int i = lane; i < 128 * spp; i += 64) {
+        const uint4 v = w[i];  // two words: (lo, hi) pairs
+        m = max(m, max(v.y >> 4, v.w >> 4));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    if (lane == 0) tmax[t] = m;
+}
+
+// One phase of the ordered splat: the chunks whose bucket is in
+// [lim[phase-1], lim[phase]), in groups of GPS_ORD_GROUP consecutive chunks
+// dealt round-robin to the XCDs; with `test`, Gaussians behind the tiles are
+// skipped.  An inactive pass is splatted by phase 0 in splat_kernel's order.
+__global__ __launch_bounds__(GPS_SPLAT_THREADS) void splat_ordered_kernel(
+    const SplatRecord* __restrict__ recs, const uint32_t* __restrict__ cidx, const uint32_t* __restrict__ coff,
+    const uint32_t* __restrict__ chunk_start, const PassCounters* __restrict__ ctr, unsigned long long* __restrict__ fb,
+    SplatConsts sc, uint32_t* __restrict__ atomics_count, const uint32_t* __restrict__ only_if,
+    const uint16_t* __restrict__ bucket, const uint32_t* __restrict__ lim, const uint32_t* __restrict__ tmax,
+    int tw64, int phase, int test, OrdGate gate, uint32_t* __restrict__ skipped_count) {
+    __shared__ uint32_t s_off[GPS_SPLAT_CHUNK + 1];
 #if GPS_SPLAT_SLOTS
-        // flush the chunk's merged words and clear the table; the next
-        // chunk inserts only after the barrier that follows its offsets
-        issued += flush_word_table<GPS_SPLAT_THREADS>(s_key, s_val, fb);
+    __shared__ uint32_t s_key[GPS_SPLAT_SLOTS];
+    __shared__ unsigned long long s_val[GPS_SPLAT_SLOTS];
+#else
+    uint32_t* const s_key = nullptr;
+    unsigned long long* const s_val = nullptr;
 #endif
+    if (only_if && *only_if == 0u) return;
+    const bool active = ord_active(ctr, gate);
+    if (!active && phase > 0) return;
+    const uint32_t P = ctr->points;
+    const uint32_t V = ctr->valid;
+    const uint32_t nchunks = ctr->chunks;
+    const int tid = threadIdx.x;
+    uint32_t issued = 0, skipped = 0;
+#if GPS_SPLAT_SLOTS
+    for (int i = tid; i < GPS_SPLAT_SLOTS; i += GPS_SPLAT_THREADS) {
+        s_key[i] = 0xFFFFFFFFu;
+        s_val[i] = ~0ull;
+    }
+#endif
+    const uint32_t xcds = (uint32_t)sc.xcds;
+    const uint32_t xcd = blockIdx.x % xcds;
+    const uint32_t per_xcd = gridDim.x / xcds;
+    if (!active) {
+        const uint32_t seg_lo = (uint32_t)(((uint64_t)nchunks * xcd) / xcds);
+        const uint32_t seg_hi = (uint32_t)(((uint64_t)nchunks * (xcd + 1)) / xcds);
+        for (uint32_t c = seg_lo + blockIdx.x / xcds; c < seg_hi; c += per_xcd)
+            issued += splat_chunk<true>(c, recs, cidx, coff, chunk_start, P, V, fb, sc, s_off, s_key, s_val, nullptr,
+                                        0, skipped);
+    } else {
+        const uint32_t lo = phase > 0 ? lim[phase - 1] : 0u, hi = lim[phase];
+        const uint32_t ngroups = (nchunks + GPS_ORD_GROUP - 1u) / GPS_ORD_GROUP;
+        const uint32_t mine = xcd < ngroups ? (ngroups - xcd + xcds - 1u) / xcds : 0u;  // groups of this XCD
+        const uint32_t items = mine * GPS_ORD_GROUP;
+        const uint32_t* const tm = test ? tmax : nullptr;
+        for (uint32_t i = blockIdx.x / xcds; i < items; i += per_xcd) {
+            const uint32_t c = ((i / GPS_ORD_GROUP) * xcds + xcd) * GPS_ORD_GROUP + i % GPS_ORD_GROUP;
+            if (c >= nchunks) continue;
+            const uint32_t b = bucket[c];
+            if (b < lo || b >= hi) continue;
+            issued += splat_chunk<true>(c, recs, cidx, coff, chunk_start, P, V, fb, sc, s_off, s_key, s_val, tm, tw64,
+                                        skipped);
+        }
     }
     if (atomics_count) add_count(atomics_count, issued);
+    if (skipped_count) add_count(skipped_count, skipped);
 }
 
 // ------------------------------------------------------------------ binned splat
@@ -2728,6 +2952,36 @@ void launch_distribute(const DistributeArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(distribute_kernel, dim3(a.ntiles), dim3(256), 0, st, a.weights, a.tile_off, a.tile_voff,
                        a.ntiles, a.ctr, a.cidx, a.coff, a.chunk_start, a.cull, a.hier, a.N, a.ntiles, 0u, 0u,
                        (uint8_t*)nullptr, GPS_SPLAT_CHUNK_SHIFT, a.tile_sum, a.tile_cnt);
+}
+
+void launch_ordered_splat(const OrderedArgs& a, hipStream_t st) {
+    const SplatArgs& s = a.s;
+    SplatConsts sc;
+    sc.spp = s.spp;
+    sc.K = s.K;
+    sc.up_w = s.up_w;
+    sc.unbiased = s.unbiased;
+    sc.frame_seed = s.frame_seed;
+    sc.xcds = s.xcds;
+    OrdGate gate;
+    gate.K = (uint32_t)s.K;
+    gate.up_w = (uint32_t)a.up_w;
+    gate.up_h = (uint32_t)a.up_h;
+    gate.forced = a.forced;
+    unsigned long long* fb = reinterpret_cast<unsigned long long*>(s.fb);
+    hipLaunchKernelGGL(ord_key_kernel, dim3(a.cus), dim3(256), 0, st, s.recs, s.cidx, s.chunk_start, s.ctr, gate,
+                       s.only_if, a.bucket, a.hist);
+    hipLaunchKernelGGL(ord_scan_kernel, dim3(1), dim3(1024), 0, st, s.ctr, gate, s.only_if, a.hist, a.lim);
+    for (int ph = 0; ph < GPS_ORD_PHASES; ++ph) {
+        const bool test = ph > 0 || a.pre_test;
+        if (test)
+            hipLaunchKernelGGL(ord_tile_max_kernel, dim3((a.ntiles + 3u) / 4u), dim3(256), 0, st,
+                               (const unsigned long long*)fb, a.tmax, a.ntiles, s.spp, s.ctr, gate, s.only_if);
+        hipLaunchKernelGGL(splat_ordered_kernel, dim3(s.grid), dim3(GPS_SPLAT_THREADS), 0, st, s.recs, s.cidx, s.coff,
+                           s.chunk_start, s.ctr, fb, sc, s.atomics_count, s.only_if, (const uint16_t*)a.bucket,
+                           (const uint32_t*)a.lim, (const uint32_t*)a.tmax, a.tw64, ph, test ? 1 : 0, gate,
+                           a.skipped_count);
+    }
 }
 
 void launch_splat(const SplatArgs& a, hipStream_t st) {

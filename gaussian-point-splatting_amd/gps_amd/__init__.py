@@ -198,6 +198,7 @@ _sig("gps_pass_set_timing", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_set_splat_mode", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_set_counters", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_set_group_culling", C.c_int32, _vp, C.c_int32)
+_sig("gps_pass_set_ordered_splat", C.c_int32, _vp, C.c_int32)
 _sig("gps_pass_timing_summary", C.c_int32, _vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
      C.POINTER(C.c_uint64))
 _sig("gps_selftest_math", C.c_int32, C.c_int32, _vp, _vp, C.c_uint64, _vp)
@@ -596,7 +597,23 @@ class Pass:
         """Per render pass of the last frame (counters on with count_rejects):
         members of groups the group occlusion test skipped that survived every
         per-Gaussian test.  0 when the test is sound."""
-        return [int(v) for v in self.debug(DBG_REJECTS, stream)[2 * len(REJECT_REASONS):]]
+        n = 2 * len(REJECT_REASONS)
+        return [int(v) for v in self.debug(DBG_REJECTS, stream)[n:n + 2]]
+
+    def ordered_skipped(self, stream: Optional[Stream] = None) -> list:
+        """Per render pass of the last frame (counters on with count_rejects):
+        points the ordered splat skipped behind nearer tiles."""
+        n = 2 * len(REJECT_REASONS) + 2
+        return [int(v) for v in self.debug(DBG_REJECTS, stream)[n:n + 2]]
+
+    ORDERED_AUTO = 0
+    ORDERED_OFF = 1
+    ORDERED_ALWAYS = 2
+
+    def set_ordered_splat(self, mode: int) -> None:
+        """ORDERED_AUTO (dense passes), ORDERED_OFF or ORDERED_ALWAYS; the
+        framebuffer is the same bit for bit in every mode."""
+        check(_lib.gps_pass_set_ordered_splat(self.handle, mode), "ordered splat")
 
     def set_group_culling(self, enable: bool) -> None:
         """The group occlusion test (on by default; outputs are the same bits either way)."""

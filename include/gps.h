@@ -233,6 +233,24 @@ gps_status gps_pass_set_counters(gps_pass* pass, int32_t flags);
  * Outputs are the same bits with it on or off.  No reference counterpart (the
  * reference tests every Gaussian of an unculled block). */
 gps_status gps_pass_set_group_culling(gps_pass* pass, int32_t enable);
+/* Ordered point-parallel splat.  GPS_ORDERED_AUTO (default): a pass whose
+ * points have at least 8 samples (K * spp) and which has more than 32
+ * samples per framebuffer word (points * K > 32 * upscaled width * height,
+ * tested on the device) is splatted nearest first: chunks of points
+ * are bucketed by the depth of their first Gaussian and splatted in four
+ * phases (the nearest 1/8, 1/4, 1/2 and all of the chunks); before each
+ * later phase (and before the first one in a second render pass) the maximum
+ * depth of every 16x16-pixel framebuffer tile is taken, and a Gaussian whose
+ * bounds cover at most 16 tiles, all nearer than it, is skipped, since none
+ * of its samples can lower a word.  GPS_ORDERED_OFF: chunk order, nothing
+ * skipped.  GPS_ORDERED_ALWAYS: ordered whatever the density.  The
+ * framebuffer is the same bit for bit in every mode; the unbiased splat
+ * only.  No reference counterpart (the reference splats every point,
+ * gaussian_point_splatting.cu:383-432). */
+#define GPS_ORDERED_AUTO 0
+#define GPS_ORDERED_OFF 1
+#define GPS_ORDERED_ALWAYS 2
+gps_status gps_pass_set_ordered_splat(gps_pass* pass, int32_t mode);
 /* Device memory held by the pass (scene not included), and what
  * gps_pass_create would allocate for these settings. */
 uint64_t gps_pass_device_bytes(const gps_pass* pass);
@@ -269,7 +287,8 @@ typedef enum gps_debug_buffer {
                                    footprint off screen; small-eigenvalue cull; Hi-Z occluded;
                                    point count 0; valid; then uint32[2]: per pass, members of
                                    groups the group occlusion test marked that survived the
-                                   per-Gaussian tests (0 when the test is sound) */
+                                   per-Gaussian tests (0 when the test is sound); then
+                                   uint32[2]: per pass, points the ordered splat skipped */
 } gps_debug_buffer;
 uint64_t gps_pass_debug_size(gps_pass* pass, int32_t which);
 gps_status gps_pass_debug_copy(gps_pass* pass, int32_t which, void* host_dst, uint64_t count, void* stream);
