@@ -453,6 +453,61 @@ def serving(G, W, args, resident):
             "hbm_used_gb": round(st["hbm_used_bytes"] / 1e9, 2)}
 
 
+def request_types(G, W, args) -> dict:
+    """GPU time per serving request type: for every resident scene and spp of
+    the serving mix, 12 of its requests (mixed resolutions) rendered back to
+    back on one pass with the server's settings (the largest request size,
+    flexible frame size, automatic block size), with the splat form each
+    request got and the points the ordered splat skipped on the first one."""
+    sw = W.SERVE
+    reqs = W.serve_requests(sw, count=400, rate=0.0)
+    mw = max(r[0] for r in sw.resolutions)
+    mh = max(r[1] for r in sw.resolutions)
+    color = G.DeviceArray((mh, mw, 3), np.float32)
+    depth = G.DeviceArray((mh, mw), np.float32)
+    stream = G.Stream()
+    rows = []
+    only = set(args.request_scenes.split(",")) if args.request_scenes else None
+    for k, w in enumerate(sw.scenes):
+        if only is not None and w.name not in only:
+            continue
+        log(f"[types] {w.name}: {w.n} Gaussians")
+        scene = W.device_scene(w, log=lambda m: log("[types] " + m))
+        p = G.Pass(G.RenderSettings(mw, mh, max(sw.spps), sw.ss, flexible_frame_size=True), scene)
+        for spp in sw.spps:
+            fs = [f for kk, f, _ in reqs if kk == k and f.samples_per_pixel == spp][:12]
+            p.set_counters(False, False)
+            p.render(fs[0], color, depth, stream)
+            stream.synchronize()
+            p.set_timing(len(fs) + 1)
+            e0, e1 = G.Event(), G.Event()
+            e0.record(stream)
+            for f in fs:
+                p.render(f, color, depth, stream)
+            e1.record(stream)
+            stream.synchronize()
+            stages = p.timing_summary()["stages"]
+            p.set_counters(False, True)
+            p.render(fs[0], color, depth, stream)
+            st = p.stats(stream)
+            rows.append({"scene": w.name, "spp": spp, "requests": len(fs),
+                         "ms_per_request": round(e0.elapsed_ms(e1) / len(fs), 3),
+                         "stages_ms": {n: round(v["ms"] / len(fs), 3) for n, v in stages.items() if v["launches"]},
+                         "first_request": {"size": [fs[0].width, fs[0].height], "points": st["num_points"][0],
+                                           "splat_binned": st["splat_binned"], "splat_fallback": st["splat_fallback"],
+                                           "ordered_skipped": sum(p.ordered_skipped(stream))}})
+            log(f"[types] {w.name} spp {spp}: {rows[-1]['ms_per_request']} ms per request")
+        p.close()
+        del scene
+    out = {"request_types": rows, "workload": "c5 request mix: 1M fp32 / 10M fp32 / 100M compact scenes, "
+           "854x480 / 1280x720 / 1920x1080, ss 2, random sphere cameras, no occlusion culling"}
+    if only is None:
+        mean = sum(r["ms_per_request"] for r in rows) / len(rows)
+        out["mean_ms_per_request"] = round(mean, 3)
+        out["requests_per_s_back_to_back"] = round(1000.0 / mean, 1)
+    return out
+
+
 def _e4m3_table():
     """FP8 E4M3 (OCP, finite-only) value of every code, from its bit fields."""
     codes = np.arange(256)
@@ -659,6 +714,10 @@ def main() -> int:
                     help="keep generation order instead of the load-time Morton order")
     ap.add_argument("--no-group-cull", action="store_true",
                     help="time the path without the group occlusion test (same outputs)")
+    ap.add_argument("--request-types", action="store_true",
+                    help="measure only the GPU time per serving request type (one JSON line)")
+    ap.add_argument("--request-scenes", default="",
+                    help="with --request-types: comma-separated scene names (c1,c2,c3; default all)")
     ap.add_argument("--no-ordered", action="store_true",
                     help="time the path with the ordered splat off (chunk order; same outputs)")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serving measurement (0 = skip)")
@@ -721,6 +780,9 @@ def main() -> int:
         return 0
     if args.batch_only:
         print(json.dumps({"batched": batched(G, W, args)}), flush=True)
+        return 0
+    if args.request_types:
+        print(json.dumps(request_types(G, W, args)), flush=True)
         return 0
     w = W.WORKLOADS[args.config]
     if args.format != "workload":
