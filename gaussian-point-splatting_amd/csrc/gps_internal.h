@@ -60,7 +60,7 @@ struct SplatRecord {
     uint32_t bmax;
     uint32_t packed_lo;        // depth28 << 36 | colour36
     uint32_t packed_hi;
-    uint32_t pad;
+    float alpha_t;             // alpha_threshold(opacity): the alpha test's exponent bound
 };
 static_assert(sizeof(SplatRecord) == 64, "record must stay 64 bytes");
 static_assert(sizeof(gps_quantized_bounds) == 102 * sizeof(float), "scene box is read as 102 floats");

@@ -582,7 +582,8 @@ __device__ __forceinline__ uint32_t preprocess_one(const DevScene& s, const Fram
                       __float_as_uint(pr.ca * inv_det));
     r[2] = make_uint4(__float_as_uint(op), __float_as_uint(corrected_inv_alpha(op)), __float_as_uint(dl),
                       (uint32_t)pr.minx | ((uint32_t)pr.miny << 16));
-    r[3] = make_uint4((uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16), (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
+    r[3] = make_uint4((uint32_t)pr.maxx | ((uint32_t)pr.maxy << 16), (uint32_t)packed, (uint32_t)(packed >> 32),
+                      __float_as_uint(alpha_threshold(op)));
     return cnt;
 }
 
@@ -848,7 +849,7 @@ __device__ __forceinline__ void preprocess_wg(const DevScene& s, const FrameCons
                 ShEval<FORMAT, DEG>::eval(s, q.g, dx, dy, dz, rgb);
             const uint64_t packed = ((uint64_t)(uint32_t)depth_quantize(q.view_z, f.near_plane, f.far_plane)
                                      << GPS_DEPTH_SHIFT) | pack_color36(rgb[0], rgb[1], rgb[2]);
-            r[3] = make_uint4(q.bmax, (uint32_t)packed, (uint32_t)(packed >> 32), 0u);
+            r[3] = make_uint4(q.bmax, (uint32_t)packed, (uint32_t)(packed >> 32), __float_as_uint(alpha_threshold(q.op)));
             s_count[q.g - g0] = cnt;
         }
         weights[q.g] = cnt;
@@ -1169,14 +1170,15 @@ __device__ __forceinline__ SplatRecord load_record(const SplatRecord* __restrict
     rec.bmax = d.x;
     rec.packed_lo = d.y;
     rec.packed_hi = d.z;
+    rec.alpha_t = __uint_as_float(d.w);
     return rec;
 }
 
-// The alpha test of a sample (gaussian_point_splatting.cu:118-123):
-// gw = sigma > 0 ? 0 : exp(sigma), rejected when opacity * gw < 1/255.
+// The alpha test of a sample (gaussian_point_splatting.cu:118-123), decided
+// from the exponent against the record's threshold and evaluated with the
+// exponential only near it (alpha_test_banded: the same decision).
 __device__ __forceinline__ bool alpha_passes(const SplatRecord& rec, float sigma) {
-    const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
-    return !(rec.opacity * gw < 1.0f / 255.0f);
+    return alpha_test_banded(rec.opacity, rec.alpha_t, sigma);
 }
 
 // One sample of the biased splat (use_unbiased_2d_splatting = 0) on its

@@ -148,6 +148,28 @@ GPS_HD f32x2 corrected_box_muller_pre(float u1, float u2, float inv_alpha, float
     return f32x2{R * c, R * s};
 }
 GPS_HD float corrected_inv_alpha(float alpha) { return 1.0f / det_fmax(alpha, 1.e-37f); }
+
+// The alpha test of a sample (gaussian_point_splatting.cu:118-123):
+// gw = sigma > 0 ? 0 : exp(sigma), rejected when opacity * gw < 1/255.
+GPS_HD bool alpha_test(float opacity, float sigma) {
+    const float gw = sigma > 0.0f ? 0.0f : det_expf(sigma);
+    return !(opacity * gw < 1.0f / 255.0f);
+}
+// The exponent at which the test flips: t = ln((1/255) / opacity).  The
+// test is monotone in sigma; its decision is fixed outside
+// [t - GPS_ALPHA_BAND, t + GPS_ALPHA_BAND], a band far wider than the
+// combined error of det_expf (a few ulps), the product's rounding and t's
+// own (det_logf of a rounded quotient): about 2e-5 in the exponent for
+// |t| < 100.  Subnormal and zero exponentials lie far below the band.
+#define GPS_ALPHA_BAND 0x1.0p-8f
+GPS_HD float alpha_threshold(float opacity) { return det_logf((1.0f / 255.0f) / opacity); }
+// alpha_test's decision, with the exponential evaluated only inside the band
+// (t = alpha_threshold(opacity); NaN anywhere falls through to alpha_test).
+GPS_HD bool alpha_test_banded(float opacity, float t, float sigma, float band = GPS_ALPHA_BAND) {
+    if (sigma < t - band) return false;
+    if (sigma > t + band && !(sigma > 0.0f)) return true;
+    return alpha_test(opacity, sigma);
+}
 GPS_HD f32x2 corrected_box_muller(float u1, float u2, float alpha) {
     return corrected_box_muller_pre(u1, u2, corrected_inv_alpha(alpha), dilog(alpha));
 }
