@@ -227,7 +227,9 @@ static_assert((GPS_SPLAT_SLOTS & (GPS_SPLAT_SLOTS - 1)) == 0, "slots: a power of
 #ifndef GPS_ORD_PHASES
 #define GPS_ORD_PHASES 4
 #endif
+#ifndef GPS_ORD_GROUP
 #define GPS_ORD_GROUP 16u
+#endif
 #define GPS_ORD_MAX_TILES 16
 #define GPS_ORD_MIN_CHUNKS 64u  // fewer chunks: one phase in chunk order
 // Measured on whole-scene requests at 1920x1080 / 1280x720 / 854x480, ss 2
