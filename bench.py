@@ -634,7 +634,8 @@ def batched(G, W, args):
     batch = G.Batch(scene, bw.views, mw, mh, max(bw.spps), bw.ss)
     tuned = batch.autotune(frames[0], cols, deps, stream, repeats=2)
     log(f"[batch] autotune: {tuned}")
-    batch.set_tuning(tuned["splat_threads"], tuned["splat_points_per_thread"], True)
+    batch.set_tuning(tuned["splat_threads"], tuned["splat_points_per_thread"], True,
+                     tuned["preprocess_views_per_workgroup"])
     batch.render(frames[0], cols, deps, stream)  # capture
     stream.synchronize()
     t0 = time.perf_counter()

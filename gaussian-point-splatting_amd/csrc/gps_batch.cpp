@@ -35,9 +35,10 @@ struct Layout {  // what a captured graph depends on
     std::vector<int32_t> shape;  // per view: w, h, ss, spp, hier, regions
     std::vector<const void*> outs;
     void* stream = nullptr;
-    int threads = 0, ppt = 0;
+    int threads = 0, ppt = 0, pre_views = 0;
     bool operator==(const Layout& o) const {
-        return shape == o.shape && outs == o.outs && stream == o.stream && threads == o.threads && ppt == o.ppt;
+        return shape == o.shape && outs == o.outs && stream == o.stream && threads == o.threads && ppt == o.ppt &&
+               pre_views == o.pre_views;
     }
 };
 
@@ -68,6 +69,7 @@ struct gps_batch {
     uint32_t grid = 0, cus = 256;
     int32_t xcds = 1;
     int threads = 256, ppt = 4;
+    int pre_views = 0;  // views per preprocess workgroup (0: all)
     int splat_mode = GPS_SPLAT_AUTO;
     // binned splat buffers, allocated on the first batch with a binned view
     bool bin_ready = false, bin_failed = false;
@@ -194,6 +196,7 @@ BatchArgs args(gps_batch* b, int nviews) {
     a.xcds = b->xcds;
     a.threads = b->threads;
     a.ppt = b->ppt;
+    a.pre_views = b->pre_views;
     a.atomics = b->atomics;
     a.bin = b->bin;
     return a;
@@ -317,6 +320,7 @@ gps_status gps_batch_render(gps_batch* b, int32_t nviews, const gps_frame_params
     lay.stream = stream;
     lay.threads = b->threads;
     lay.ppt = b->ppt;
+    lay.pre_views = b->pre_views;
     uint32_t regions = 0, max_view_regions = 0;
     for (int v = 0; v < nviews; ++v) {
         const gps_frame_params& f = views[v];
@@ -400,10 +404,14 @@ gps_status gps_batch_set_tuning(gps_batch* b, const gps_batch_tuning* t) {
     bool ok = false;
     for (int i = 0; i < kNumBatchSplatVariants; ++i)
         ok |= kBatchSplatVariants[i][0] == t->splat_threads && kBatchSplatVariants[i][1] == t->splat_points_per_thread;
-    if (!ok) return GPS_ERR_INVALID_ARG;
-    if (b->threads != t->splat_threads || b->ppt != t->splat_points_per_thread) drop_graph(b);
+    if (!ok || t->preprocess_views_per_workgroup < 0 || t->preprocess_views_per_workgroup > GPS_BATCH_MAX_VIEWS)
+        return GPS_ERR_INVALID_ARG;
+    if (b->threads != t->splat_threads || b->ppt != t->splat_points_per_thread ||
+        b->pre_views != t->preprocess_views_per_workgroup)
+        drop_graph(b);
     b->threads = t->splat_threads;
     b->ppt = t->splat_points_per_thread;
+    b->pre_views = t->preprocess_views_per_workgroup;
     b->use_graph = t->use_graph != 0;
     return GPS_OK;
 }
@@ -420,16 +428,23 @@ gps_status gps_batch_autotune(gps_batch* b, int32_t nviews, const gps_frame_para
     double best = 1e30;
     int best_i = 0;
     gps_status s = GPS_OK;
+    auto time_batch = [&](float* ms) {
+        gps_status r = gps_batch_render(b, nviews, views, colors, depths, stream);  // warm-up
+        *ms = 0.0f;
+        if (r == GPS_OK && hipEventRecord(e0, st) != hipSuccess) r = GPS_ERR_HIP;
+        for (int k = 0; k < repeats && r == GPS_OK; ++k) r = gps_batch_render(b, nviews, views, colors, depths, stream);
+        if (r == GPS_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+            r = GPS_ERR_HIP;
+        return r;
+    };
+    // splat variants at the current preprocess split, then the preprocess
+    // splits with the fastest splat variant
     for (int i = 0; i < kNumBatchSplatVariants && s == GPS_OK; ++i) {
         b->threads = kBatchSplatVariants[i][0];
         b->ppt = kBatchSplatVariants[i][1];
-        s = gps_batch_render(b, nviews, views, colors, depths, stream);  // warm-up
         float ms = 0.0f;
-        if (s == GPS_OK && hipEventRecord(e0, st) != hipSuccess) s = GPS_ERR_HIP;
-        for (int r = 0; r < repeats && s == GPS_OK; ++r) s = gps_batch_render(b, nviews, views, colors, depths, stream);
-        if (s == GPS_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            s = GPS_ERR_HIP;
+        s = time_batch(&ms);
         out->variant_ms[i] = ms / repeats;
         out->variant_threads[i] = kBatchSplatVariants[i][0];
         out->variant_points_per_thread[i] = kBatchSplatVariants[i][1];
@@ -438,14 +453,29 @@ gps_status gps_batch_autotune(gps_batch* b, int32_t nviews, const gps_frame_para
             best_i = i;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     out->variants = kNumBatchSplatVariants;
     out->splat_threads = kBatchSplatVariants[best_i][0];
     out->splat_points_per_thread = kBatchSplatVariants[best_i][1];
-    out->use_graph = graph ? 1 : 0;
     b->threads = out->splat_threads;
     b->ppt = out->splat_points_per_thread;
+    int best_v = b->pre_views;
+    for (int i = 0; i < kNumBatchPreViews && s == GPS_OK; ++i) {
+        b->pre_views = kBatchPreViews[i];
+        float ms = 0.0f;
+        s = time_batch(&ms);
+        out->pre_variant_views[i] = kBatchPreViews[i];
+        out->pre_variant_ms[i] = ms / repeats;
+        if (s == GPS_OK && ms / repeats < best) {
+            best = ms / repeats;
+            best_v = kBatchPreViews[i];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out->pre_variants = kNumBatchPreViews;
+    out->preprocess_views_per_workgroup = best_v;
+    out->use_graph = graph ? 1 : 0;
+    b->pre_views = best_v;
     b->use_graph = graph;
     drop_graph(b);
     return s;

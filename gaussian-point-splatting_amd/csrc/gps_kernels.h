@@ -153,11 +153,14 @@ struct BatchArgs {
     uint32_t grid;
     int32_t xcds;
     int32_t threads, ppt;    // splat variant
+    int32_t pre_views;       // views one preprocess workgroup evaluates (0: all)
     uint32_t* atomics;
     BatchBinArgs bin;
 };
 extern const int kBatchSplatVariants[][2];
 extern const int kNumBatchSplatVariants;
+extern const int kBatchPreViews[];
+extern const int kNumBatchPreViews;
 void launch_batch_clear(const BatchArgs& a, hipStream_t st);
 void launch_batch_cull(const BatchArgs& a, hipStream_t st);
 void launch_batch_preprocess(const BatchArgs& a, hipStream_t st);

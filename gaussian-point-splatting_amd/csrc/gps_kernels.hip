@@ -2702,7 +2702,8 @@ __global__ __launch_bounds__(256) void batch_cull_kernel(BatchView* __restrict__
 // stored, not accumulated.
 template <int FORMAT, int DEG>
 __global__ __launch_bounds__(256) void batch_preprocess_kernel(DevScene s, const BatchView* __restrict__ views,
-                                                               int nviews, uint32_t npad, uint32_t tiles_per_view,
+                                                               int nviews, int views_per_wg, uint32_t npad,
+                                                               uint32_t tiles_per_view,
                                                                uint32_t* __restrict__ weights,
                                                                SplatRecord* __restrict__ recs,
                                                                uint32_t* __restrict__ tile_sum,
@@ -2711,7 +2712,10 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(DevScene s, const
     __shared__ uint32_t s_sum[GPS_BATCH_MAX_VIEWS], s_cnt[GPS_BATCH_MAX_VIEWS];
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t tile = blockIdx.x;
-    for (int v = tid; v < nviews; v += 256) {
+    // grid row y evaluates views [v0, v1): the (view, tile) sums of a view
+    // come from one workgroup whatever the split
+    const int v0 = (int)blockIdx.y * views_per_wg, v1 = min(nviews, v0 + views_per_wg);
+    for (int v = v0 + tid; v < v1; v += 256) {
         s_sum[v] = 0u;
         s_cnt[v] = 0u;
     }
@@ -2719,7 +2723,7 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(DevScene s, const
     const HiZView no_hiz{};
     for (int i = 0; i < GPS_TILE / 256; ++i) {
         const uint32_t g = (tile << GPS_TILE_SHIFT) + (uint32_t)(i * 256 + tid);
-        for (int v = 0; v < nviews; ++v) {
+        for (int v = v0; v < v1; ++v) {
             const FrameConsts& f = views[v].f;
             uint32_t n = 0;
             if (g < s.N) {
@@ -2741,7 +2745,7 @@ __global__ __launch_bounds__(256) void batch_preprocess_kernel(DevScene s, const
         }
     }
     __syncthreads();
-    for (int v = tid; v < nviews; v += 256) {
+    for (int v = v0 + tid; v < v1; v += 256) {
         tile_sum[(uint64_t)v * tiles_per_view + tile] = s_sum[v];
         tile_cnt[(uint64_t)v * tiles_per_view + tile] = s_cnt[v];
     }
@@ -3115,9 +3119,10 @@ void launch_batch_cull(const BatchArgs& a, hipStream_t st) {
 }
 
 template <int FORMAT, int DEG> static void launch_batch_pre_t(const BatchArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((batch_preprocess_kernel<FORMAT, DEG>), dim3(a.tiles_per_view), dim3(256), 0, st, a.scene,
-                       a.views, a.nviews, a.npad, a.tiles_per_view, a.weights, a.recs, a.tile_sum, a.tile_cnt, a.cull,
-                       a.mask_stride);
+    const int vpw = a.pre_views > 0 && a.pre_views < a.nviews ? a.pre_views : a.nviews;
+    hipLaunchKernelGGL((batch_preprocess_kernel<FORMAT, DEG>), dim3(a.tiles_per_view, (a.nviews + vpw - 1) / vpw),
+                       dim3(256), 0, st, a.scene, a.views, a.nviews, vpw, a.npad, a.tiles_per_view, a.weights, a.recs,
+                       a.tile_sum, a.tile_cnt, a.cull, a.mask_stride);
 }
 
 void launch_batch_preprocess(const BatchArgs& a, hipStream_t st) {
@@ -3165,6 +3170,10 @@ template <int THREADS, int PPT> static void launch_batch_splat_t(const BatchArgs
 // Splat variants the batch autotuner chooses from: (threads, points per thread).
 const int kBatchSplatVariants[][2] = {{256, 4}, {256, 2}, {256, 8}, {128, 4}, {512, 2}, {512, 4}};
 const int kNumBatchSplatVariants = 6;
+// Preprocess splits the batch autotuner chooses from: views per workgroup
+// (0: all views of the batch in one workgroup per tile).
+const int kBatchPreViews[] = {0, 16, 4};
+const int kNumBatchPreViews = 3;
 
 bool launch_batch_splat(const BatchArgs& a, hipStream_t st) {
     const int t = a.threads, q = a.ppt;

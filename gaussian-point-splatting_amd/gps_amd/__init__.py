@@ -847,7 +847,9 @@ class BatchSettings(C.Structure):
 class BatchTuning(C.Structure):
     _fields_ = [("splat_threads", C.c_int32), ("splat_points_per_thread", C.c_int32), ("use_graph", C.c_int32),
                 ("variants", C.c_int32), ("variant_threads", C.c_int32 * 8),
-                ("variant_points_per_thread", C.c_int32 * 8), ("variant_ms", C.c_float * 8)]
+                ("variant_points_per_thread", C.c_int32 * 8), ("variant_ms", C.c_float * 8),
+                ("preprocess_views_per_workgroup", C.c_int32), ("pre_variants", C.c_int32),
+                ("pre_variant_views", C.c_int32 * 4), ("pre_variant_ms", C.c_float * 4)]
 
 
 _sig("gps_batch_create", C.c_int32, C.POINTER(BatchSettings), _vp, C.POINTER(_vp))
@@ -892,12 +894,18 @@ class Batch:
         check(_lib.gps_batch_autotune(self.handle, n, fa, ca, da, stream.handle if stream else None, repeats,
                                       C.byref(t)), "batch autotune")
         return {"splat_threads": t.splat_threads, "splat_points_per_thread": t.splat_points_per_thread,
+                "preprocess_views_per_workgroup": t.preprocess_views_per_workgroup,
                 "variants": [{"threads": t.variant_threads[i], "points_per_thread": t.variant_points_per_thread[i],
-                              "ms": round(float(t.variant_ms[i]), 4)} for i in range(t.variants)]}
+                              "ms": round(float(t.variant_ms[i]), 4)} for i in range(t.variants)],
+                "preprocess_variants": [{"views_per_workgroup": t.pre_variant_views[i],
+                                         "ms": round(float(t.pre_variant_ms[i]), 4)} for i in range(t.pre_variants)]}
 
-    def set_tuning(self, threads: int, points_per_thread: int, graph: bool) -> None:
+    def set_tuning(self, threads: int, points_per_thread: int, graph: bool, views_per_workgroup: int = 0) -> None:
+        """Splat variant (threads x points per thread), graph replay, and the
+        preprocess split (views per workgroup, 0 = all)."""
         t = BatchTuning()
         t.splat_threads, t.splat_points_per_thread, t.use_graph = threads, points_per_thread, int(graph)
+        t.preprocess_views_per_workgroup = views_per_workgroup
         check(_lib.gps_batch_set_tuning(self.handle, C.byref(t)), "batch tuning")
 
     def set_splat_mode(self, mode: int) -> None:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GPS_ABI_VERSION 2
+#define GPS_ABI_VERSION 3
 
 typedef enum gps_status {
     GPS_OK = 0,
@@ -539,12 +539,19 @@ typedef struct gps_batch_tuning {
     int32_t variant_threads[8];
     int32_t variant_points_per_thread[8];
     float variant_ms[8];                 /* batch time of each variant */
+    int32_t preprocess_views_per_workgroup; /* views one preprocess workgroup evaluates (grid rows split
+                                               the batch's views; 0: all of them) */
+    int32_t pre_variants;                /* filled by gps_batch_autotune */
+    int32_t pre_variant_views[4];
+    float pre_variant_ms[4];             /* batch time of each split, with the chosen splat variant */
 } gps_batch_tuning;
 gps_status gps_batch_create(const gps_batch_settings* settings, gps_scene* scene, gps_batch** out);
 gps_status gps_batch_render(gps_batch* batch, int32_t num_views, const gps_frame_params* views,
                             float* const* d_colors, float* const* d_depths, void* stream);
-/* Times every splat variant on this batch (`repeats` batches each), keeps
- * the fastest and reports all; the graph setting is kept. */
+/* Times every splat variant on this batch (`repeats` batches each), then
+ * every preprocess split (views per workgroup 0 = all, 16, 4) with the
+ * fastest splat variant, keeps the fastest and reports all; the graph
+ * setting is kept.  Outputs are the same bits in every variant. */
 gps_status gps_batch_autotune(gps_batch* batch, int32_t num_views, const gps_frame_params* views,
                               float* const* d_colors, float* const* d_depths, void* stream, int32_t repeats,
                               gps_batch_tuning* out);

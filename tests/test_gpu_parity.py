@@ -505,6 +505,13 @@ def test_batch_views_identical_to_single_renders(gpu):
         batch.set_tuning(t, q, False)
         batch.render(frames, cols, deps, stream)
         check_batch(frames, cols, deps, ref, f"eager {t}x{q}")
+    # preprocess splits: views per workgroup (grid rows over the views; 5
+    # leaves a last row of one view, 16 covers all six in one row)
+    for vpw in (1, 4, 5, 16):
+        flush(f"batch eager, preprocess views per workgroup {vpw}")
+        batch.set_tuning(256, 4, False, vpw)
+        batch.render(frames, cols, deps, stream)
+        check_batch(frames, cols, deps, ref, f"eager vpw {vpw}")
     # oracle for three views at their exact sizes (view 4: biased)
     for v in (0, 3, 4):
         f = frames[v]
@@ -515,7 +522,9 @@ def test_batch_views_identical_to_single_renders(gpu):
         assert_same(f"view {v} vs oracle", ref[v][0], c)
     tuned = batch.autotune(frames, cols, deps, stream, repeats=1)
     flush(f"autotune: {tuned}")
-    batch.set_tuning(tuned["splat_threads"], tuned["splat_points_per_thread"], True)
+    assert len(tuned["preprocess_variants"]) == 3
+    batch.set_tuning(tuned["splat_threads"], tuned["splat_points_per_thread"], True,
+                     tuned["preprocess_views_per_workgroup"])
     for k in range(3):
         fr = W.batch_views(bw, k + 1)
         fr[1].enable_hierarchical_culling = 0

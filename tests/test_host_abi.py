@@ -32,7 +32,7 @@ def test_library_exports_every_declared_function():
 
 
 def test_abi_version_and_status_strings():
-    assert G.lib.gps_abi_version() == 2
+    assert G.lib.gps_abi_version() == 3
     for s in range(7):
         assert G.lib.gps_status_str(s)
 
