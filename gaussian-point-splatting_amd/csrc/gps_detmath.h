@@ -124,6 +124,24 @@ GPS_HD float det_expf(float x) {
 }
 
 // ---------------------------------------------------------------- log
+// f / (2 + f), IEEE-rounded, for the f of det_logf_normal (m - 1 with m a
+// float in (sqrt(2)/2, sqrt(2)]: 2^23 values).  The device takes the
+// hardware reciprocal and one fma correction of the quotient (q0 = f * y,
+// the remainder f - q0 * d exact in one fma, q0 + r * y); the GPU test
+// test_device_log_quotient_exhaustive compares it with the division for
+// every one of those 2^23 inputs.  The host divides.
+GPS_HD float det_log_quotient(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float d = 2.0f + f;
+    const float y = __builtin_amdgcn_rcpf(d);
+    const float q0 = f * y;
+    const float r = __builtin_fmaf(-q0, d, f);
+    return __builtin_fmaf(r, y, q0);
+#else
+    return f / (2.0f + f);
+#endif
+}
+
 // det_logf for a positive normal finite x (the special cases of det_logf
 // cannot occur); same arithmetic, so the same bits for such x.
 GPS_HD float det_logf_normal(float x, int32_t e = 0) {
@@ -135,7 +153,7 @@ GPS_HD float det_logf_normal(float x, int32_t e = 0) {
         e += 1;
     }
     const float f = m - 1.0f;                                // exact
-    const float s = f / (2.0f + f);
+    const float s = det_log_quotient(f);
     const float z = s * s;
     float p = 0x1.745d18p-4f;                                // 1/11
     p = det_fma(p, z, 0x1.c71c72p-4f);                       // 1/9

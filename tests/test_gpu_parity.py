@@ -37,6 +37,25 @@ def test_device_math_bitexact(gpu, kind, lo, hi):
     assert_same(f"math kind {kind}", dev, host)
 
 
+def test_device_log_quotient_exhaustive(gpu):
+    """The log's quotient f / (2 + f) on the device (reciprocal and one fma
+    correction) equals the IEEE-rounded division for every input it gets:
+    f = m - 1, m a float in (sqrt(2)/2, sqrt(2)] from a mantissa (halved above
+    sqrt(2)), i.e. all 2^23 mantissas."""
+    k = np.arange(1 << 23, dtype=np.uint32)
+    m = ((k | np.uint32(0x3F800000)).view(np.float32)).copy()
+    big = m > np.float32(1.41421354)  # 0x1.6a09e6p+0
+    m[big] = m[big] * np.float32(0.5)
+    f = (m - np.float32(1.0)).astype(np.float32)
+    host = (f / (np.float32(2.0) + f)).astype(np.float32)
+    d_in = gpu.DeviceArray(f.shape, np.float32)
+    d_out = gpu.DeviceArray(f.shape, np.float32)
+    d_in.upload(f)
+    flush("log quotient: 2^23 inputs")
+    gpu.check(gpu.lib.gps_selftest_math(12, d_in.ptr, d_out.ptr, f.size, None), "selftest")
+    assert_same("log quotient", d_out.download(), host)
+
+
 def test_device_rng_bitexact(gpu):
     import orc
     n, draws, frame = 4096, 6, 4 * 7 + 1
