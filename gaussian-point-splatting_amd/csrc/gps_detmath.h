@@ -51,6 +51,27 @@ GPS_HD float u2f_bits(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 GPS_HD float det_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 GPS_HD float det_sqrt(float x) { return __builtin_sqrtf(x); }
+// det_sqrt for x = 0 or 2^-96 <= x < 2^126 (finite): the hardware square
+// root and the +-1-ulp fma correction of the IEEE-rounded sequence, without
+// its scaling of tiny inputs and its special-case selects, which change
+// nothing in that range.  The GPU test test_device_sqrt_range_exhaustive
+// compares it with det_sqrt for every float in [1, 4) (every mantissa at
+// both exponent parities; the rounded root commutes with exact scaling by
+// 4) and for a sample of other exponents.  The host takes det_sqrt.
+GPS_HD float det_sqrt_normal(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y = __builtin_amdgcn_sqrtf(x);
+    const float ym = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, y) - 1u);
+    const float yp = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, y) + 1u);
+    const float rm = __builtin_fmaf(-ym, y, x);
+    const float rp = __builtin_fmaf(-yp, y, x);
+    float r = rm <= 0.0f ? ym : y;
+    r = rp > 0.0f ? yp : r;
+    return x == 0.0f ? 0.0f : r;
+#else
+    return det_sqrt(x);
+#endif
+}
 GPS_HD float det_floor(float x) { return __builtin_floorf(x); }
 GPS_HD float det_ceil(float x) { return __builtin_ceilf(x); }
 GPS_HD float det_rint(float x) { return __builtin_rintf(x); }

@@ -141,7 +141,8 @@ GPS_HD float inv_dilog(float x) {
 GPS_HD f32x2 corrected_box_muller_pre(float u1, float u2, float inv_alpha, float dilog_alpha) {
     float a = inv_alpha * inv_dilog((1.0f - u1) * dilog_alpha);
     a = glm_clamp(a, 1.e-37f, 1.0f);  // normal: det_logf needs no special cases; NaN impossible
-    const float R = det_sqrt(det_fmax(-2.0f * det_logf_normal(a), 0.0f));
+    // -2 log a is 0 (a = 1) or at least ~1.2e-7 (a <= 1 - 2^-24), at most ~170
+    const float R = det_sqrt_normal(det_fmax(-2.0f * det_logf_normal(a), 0.0f));
     const float theta = GPS_TWO_PI_F * u2;  // u2 in [0, 1]: in det_sincosf's range
     float s, c;
     det_sincosf_inrange(theta, &s, &c);

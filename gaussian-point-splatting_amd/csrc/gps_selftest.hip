@@ -26,6 +26,7 @@ __global__ void selftest_math_kernel(int kind, const float* __restrict__ in, flo
         case 10: r = corrected_box_muller(x, in[(i + 1) % n], 0.7f).y; break;
         case 11: r = fp8_e4m3_to_float_hw((uint32_t)(int)x & 0xFFu, 0); break;  // FP8 code x
         case 12: r = det_log_quotient(x); break;
+        case 13: r = det_sqrt_normal(x); break;
     }
     out[i] = r;
 }
@@ -93,7 +94,7 @@ extern "C" gps_status gps_selftest_copy(void* d_dst, const void* d_src, uint64_t
 }
 
 extern "C" gps_status gps_selftest_math(int32_t kind, const float* d_in, float* d_out, uint64_t n, void* stream) {
-    if (!d_in || !d_out || kind < 0 || kind > 12) return GPS_ERR_INVALID_ARG;
+    if (!d_in || !d_out || kind < 0 || kind > 13) return GPS_ERR_INVALID_ARG;
     if (n == 0) return GPS_OK;
     hipLaunchKernelGGL(gps::selftest_math_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, kind, d_in, d_out, n);

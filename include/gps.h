@@ -635,7 +635,8 @@ void gps_server_destroy(gps_server* server);
  * kind: 0 exp, 1 log, 2 sin, 3 cos, 4 sqrt, 5 reciprocal, 6 dilog,
  * 7 inv_dilog, 8 float->int round-to-nearest-even (saturating),
  * 9 Box-Muller x of (in[i], in[i+1]), 10 corrected Box-Muller y (alpha 0.7),
- * 11 FP8 E4M3 decode of code in[i], 12 the log's quotient x / (2 + x). */
+ * 11 FP8 E4M3 decode of code in[i], 12 the log's quotient x / (2 + x),
+ * 13 the square root of the sampling transform (x = 0 or normal). */
 gps_status gps_selftest_math(int32_t kind, const float* d_in, float* d_out, uint64_t n, void* stream);
 /* Per item i: make_seed(i, frame) then `draws` pcg2d states (2+2*draws u32). */
 gps_status gps_selftest_rng(uint32_t n, uint32_t frame, int32_t draws, uint32_t* d_out, void* stream);

@@ -56,6 +56,28 @@ def test_device_log_quotient_exhaustive(gpu):
     assert_same("log quotient", d_out.download(), host)
 
 
+def test_device_sqrt_range_exhaustive(gpu):
+    """The sampling transform's square root (hardware root and the +-1-ulp
+    fma correction, no tiny-input scaling) equals det_sqrt's IEEE-rounded
+    root for every float in [1, 4) -- every mantissa at both exponent
+    parities -- and for 0 and a sample of floats from 2^-96 to 2^100."""
+    k = np.arange(1 << 24, dtype=np.uint32)
+    x = ((k + np.uint32(0x3F800000)).view(np.float32)).copy()  # [1, 4)
+    rng = np.random.default_rng(13)
+    sample = np.exp(rng.uniform(np.log(2.0 ** -96), np.log(2.0 ** 100), 1 << 20)).astype(np.float32)
+    x = np.concatenate([x, sample, np.array([0.0, 2.0 ** -96, 170.5], np.float32)])
+    d_in = gpu.DeviceArray(x.shape, np.float32)
+    d_fast = gpu.DeviceArray(x.shape, np.float32)
+    d_ref = gpu.DeviceArray(x.shape, np.float32)
+    d_in.upload(x)
+    flush("sqrt: 2^24 + 2^20 inputs")
+    gpu.check(gpu.lib.gps_selftest_math(13, d_in.ptr, d_fast.ptr, x.size, None), "selftest")
+    gpu.check(gpu.lib.gps_selftest_math(4, d_in.ptr, d_ref.ptr, x.size, None), "selftest")
+    ref = d_ref.download()
+    assert_same("sqrt vs device det_sqrt", d_fast.download(), ref)
+    assert_same("device det_sqrt vs numpy", ref, np.sqrt(x))
+
+
 def test_device_rng_bitexact(gpu):
     import orc
     n, draws, frame = 4096, 6, 4 * 7 + 1
