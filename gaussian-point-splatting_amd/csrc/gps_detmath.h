@@ -104,6 +104,16 @@ GPS_HD int32_t det_f2i_rz(float x) {
     return (x >= 2147483648.0f) ? 2147483647 : r;
 }
 GPS_HD int32_t det_f2i_rn(float x) { return det_f2i_rz(det_rint(x)); }
+// Whether det_f2i_rz(v) lies in [lo, hi) for integral-valued (floored) v and
+// 0 <= lo < hi <= 65536, and that value in *p: NaN converts to 0, and
+// infinities and values beyond the int range convert to values outside such
+// bounds, so the test is two float compares of v with NaN read as 0.
+GPS_HD bool det_pixel_in(float v, int32_t lo, int32_t hi, int32_t* p) {
+    const float s = (v != v) ? 0.0f : v;
+    const bool in = s >= (float)lo && s < (float)hi;
+    *p = in ? (int32_t)s : 0;
+    return in;
+}
 GPS_HD uint32_t det_f2u_rz(float x) {
     if (!(x > 0.0f)) return 0u;  // NaN, negative, zero
     if (x >= 4294967296.0f) return 0xFFFFFFFFu;

@@ -1229,9 +1229,8 @@ __device__ __forceinline__ uint32_t splat_point(const SplatRecord& rec, uint32_t
                                   : box_muller(u.x, u.y);
         const float fx = det_floor(rec.pmx + rec.c0 * xy.x);
         const float fy = det_floor((rec.pmy + rec.c1 * xy.x) + rec.c2 * xy.y);
-        const int x = det_f2i_rz(fx);
-        const int y = det_f2i_rz(fy);
-        if (x < bminx || x >= bmaxx || y < bminy || y >= bmaxy) continue;
+        int x, y;  // det_f2i_rz(fx), det_f2i_rz(fy) inside the bounds
+        if (!det_pixel_in(fx, bminx, bmaxx, &x) || !det_pixel_in(fy, bminy, bmaxy, &y)) continue;
         const float dx = rec.pmx - fx, dy = rec.pmy - fy;
         const float sigma = -0.5f * (rec.conx * dx * dx + rec.conz * dy * dy) - rec.cony * dx * dy;
         if (!alpha_passes(rec, sigma)) continue;
@@ -1984,9 +1983,8 @@ __device__ __forceinline__ void splat_point_region(const SplatRecord& rec, uint3
                                     : box_muller(u.x, u.y);
         const float fx = det_floor(rec.pmx + rec.c0 * xy.x);
         const float fy = det_floor((rec.pmy + rec.c1 * xy.x) + rec.c2 * xy.y);
-        const int x = det_f2i_rz(fx);
-        const int y = det_f2i_rz(fy);
-        if (x < bminx || x >= bmaxx || y < bminy || y >= bmaxy) continue;
+        int x, y;  // det_f2i_rz(fx), det_f2i_rz(fy) inside the bounds
+        if (!det_pixel_in(fx, bminx, bmaxx, &x) || !det_pixel_in(fy, bminy, bmaxy, &y)) continue;
         const int lx = x - rx0, ly = y - ry0;
         if ((unsigned)lx >= (unsigned)rw || (unsigned)ly >= (unsigned)rh) continue;  // another region's sample
         const float dx = rec.pmx - fx, dy = rec.pmy - fy;
