@@ -104,6 +104,19 @@ GPS_HD int32_t det_f2i_rz(float x) {
     return (x >= 2147483648.0f) ? 2147483647 : r;
 }
 GPS_HD int32_t det_f2i_rn(float x) { return det_f2i_rz(det_rint(x)); }
+// det_f2i_rz(v) where only its order against lo and hi matters (the caller
+// clamps it into [lo, hi] or rejects beyond them; lo <= 0 <= hi, small
+// integers): NaN reads as 0 as there, the float is clamped, then converted.
+GPS_HD int32_t det_f2i_clamped(float v, float lo, float hi) {
+    const float s = (v != v) ? 0.0f : glm_min(glm_max(v, lo), hi);
+    return (int32_t)s;
+}
+// (float)det_f2i_rz(v) for integral-valued v >= 0 or NaN: v below 2^31, the
+// saturated 2^31 from there, 0 for NaN.
+GPS_HD float det_f2i_rz_float(float v) {
+    return (v != v) ? 0.0f : glm_min(v, 2147483648.0f);
+}
+
 // Whether det_f2i_rz(v) lies in [lo, hi) for integral-valued (floored) v and
 // 0 <= lo < hi <= 65536, and that value in *p: NaN converts to 0, and
 // infinities and values beyond the int range convert to values outside such

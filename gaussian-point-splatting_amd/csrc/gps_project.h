@@ -162,16 +162,20 @@ GPS_HD bool project_footprint(const FrameConsts& f, const Center& ce, float qw, 
     o.pmy = pixel_mean(o.py, f.up_h);
 
     // compute2DGaussianBounds3DGS (linalg.cuh:251-296)
-    const int32_t ex = det_f2i_rz(det_ceil(3.0f * det_sqrt(o.ca)));
-    const int32_t ey = det_f2i_rz(det_ceil(3.0f * det_sqrt(o.cc)));
-    const float mnx = o.pmx - (float)ex, mny = o.pmy - (float)ey;
-    const float mxx = o.pmx + (float)ex, mxy = o.pmy + (float)ey;
-    int32_t t0x = det_f2i_rz(det_floor(mnx / 16.0f));
-    int32_t t0y = det_f2i_rz(det_floor(mny / 16.0f));
-    int32_t t1x = det_f2i_rz(det_ceil(mxx / 16.0f));
-    int32_t t1y = det_f2i_rz(det_ceil(mxy / 16.0f));
+    // (float)(int) of the ceilings (the 3-sigma half extents, >= 0 or NaN)
+    const float ex = det_f2i_rz_float(det_ceil(3.0f * det_sqrt(o.ca)));
+    const float ey = det_f2i_rz_float(det_ceil(3.0f * det_sqrt(o.cc)));
+    const float mnx = o.pmx - ex, mny = o.pmy - ey;
+    const float mxx = o.pmx + ex, mxy = o.pmy + ey;
     const int32_t tiles_x = (f.up_w + 15) / 16;
     const int32_t tiles_y = (f.up_h + 15) / 16;
+    // the tile bounds only meet the tests and clamps below: converted after
+    // clamping into [-1, tiles] / [0, tiles], with the same outcome as the
+    // saturating conversion
+    int32_t t0x = det_f2i_clamped(det_floor(mnx / 16.0f), -1.0f, (float)tiles_x);
+    int32_t t0y = det_f2i_clamped(det_floor(mny / 16.0f), -1.0f, (float)tiles_y);
+    int32_t t1x = det_f2i_clamped(det_ceil(mxx / 16.0f), 0.0f, (float)tiles_x);
+    int32_t t1y = det_f2i_clamped(det_ceil(mxy / 16.0f), 0.0f, (float)tiles_y);
     if (t1x <= 0 || t1y <= 0 || t0x >= tiles_x || t0y >= tiles_y) return false;
     t0x = t0x < 0 ? 0 : t0x;
     t0y = t0y < 0 ? 0 : t0y;
