@@ -51,13 +51,14 @@ GPS_HD float u2f_bits(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 GPS_HD float det_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 GPS_HD float det_sqrt(float x) { return __builtin_sqrtf(x); }
-// det_sqrt for x = 0 or 2^-96 <= x < 2^126 (finite): the hardware square
-// root and the +-1-ulp fma correction of the IEEE-rounded sequence, without
-// its scaling of tiny inputs and its special-case selects, which change
-// nothing in that range.  The GPU test test_device_sqrt_range_exhaustive
-// compares it with det_sqrt for every float in [1, 4) (every mantissa at
-// both exponent parities; the rounded root commutes with exact scaling by
-// 4) and for a sample of other exponents.  The host takes det_sqrt.
+// det_sqrt for x = 0, x >= 2^-96 (up to infinity), negative x or NaN (both
+// NaN): the hardware square root and the +-1-ulp fma correction of the
+// IEEE-rounded sequence, without its scaling of tiny inputs and its
+// special-case selects, which change nothing there.  The GPU test
+// test_device_sqrt_range_exhaustive compares it with det_sqrt for every
+// float in [1, 4) (every mantissa at both exponent parities; the rounded
+// root commutes with exact scaling by 4), a sample of 2^-96 .. FLT_MAX and
+// the special values.  The host takes det_sqrt.
 GPS_HD float det_sqrt_normal(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const float y = __builtin_amdgcn_sqrtf(x);

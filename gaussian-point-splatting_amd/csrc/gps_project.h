@@ -162,9 +162,11 @@ GPS_HD bool project_footprint(const FrameConsts& f, const Center& ce, float qw, 
     o.pmy = pixel_mean(o.py, f.up_h);
 
     // compute2DGaussianBounds3DGS (linalg.cuh:251-296)
-    // (float)(int) of the ceilings (the 3-sigma half extents, >= 0 or NaN)
-    const float ex = det_f2i_rz_float(det_ceil(3.0f * det_sqrt(o.ca)));
-    const float ey = det_f2i_rz_float(det_ceil(3.0f * det_sqrt(o.cc)));
+    // (float)(int) of the ceilings (the 3-sigma half extents, >= 0 or NaN);
+    // ca and cc carry the dilation (>= 0.3), so the root needs no scaling of
+    // tiny inputs
+    const float ex = det_f2i_rz_float(det_ceil(3.0f * det_sqrt_normal(o.ca)));
+    const float ey = det_f2i_rz_float(det_ceil(3.0f * det_sqrt_normal(o.cc)));
     const float mnx = o.pmx - ex, mny = o.pmy - ey;
     const float mxx = o.pmx + ex, mxy = o.pmy + ey;
     const int32_t tiles_x = (f.up_w + 15) / 16;

@@ -60,12 +60,15 @@ def test_device_sqrt_range_exhaustive(gpu):
     """The sampling transform's square root (hardware root and the +-1-ulp
     fma correction, no tiny-input scaling) equals det_sqrt's IEEE-rounded
     root for every float in [1, 4) -- every mantissa at both exponent
-    parities -- and for 0 and a sample of floats from 2^-96 to 2^100."""
+    parities -- and for 0, a sample of floats from 2^-96 to 2^128, the 4096
+    largest floats, infinity, and NaN for negative and NaN inputs."""
     k = np.arange(1 << 24, dtype=np.uint32)
     x = ((k + np.uint32(0x3F800000)).view(np.float32)).copy()  # [1, 4)
     rng = np.random.default_rng(13)
-    sample = np.exp(rng.uniform(np.log(2.0 ** -96), np.log(2.0 ** 100), 1 << 20)).astype(np.float32)
-    x = np.concatenate([x, sample, np.array([0.0, 2.0 ** -96, 170.5], np.float32)])
+    sample = np.exp(rng.uniform(np.log(2.0 ** -96), np.log(2.0 ** 127.99), 1 << 20)).astype(np.float32)
+    big = (np.float32(3.4028235e38).view(np.uint32) - np.arange(4096, dtype=np.uint32)).view(np.float32)
+    special = np.array([0.0, 2.0 ** -96, 170.5, np.inf, -1.0, -np.inf, np.nan], np.float32)
+    x = np.concatenate([x, sample, big, special])
     d_in = gpu.DeviceArray(x.shape, np.float32)
     d_fast = gpu.DeviceArray(x.shape, np.float32)
     d_ref = gpu.DeviceArray(x.shape, np.float32)
@@ -74,8 +77,13 @@ def test_device_sqrt_range_exhaustive(gpu):
     gpu.check(gpu.lib.gps_selftest_math(13, d_in.ptr, d_fast.ptr, x.size, None), "selftest")
     gpu.check(gpu.lib.gps_selftest_math(4, d_in.ptr, d_ref.ptr, x.size, None), "selftest")
     ref = d_ref.download()
-    assert_same("sqrt vs device det_sqrt", d_fast.download(), ref)
-    assert_same("device det_sqrt vs numpy", ref, np.sqrt(x))
+    fast = d_fast.download()
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(fast), nan), "NaN where det_sqrt is not, or the reverse"
+    assert nan.sum() == 3  # -1, -inf, NaN
+    assert_same("sqrt vs device det_sqrt", fast[~nan], ref[~nan])
+    ok = np.isfinite(x) & (x >= 0)
+    assert_same("device det_sqrt vs numpy", ref[ok], np.sqrt(x[ok]))
 
 
 def test_device_rng_bitexact(gpu):
