@@ -118,9 +118,8 @@ GPS_HD float det_div_li2max(float x) {
     return det_fma(r, z, q0);
 }
 
-// Polynomial fit of the inverse of Li2 (random.cuh:110-127).
-GPS_HD float inv_dilog(float x) {
-    const float t = det_fmin(det_div_li2max(x), 1.0f);
+// The polynomial of inv_dilog at t = min(x / Li2(1), 1).
+GPS_HD float inv_dilog_poly(float t) {
     float y = -1.27463503e+01f;
     y = det_fma(y, t, 5.88993459e+01f);
     y = det_fma(y, t, -1.16025780e+02f);
@@ -134,13 +133,24 @@ GPS_HD float inv_dilog(float x) {
     y = det_fma(y, t, -2.82836687e-05f);
     return y;
 }
+// Polynomial fit of the inverse of Li2 (random.cuh:110-127).
+GPS_HD float inv_dilog(float x) { return inv_dilog_poly(det_fmin(det_div_li2max(x), 1.0f)); }
 
 // Opacity-corrected radial sampling (random.cuh:132-141).  `inv_alpha` is
 // 1/max(alpha, 1e-37) and `dilog_alpha` is dilog(alpha); both depend only on
 // the Gaussian, so callers may compute them once per Gaussian.
 GPS_HD f32x2 corrected_box_muller_pre(float u1, float u2, float inv_alpha, float dilog_alpha) {
-    float a = inv_alpha * inv_dilog((1.0f - u1) * dilog_alpha);
-    a = glm_clamp(a, 1.e-37f, 1.0f);  // normal: det_logf needs no special cases; NaN impossible
+    // (1 - u1) * dilog(alpha) is finite and >= 0, so det_fmin's NaN cases
+    // cannot arise and a compare-select gives its result; a is finite
+    // (inv_alpha <= 1e37), so the clamp is the hardware median of three
+    const float q = det_div_li2max((1.0f - u1) * dilog_alpha);
+    float a = inv_alpha * inv_dilog_poly(1.0f < q ? 1.0f : q);
+#if defined(__HIP_DEVICE_COMPILE__)
+    a = __builtin_amdgcn_fmed3f(a, 1.e-37f, 1.0f);
+#else
+    a = glm_clamp(a, 1.e-37f, 1.0f);
+#endif
+    // a is normal: det_logf needs no special cases
     // -2 log a is 0 (a = 1) or at least ~1.2e-7 (a <= 1 - 2^-24), at most ~170
     const float R = det_sqrt_normal(det_fmax(-2.0f * det_logf_normal(a), 0.0f));
     const float theta = GPS_TWO_PI_F * u2;  // u2 in [0, 1]: in det_sincosf's range
