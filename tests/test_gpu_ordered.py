@@ -114,3 +114,21 @@ def test_ordered_splat_equals_oracle(gpu, case):
         for key in ("valid_gaussians", "num_points", "samples"):
             assert rg.stats[key] == ro.stats[key], key
     assert total_skipped > 0
+
+
+def test_switches_reject_unknown_values(gpu):
+    """An unknown ordered-splat mode and a preprocess split beyond the batch
+    limit (256 views) are refused with an argument error."""
+    arrays = slabs(0x5EED0971, 2_000, 1)
+    scene = gpu.Scene(arrays, gpu.FORMAT_FP32)
+    p = gpu.Pass(gpu.RenderSettings(64, 48, 1, 1), scene)
+    for mode in (gpu.Pass.ORDERED_AUTO, gpu.Pass.ORDERED_OFF, gpu.Pass.ORDERED_ALWAYS):
+        p.set_ordered_splat(mode)
+    with pytest.raises(Exception):
+        p.set_ordered_splat(3)
+    batch = gpu.Batch(scene, 2, 64, 48, max_spp=1, max_supersampling=1)
+    batch.set_tuning(256, 4, False, 16)
+    with pytest.raises(Exception):
+        batch.set_tuning(256, 4, False, 257)
+    with pytest.raises(Exception):
+        batch.set_tuning(256, 3, False, 0)  # not a splat variant

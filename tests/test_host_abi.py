@@ -63,6 +63,13 @@ def test_entry_points_reject_bad_arguments():
     assert G.lib.gps_bvh_order(None, None, None, 0, None) == 0  # nothing to order
     assert G.lib.gps_selftest_atomics(None, 16, 1, 1, None) == 1
     assert G.lib.gps_selftest_atomics(C.c_void_p(8), 0, 1, 1, None) == 1
+    assert G.lib.gps_selftest_math(13, None, None, 4, None) == 1  # null buffers
+    assert G.lib.gps_selftest_math(14, C.c_void_p(8), C.c_void_p(8), 4, None) == 1  # unknown kind
+    # per-pass and per-batch switches: a null handle is refused
+    assert G.lib.gps_pass_set_splat_mode(None, 0) == 1
+    assert G.lib.gps_pass_set_ordered_splat(None, 0) == 1
+    assert G.lib.gps_pass_set_group_culling(None, 1) == 1
+    assert G.lib.gps_batch_set_tuning(None, None) == 1
 
 
 def lookat_ref(eye, center, up):
