@@ -124,11 +124,11 @@ def test_switches_reject_unknown_values(gpu):
     p = gpu.Pass(gpu.RenderSettings(64, 48, 1, 1), scene)
     for mode in (gpu.Pass.ORDERED_AUTO, gpu.Pass.ORDERED_OFF, gpu.Pass.ORDERED_ALWAYS):
         p.set_ordered_splat(mode)
-    with pytest.raises(Exception):
+    with pytest.raises(gpu.GpsError):
         p.set_ordered_splat(3)
     batch = gpu.Batch(scene, 2, 64, 48, max_spp=1, max_supersampling=1)
     batch.set_tuning(256, 4, False, 16)
-    with pytest.raises(Exception):
+    with pytest.raises(gpu.GpsError):
         batch.set_tuning(256, 4, False, 257)
-    with pytest.raises(Exception):
+    with pytest.raises(gpu.GpsError):
         batch.set_tuning(256, 3, False, 0)  # not a splat variant
